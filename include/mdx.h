@@ -1,0 +1,165 @@
+/*
+ * mdx.h — C-ABI of libmdx.so, the MI355X (gfx950) force-and-integrate step behind mdpy's
+ * Constraint / Integrator / Simulation API.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative MDX_ERR_* code; mdx_last_error()
+ *     returns a thread-local message for the last failure.
+ *   - Units are mdpy's internal units: A, Da, fs, e, K; energy Da*A^2/fs^2
+ *     (mdpy/unit/__init__.py:31-41).
+ *   - "d_" pointers are device pointers (HIP global memory) owned by the caller (PyTorch
+ *     tensors); the library only borrows them. "h_" pointers are host memory read during
+ *     the call. The library owns its scratch (cell tables, neighbour lists, PME grids,
+ *     fixed-point accumulators) inside the context.
+ *   - All device work is enqueued on the context's stream; nothing synchronises except
+ *     functions whose name ends in _fetch or that return host-visible results (documented).
+ *   - One context per host thread; contexts are independent.
+ *   - Positions and velocities are float64 N x 3 row-major (x0,y0,z0,x1,...).
+ *   - Orthorhombic boxes only (edge lengths); the reference's CUDA kernels make the same
+ *     restriction (mdpy/constraint/charmm_nonbonded_constraint.py:145-150).
+ */
+#ifndef MDX_H
+#define MDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDX_OK                 0
+#define MDX_ERR_INVALID       -1   /* bad argument or call order */
+#define MDX_ERR_HIP           -2   /* a HIP runtime call failed */
+#define MDX_ERR_OVERFLOW      -3   /* cell or neighbour-list capacity exceeded */
+#define MDX_ERR_PARTICLE_LOSS -4   /* an atom moved >= 2 box images (mdpy ParticleLossError) */
+#define MDX_ERR_NONFINITE     -5   /* NaN/Inf positions or forces detected on the device */
+#define MDX_ERR_CELL_CUTOFF   -6   /* cutoff too large for the box (mdpy CellListPoorDefinedError) */
+
+/* force terms (bit mask) */
+#define MDX_TERM_LJ            (1u << 0)  /* CHARMM 12-6 LJ, r <= rc (CharmmNonbondedConstraint) */
+#define MDX_TERM_COUL_DIRECT   (1u << 1)  /* PME direct space: q_i q_j erfc(beta r)/r, r <= rc   */
+#define MDX_TERM_COUL_RECIP    (1u << 2)  /* PME reciprocal + self + exclusion + background       */
+#define MDX_TERM_BOND          (1u << 3)
+#define MDX_TERM_ANGLE         (1u << 4)
+#define MDX_TERM_DIHEDRAL      (1u << 5)
+#define MDX_TERM_IMPROPER      (1u << 6)
+#define MDX_TERM_COUL_PLAIN    (1u << 7)  /* plain Coulomb, all pairs, minimum image, no cutoff  */
+#define MDX_NTERMS             8
+/* energy slot of the kinetic energy in mdx_energies_fetch output (after the MDX_NTERMS terms) */
+#define MDX_E_KINETIC          MDX_NTERMS
+
+/* neighbour-set modes */
+#define MDX_PAIRS_EXACT        0  /* every minimum-image pair with r <= rc */
+#define MDX_PAIRS_REFERENCE    1  /* plus mdpy's cell-stencil restriction (cell_list.py:74-116) */
+
+/* integrators */
+#define MDX_INT_VELOCITY_VERLET 0 /* textbook velocity Verlet (NVE)                           */
+#define MDX_INT_LANGEVIN        1 /* G-JF Langevin with mdpy's a, b, sigma                    */
+#define MDX_INT_VERLET_REF      2 /* verlet_integrator.py:20-50 as written                    */
+#define MDX_INT_LANGEVIN_REF    3 /* langevin_integrator.py:37-72 as written                  */
+
+typedef struct mdx_ctx mdx_ctx;
+
+/* ---- library / context ------------------------------------------------------------------ */
+const char* mdx_last_error(void);
+/* 1/(4 pi eps0) in internal units, the Coulomb factor every kernel uses (electrostatic_constraint.py:21,60). */
+double mdx_coulomb_constant(void);
+int mdx_version(void);
+/* Create a context on HIP device `device`, enqueuing on `hip_stream` (NULL = default stream). */
+int mdx_ctx_create(int device, void* hip_stream, mdx_ctx** out);
+int mdx_ctx_destroy(mdx_ctx* ctx);
+int mdx_ctx_set_stream(mdx_ctx* ctx, void* hip_stream);
+
+/* ---- system (State / Topology.join, mdpy/core/state.py:19-28, topology.py:59-80) --------- */
+/* Bind caller-owned device arrays: positions (N*3 f64), velocities (N*3 f64). `h_box` = 3 edges. */
+int mdx_system_set(mdx_ctx* ctx, int n_atoms, double* d_pos, double* d_vel, const double* h_box);
+int mdx_set_box(mdx_ctx* ctx, const double* h_box);
+/* Per-atom data from host arrays (copied): masses (N), charges (N, e), LJ (N*4: eps, sigma,
+ * eps14, sigma14 — CharmmNonbondedConstraint.bind_ensemble, charmm_nonbonded_constraint.py:48-62). */
+int mdx_set_masses(mdx_ctx* ctx, const double* h_mass);
+int mdx_set_charges(mdx_ctx* ctx, const double* h_charge);
+int mdx_set_lj(mdx_ctx* ctx, const double* h_lj);
+/* Exclusions (1-2 + 1-3, topology.py:122-167) and 1-4 partners (topology.py:188-200) as CSR
+ * (ptr has N+1 entries). */
+int mdx_set_exclusions(mdx_ctx* ctx, const int* h_ptr, const int* h_idx);
+int mdx_set_scaling(mdx_ctx* ctx, const int* h_ptr, const int* h_idx);
+/* Bonded terms (bind_ensemble of charmm_{bond,angle,dihedral,improper}_constraint.py):
+ * bonds nb x 2 ids + nb x 2 (k, r0); angles na x 3 + na x 4 (k, theta0, k_ub, u0);
+ * dihedral terms nd x 4 + nd x 3 (k, n, delta); impropers ni x 4 + ni x 2 (k, psi0).
+ * dihedral_refcompat != 0 reproduces force_val = -k(1 - n sin(n phi - delta))
+ * (charmm_dihedral_constraint.py:81) instead of -dE/dphi. */
+int mdx_set_bonds(mdx_ctx* ctx, int nb, const int* h_idx, const double* h_par);
+int mdx_set_angles(mdx_ctx* ctx, int na, const int* h_idx, const double* h_par);
+int mdx_set_dihedrals(mdx_ctx* ctx, int nd, const int* h_idx, const double* h_par, int dihedral_refcompat);
+int mdx_set_impropers(mdx_ctx* ctx, int ni, const int* h_idx, const double* h_par);
+
+/* ---- State.set_positions: wrap + reference cell list (pbc.py:28-36, cell_list.py:74-116) ---- */
+/* Wrap d_pos in place with x + (-rint(x * inv_L)) * L; returns MDX_ERR_PARTICLE_LOSS (after a
+ * sync) if any |shift| >= 2. Synchronises. */
+int mdx_wrap_positions(mdx_ctx* ctx);
+/* Reference binning of the current positions with the cell-list cutoff `cell_cutoff`:
+ * n = max(3, floor((L + 2)/rc)), w = (L + 2)/n, idx = floor((x - min x)/w) (bit-exact float64).
+ * Writes d_cell_index (N*3 int32) and returns grid dims and the largest cell population in
+ * h_n[3], h_maxpc. Synchronises. */
+int mdx_celllist_reference(mdx_ctx* ctx, double cell_cutoff, int* d_cell_index, int* h_n, int* h_maxpc);
+/* Dense cell table [nx][ny][nz][maxpc], atoms ascending within a cell, -1 padded (cell_list.py:105-116),
+ * for the binning of the last mdx_celllist_reference call. */
+int mdx_celllist_table(mdx_ctx* ctx, int* d_table, int maxpc);
+
+/* ---- nonbonded (CharmmNonbondedConstraint, charmm_nonbonded_constraint.py:64-226) -------- */
+/* cutoff rc, list skin (0 = rebuild every evaluation), pair set (MDX_PAIRS_*), cell_cutoff for
+ * the reference stencil (the reference cell list's cutoff, 12 A by default). */
+int mdx_nonbonded_config(mdx_ctx* ctx, double cutoff, double skin, int pair_mode, double cell_cutoff);
+/* Enumerate the unique neighbour pairs (i<j, original ids) the configured pair set contains at
+ * the current positions, decided in float64. d_pairs holds capacity x 2 int32. Returns the pair
+ * count in *h_count (may exceed capacity: then only capacity pairs are written). Synchronises. */
+int mdx_neighbor_pairs(mdx_ctx* ctx, int* d_pairs, int64_t capacity, int64_t* h_count);
+
+/* ---- PME (new; no reference counterpart, mdpy/constraint/__init__.py:22) ------------------ */
+/* grid K (3 ints), B-spline order (4), Ewald beta (1/A). */
+int mdx_pme_config(mdx_ctx* ctx, const int* h_grid, int order, double beta);
+
+/* ---- force evaluation (Constraint.update / Ensemble.update, ensemble.py:53-61) ------------ */
+/* Evaluate the terms in `terms` at the current positions. If d_force_out != NULL, write the
+ * summed forces (N*3 f64). Energies accumulate on the device; read them with mdx_energies_fetch. */
+int mdx_compute_forces(mdx_ctx* ctx, unsigned terms, double* d_force_out);
+/* Copy the per-term energies of the last evaluation (+ kinetic energy at MDX_E_KINETIC) to
+ * h_out[MDX_NTERMS + 1]. Synchronises. */
+int mdx_energies_fetch(mdx_ctx* ctx, double* h_out);
+
+/* ---- integration (Integrator.integrate, integrator/*.py) --------------------------------- */
+int mdx_integrator_config(mdx_ctx* ctx, int kind, double dt, double gamma, double kT, uint64_t seed);
+/* Forget integrator history (Integrator.erase_cache, integrator.py:17-22). */
+int mdx_integrator_reset(mdx_ctx* ctx);
+/* Use caller-provided standard normals d_noise[nsteps][N][3] (f64) instead of the built-in
+ * Philox4x32-10 stream, step by step from the next step on (NULL switches back). */
+int mdx_set_external_noise(mdx_ctx* ctx, const double* d_noise, int nsteps);
+/* Advance `nsteps` steps with force terms `terms`. use_graph != 0 captures the step into a
+ * hipGraph and replays it. Positions/velocities in the bound arrays are updated (positions
+ * wrapped as State.set_positions does); per-step energies go to the energy history. */
+int mdx_integrate(mdx_ctx* ctx, unsigned terms, int nsteps, int use_graph);
+/* Evaluate potential energies every `every` steps during mdx_integrate (default 1; forces are
+ * evaluated every step). The energy history then holds rows for steps 0, every, 2*every, ... */
+int mdx_set_energy_interval(mdx_ctx* ctx, int every);
+/* Energies of the last mdx_integrate call, one row per energy step (steps 0, every, ...):
+ * h_out[k*(MDX_NTERMS+1) + t] for k < min(rows, h_cap); *h_count = rows. Synchronises. */
+int mdx_energy_history_fetch(mdx_ctx* ctx, double* h_out, int h_cap, int* h_count);
+/* Status flags raised on the device since the last call (bit 0 overflow, 1 particle loss,
+ * 2 non-finite); clears them. Synchronises. */
+int mdx_status_fetch(mdx_ctx* ctx, unsigned* h_flags);
+
+/* ---- instrumentation ------------------------------------------------------------------- */
+/* Number of neighbour-list rebuilds and evaluations since the context was created. */
+int mdx_stats_fetch(mdx_ctx* ctx, int64_t* h_rebuilds, int64_t* h_evals);
+/* Average duration (ms) of one launch of a hot kernel, timed with HIP events on the context's
+ * stream: which 0 = nonbonded kernel (current list, forces only), 1 = force evaluation of
+ * `terms`, 2 = PME reciprocal chain, 3 = one integrator step of `terms`. Synchronises. */
+int mdx_time_kernel(mdx_ctx* ctx, int which, unsigned terms, int reps, double* h_avg_ms);
+/* Candidate (list) pair slots and in-cutoff unique pairs of the last nonbonded evaluation
+ * (counts the list on the device; for roofline accounting; synchronises). */
+int mdx_pair_counts(mdx_ctx* ctx, int64_t* h_list_slots, int64_t* h_unique_in_cutoff);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDX_H */

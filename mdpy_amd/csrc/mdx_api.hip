@@ -1,0 +1,983 @@
+// libmdx C-ABI: context, system description, force evaluation, integrators, hipGraph replay.
+//
+// Boundary replaced (mdpy v0.2.1):
+//   Constraint.update / Ensemble.update      mdpy/constraint/constraint.py:45-46, mdpy/ensemble.py:53-71
+//   Integrator.integrate                     mdpy/integrator/integrator.py:36-39,
+//                                            verlet_integrator.py:20-50, langevin_integrator.py:37-72
+//   State.set_positions / CellList.update    mdpy/core/state.py:56-61, mdpy/core/cell_list.py:83-116
+// The context runs on its own non-blocking stream (default priority) and joins the caller's
+// stream with events at every entry and exit, so callers may pass PyTorch's current stream and
+// the step can still be captured into a hipGraph.
+#include "mdx_internal.h"
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <set>
+
+static thread_local std::string g_err;
+
+void mdx_set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+extern "C" const char* mdx_last_error(void) { return g_err.c_str(); }
+extern "C" int mdx_version(void) { return 100; }
+// 1/(4 pi eps0) in internal units (eps0 = 8.85418e-12 C^2 s^2/(kg m^3), mdpy/unit/__init__.py:56-58)
+extern "C" double mdx_coulomb_constant(void) { return 1.0 / (4.0 * M_PI * 0.5727652348434076); }
+
+// ------------------------------------------------------------------ stream plumbing
+struct ExtSync {
+  hipStream_t ext = nullptr;
+  hipEvent_t in = nullptr, out = nullptr;
+};
+static std::vector<std::pair<mdx_ctx*, ExtSync>> g_sync;  // per context (small)
+
+static ExtSync* sync_of(mdx_ctx* c) {
+  for (auto& p : g_sync)
+    if (p.first == c) return &p.second;
+  return nullptr;
+}
+
+int mdx_enter(mdx_ctx* c) {
+  ExtSync* s = sync_of(c);
+  if (!s) return 0;
+  MDX_HIP(hipSetDevice(c->device));
+  MDX_HIP(hipEventRecord(s->in, s->ext));
+  MDX_HIP(hipStreamWaitEvent(c->stream, s->in, 0));
+  return 0;
+}
+int mdx_leave(mdx_ctx* c) {
+  ExtSync* s = sync_of(c);
+  if (!s) return 0;
+  MDX_HIP(hipEventRecord(s->out, c->stream));
+  MDX_HIP(hipStreamWaitEvent(s->ext, s->out, 0));
+  return 0;
+}
+#define ENTER()                   \
+  do {                            \
+    int _r = mdx_enter(c);            \
+    if (_r) return _r;            \
+  } while (0)
+#define LEAVE_RET(v)              \
+  do {                            \
+    int _v = (v);                 \
+    int _r = mdx_leave(c);            \
+    return _v ? _v : _r;          \
+  } while (0)
+
+// ------------------------------------------------------------------ small kernels
+// Zero the force/energy accumulators. With record != 0 this also closes the previous step:
+// when its index (relative to the call's first step) is a multiple of `every`, its energies go
+// to history row index/every; then the step counter advances.
+__global__ void k_zero_acc(long long* __restrict__ ffix, long long n3, long long* __restrict__ efix, int ne,
+                           double* __restrict__ ehist, int ehist_cap, DevState* st, int record, int every) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3) ffix[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (record) {
+      long long k = st->step - st->hist_base;
+      if (k >= 0 && k % every == 0 && k / every < ehist_cap)
+        for (int t = 0; t < ne; ++t) ehist[(k / every) * ne + t] = (double)efix[t] * kFixInv;
+      st->step += 1;
+    }
+    for (int t = 0; t < ne; ++t) efix[t] = 0;
+  }
+}
+
+__global__ void k_record(const long long* __restrict__ efix, int ne, double* __restrict__ ehist, int ehist_cap,
+                         DevState* st, int every) {
+  long long k = st->step - st->hist_base;
+  if (k >= 0 && k % every == 0 && k / every < ehist_cap)
+    for (int t = 0; t < ne; ++t) ehist[(k / every) * ne + t] = (double)efix[t] * kFixInv;
+}
+
+__global__ void k_forces_out(const long long* __restrict__ ffix, int n, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int d = 0; d < 3; ++d) out[3 * i + d] = (double)ffix[(long long)d * n + i] * kFixInv;
+}
+
+__device__ void block_add_energy(double e, long long* slot) {
+  __shared__ double sm[4];
+  for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sm[w];
+    atomicAdd((unsigned long long*)slot, (unsigned long long)to_fix(s));
+  }
+}
+
+__global__ void k_kinetic(int n, const double* __restrict__ v, const double* __restrict__ m, long long* __restrict__ slot) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0;
+  if (i < n) e = 0.5 * m[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+  block_add_energy(e, slot);
+}
+
+// ------------------------------------------------------------------ Philox4x32-10
+// Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC'11.
+__device__ __forceinline__ uint4 philox4x32_10(uint4 ctr, uint2 key) {
+  const unsigned M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  for (int r = 0; r < 10; ++r) {
+    unsigned hi0 = __umulhi(M0, ctr.x), lo0 = M0 * ctr.x;
+    unsigned hi1 = __umulhi(M1, ctr.z), lo1 = M1 * ctr.z;
+    ctr = make_uint4(hi1 ^ ctr.y ^ key.x, lo1, hi0 ^ ctr.w ^ key.y, lo0);
+    key.x += W0;
+    key.y += W1;
+  }
+  return ctr;
+}
+// three standard normals for (atom, step) by Box-Muller
+__device__ void normal3(unsigned long long seed, long long step, int atom, double* g) {
+  uint2 key = make_uint2((unsigned)seed, (unsigned)(seed >> 32));
+  uint4 a = philox4x32_10(make_uint4((unsigned)atom, (unsigned)step, (unsigned)((unsigned long long)step >> 32), 0u), key);
+  uint4 b = philox4x32_10(make_uint4((unsigned)atom, (unsigned)step, (unsigned)((unsigned long long)step >> 32), 1u), key);
+  const double s = 2.3283064365386963e-10;  // 2^-32
+  double u1 = (a.x + 0.5) * s, u2 = (a.y + 0.5) * s, u3 = (a.z + 0.5) * s, u4 = (a.w + 0.5) * s;
+  double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+  double s1, c1, s2, c2;
+  sincospi(2.0 * u2, &s1, &c1);
+  sincospi(2.0 * u4, &s2, &c2);
+  g[0] = r1 * c1;
+  g[1] = r1 * s1;
+  g[2] = r2 * c2;
+  (void)b;
+}
+
+struct NoiseSrc {
+  unsigned long long seed;
+  const double* ext;
+  long long ext_base;
+  int ext_steps;
+};
+__device__ __forceinline__ void noise3(const NoiseSrc& ns, long long step, int atom, int n, double* g) {
+  long long k = step - ns.ext_base;
+  if (ns.ext && k >= 0 && k < ns.ext_steps) {
+    const double* p = ns.ext + (k * n + atom) * 3;
+    g[0] = p[0];
+    g[1] = p[1];
+    g[2] = p[2];
+  } else {
+    normal3(ns.seed, step, atom, g);
+  }
+}
+
+// ------------------------------------------------------------------ integrator kernels
+struct IntArgs {
+  int n;
+  double dt, a, b, sigma;
+  double L[3], inv[3];
+  NoiseSrc ns;
+};
+
+__device__ __forceinline__ double fix_force(const long long* ffix, int n, int i, int d) {
+  return (double)ffix[(long long)d * n + i] * kFixInv;
+}
+
+// velocity Verlet, first half: v += dt/2 a; x += dt v
+__global__ void k_vv_a(IntArgs A, double* __restrict__ x, double* __restrict__ v, const double* __restrict__ im,
+                       const long long* __restrict__ ffix) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  for (int d = 0; d < 3; ++d) {
+    double vv = v[3 * i + d] + 0.5 * A.dt * fix_force(ffix, A.n, i, d) * im[i];
+    v[3 * i + d] = vv;
+    x[3 * i + d] += A.dt * vv;
+  }
+}
+// velocity Verlet, second half: v += dt/2 a; kinetic energy
+__global__ void k_vv_b(IntArgs A, double* __restrict__ v, const double* __restrict__ im, const double* __restrict__ m,
+                       const long long* __restrict__ ffix, long long* __restrict__ ke) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0;
+  if (i < A.n) {
+    for (int d = 0; d < 3; ++d) {
+      double vv = v[3 * i + d] + 0.5 * A.dt * fix_force(ffix, A.n, i, d) * im[i];
+      v[3 * i + d] = vv;
+      e += vv * vv;
+    }
+    e *= 0.5 * m[i];
+  }
+  block_add_energy(e, ke);
+}
+
+// G-JF Langevin (Gronbech-Jensen & Farago, Mol. Phys. 111, 983 (2013)) with mdpy's a, b, sigma
+// (langevin_integrator.py:22-28). Position half: uses a_n and xi_{n+1}; stores a_n.
+__global__ void k_gjf_a(IntArgs A, double* __restrict__ x, const double* __restrict__ v, const double* __restrict__ im,
+                        const long long* __restrict__ ffix, double* __restrict__ acc_prev, const DevState* st) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  double g[3];
+  noise3(A.ns, st->step + 1, i, A.n, g);
+  double sm = sqrt(im[i]);
+  for (int d = 0; d < 3; ++d) {
+    double acc = fix_force(ffix, A.n, i, d) * im[i];
+    acc_prev[3 * i + d] = acc;
+    x[3 * i + d] += A.b * A.dt * v[3 * i + d] + A.b * A.dt * A.dt * 0.5 * acc +
+                    A.b * A.sigma * A.dt * sqrt(A.dt) * 0.5 * g[d] * sm;
+  }
+}
+// velocity half: v_{n+1} = a v_n + dt/2 (a a_n + a_{n+1}) + b sigma sqrt(dt) xi_{n+1}/sqrt(m)
+__global__ void k_gjf_b(IntArgs A, double* __restrict__ v, const double* __restrict__ im, const double* __restrict__ m,
+                        const long long* __restrict__ ffix, const double* __restrict__ acc_prev, const DevState* st,
+                        long long* __restrict__ ke) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0;
+  if (i < A.n) {
+    double g[3];
+    noise3(A.ns, st->step, i, A.n, g);
+    double sm = sqrt(im[i]);
+    for (int d = 0; d < 3; ++d) {
+      double acc = fix_force(ffix, A.n, i, d) * im[i];
+      double vv = A.a * v[3 * i + d] + 0.5 * A.dt * (A.a * acc_prev[3 * i + d] + acc) + A.b * A.sigma * sqrt(A.dt) * g[d] * sm;
+      v[3 * i + d] = vv;
+      e += vv * vv;
+    }
+    e *= 0.5 * m[i];
+  }
+  block_add_energy(e, ke);
+}
+
+// reference wrap (pbc.py:28-36) of the integrator's unwrapped positions into the state array
+__device__ __forceinline__ void wrap_into(const IntArgs& A, const double* xin, double* xout, DevState* st) {
+  bool lost = false;
+  double o[3];
+  for (int d = 0; d < 3; ++d) {
+    double m = -rint(xin[d] * A.inv[d]);
+    if (fabs(m) >= 2.0) lost = true;
+    o[d] = xin[d] + m * A.L[d];
+  }
+  if (lost) atomicOr(&st->flags, MDX_FLAG_LOSS);
+  for (int d = 0; d < 3; ++d) xout[d] = o[d];
+}
+
+// reference position Verlet step (verlet_integrator.py:37-43): new = 2 cur - pre + a dt^2
+__global__ void k_vref(IntArgs A, double* __restrict__ cur, double* __restrict__ pre, double* __restrict__ pos,
+                       const double* __restrict__ im, const long long* __restrict__ ffix, DevState* st) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  double nx[3];
+  for (int d = 0; d < 3; ++d) {
+    double acc = fix_force(ffix, A.n, i, d) * im[i];
+    nx[d] = 2 * cur[3 * i + d] - pre[3 * i + d] + acc * A.dt * A.dt;
+    pre[3 * i + d] = cur[3 * i + d];
+    cur[3 * i + d] = nx[d];
+  }
+  wrap_into(A, nx, pos + 3 * i, st);
+}
+
+// reference Langevin, position half (langevin_integrator.py:54-61)
+__global__ void k_lref_a(IntArgs A, double* __restrict__ cx, double* __restrict__ px, const double* __restrict__ pv,
+                         double* __restrict__ pos, const double* __restrict__ im, const double* __restrict__ acc_cur,
+                         double* __restrict__ acc_pre, DevState* st) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  double g[3];
+  noise3(A.ns, st->step + 1, i, A.n, g);
+  double sm = sqrt(im[i]);
+  double nx[3];
+  for (int d = 0; d < 3; ++d) {
+    double pa = acc_cur[3 * i + d];
+    acc_pre[3 * i + d] = pa;
+    double xi = g[d] * sm;
+    nx[d] = px[3 * i + d] + A.b * A.dt * pv[3 * i + d] - A.b * A.dt * A.dt / 2 * pa +
+            A.b * A.sigma * pow(A.dt, 1.5) / 2 * xi;
+    px[3 * i + d] = cx[3 * i + d];
+    cx[3 * i + d] = nx[d];
+  }
+  wrap_into(A, nx, pos + 3 * i, st);
+}
+
+// reference Langevin, velocity half (langevin_integrator.py:64-70)
+__global__ void k_lref_b(IntArgs A, double* __restrict__ cv, double* __restrict__ pv, const double* __restrict__ im,
+                         const double* __restrict__ m, const long long* __restrict__ ffix, double* __restrict__ acc_cur,
+                         const double* __restrict__ acc_pre, const DevState* st, long long* __restrict__ ke) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0;
+  if (i < A.n) {
+    double g[3];
+    noise3(A.ns, st->step, i, A.n, g);
+    double sm = sqrt(im[i]);
+    for (int d = 0; d < 3; ++d) {
+      double ca = fix_force(ffix, A.n, i, d) * im[i];
+      acc_cur[3 * i + d] = ca;
+      double nv = A.a * pv[3 * i + d] - A.dt / 2 * (A.a * acc_pre[3 * i + d] + ca) + A.b * A.sigma * sqrt(A.dt) * g[d] * sm;
+      pv[3 * i + d] = cv[3 * i + d];
+      cv[3 * i + d] = nv;
+      e += nv * nv;
+    }
+    e *= 0.5 * m[i];
+  }
+  block_add_energy(e, ke);
+}
+
+// prime kernels
+__global__ void k_vref_prime(IntArgs A, const double* __restrict__ x, const double* __restrict__ v,
+                             const double* __restrict__ im, const long long* __restrict__ ffix, double* __restrict__ cur,
+                             double* __restrict__ pre, int cached) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n || cached) return;
+  for (int d = 0; d < 3; ++d) {
+    double acc = fix_force(ffix, A.n, i, d) * im[i];
+    cur[3 * i + d] = x[3 * i + d];
+    pre[3 * i + d] = x[3 * i + d] - v[3 * i + d] * A.dt + acc * A.dt * A.dt;  // verlet_integrator.py:31-34
+  }
+}
+__global__ void k_lref_prime(IntArgs A, const double* __restrict__ x, const double* __restrict__ v,
+                             const double* __restrict__ im, const long long* __restrict__ ffix, double* __restrict__ cx,
+                             double* __restrict__ px, double* __restrict__ cv, double* __restrict__ pv,
+                             double* __restrict__ acc_cur, int cached) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  for (int d = 0; d < 3; ++d) {
+    acc_cur[3 * i + d] = fix_force(ffix, A.n, i, d) * im[i];  // langevin_integrator.py:42
+    if (!cached) {
+      cx[3 * i + d] = px[3 * i + d] = x[3 * i + d];
+      cv[3 * i + d] = pv[3 * i + d] = v[3 * i + d];
+    }
+  }
+}
+// end of a reference Verlet call: v = mi(cur - pre)/(2 dt) (verlet_integrator.py:47-50)
+__global__ void k_vref_finish(IntArgs A, const double* __restrict__ cur, const double* __restrict__ pre,
+                              double* __restrict__ v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  for (int d = 0; d < 3; ++d) {
+    double s = (cur[3 * i + d] - pre[3 * i + d]) * A.inv[d];
+    s -= rint(s);
+    v[3 * i + d] = s * A.L[d] / 2 / A.dt;
+  }
+}
+__global__ void k_copy3(const double* __restrict__ src, double* __restrict__ dst, long long n3) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3) dst[i] = src[i];
+}
+// wrap the state positions at the end of a velocity-Verlet / G-JF call (State.set_positions)
+__global__ void k_wrap_state(IntArgs A, double* __restrict__ pos, DevState* st) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  double x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  wrap_into(A, x, pos + 3 * i, st);
+}
+
+// ------------------------------------------------------------------ context API
+extern "C" int mdx_ctx_create(int device, void* hip_stream, mdx_ctx** out) {
+  MDX_CHECK(out, MDX_ERR_INVALID, "mdx_ctx_create: out is NULL");
+  int ndev = 0;
+  MDX_HIP(hipGetDeviceCount(&ndev));
+  MDX_CHECK(device >= 0 && device < ndev, MDX_ERR_INVALID, "device %d not present (%d devices)", device, ndev);
+  MDX_HIP(hipSetDevice(device));
+  mdx_ctx* c = new mdx_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    mdx_set_error("hipStreamCreateWithFlags failed");
+    return MDX_ERR_HIP;
+  }
+  ExtSync s;
+  s.ext = (hipStream_t)hip_stream;
+  hipEventCreateWithFlags(&s.in, hipEventDisableTiming);
+  hipEventCreateWithFlags(&s.out, hipEventDisableTiming);
+  g_sync.push_back({c, s});
+  int rc;
+  if ((rc = c->dstate.ensure(1)) || (rc = c->efix.ensure(MDX_NTERMS + 1))) return rc;
+  MDX_HIP(hipMemset(c->dstate.p, 0, sizeof(DevState)));
+  MDX_HIP(hipMemset(c->efix.p, 0, sizeof(long long) * (MDX_NTERMS + 1)));
+  int one = 1;
+  MDX_HIP(hipMemcpy(&c->dstate.p->rebuild, &one, sizeof(int), hipMemcpyHostToDevice));
+  *out = c;
+  return 0;
+}
+
+extern "C" int mdx_ctx_set_stream(mdx_ctx* c, void* hip_stream) {
+  MDX_CHECK(c, MDX_ERR_INVALID, "null context");
+  ExtSync* s = sync_of(c);
+  if (s) s->ext = (hipStream_t)hip_stream;
+  return 0;
+}
+
+extern "C" int mdx_ctx_destroy(mdx_ctx* c) {
+  if (!c) return 0;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+  for (size_t k = 0; k < g_sync.size(); ++k)
+    if (g_sync[k].first == c) {
+      hipEventDestroy(g_sync[k].second.in);
+      hipEventDestroy(g_sync[k].second.out);
+      g_sync.erase(g_sync.begin() + k);
+      break;
+    }
+  c->inv_mass.release(); c->mass.release(); c->charge64.release(); c->charge_s.release(); c->lj64.release();
+  c->excl_ptr.release(); c->excl_idx.release(); c->s14_ptr.release(); c->s14_idx.release();
+  c->bond_idx.release(); c->angle_idx.release(); c->dih_idx.release(); c->imp_idx.release();
+  c->bond_par.release(); c->angle_par.release(); c->dih_par.release(); c->imp_par.release();
+  c->ffix.release(); c->efix.release(); c->dstate.release();
+  c->ref_cell_lin.release(); c->ref_counts.release(); c->ref_cursor.release();
+  c->cell_count.release(); c->cell_start.release(); c->cell_cursor.release(); c->atom_cell.release();
+  c->sorted_orig.release(); c->orig_sorted.release(); c->xq.release(); c->ljs.release(); c->ljs14.release();
+  c->ref_cell_sorted.release(); c->pos_at_build.release(); c->nbr_count.release(); c->nbr_list.release();
+  c->d_stencil.release(); c->qfix.release(); c->cgrid.release(); c->conv.release(); c->bc.release();
+  c->twiddle.release(); c->excl_pairs.release(); c->xa.release(); c->xb.release(); c->va.release();
+  c->vb.release(); c->acc_prev.release(); c->acc_cur.release(); c->ehist.release(); c->tmp64.release();
+  hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+static void invalidate_graph(mdx_ctx* c) {
+  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+  c->graph_exec = nullptr;
+  c->graph_kind = -1;
+}
+
+extern "C" int mdx_system_set(mdx_ctx* c, int n, double* d_pos, double* d_vel, const double* h_box) {
+  MDX_CHECK(c && n > 0 && d_pos && h_box, MDX_ERR_INVALID, "mdx_system_set: bad arguments");
+  MDX_CHECK(n < (1 << 30), MDX_ERR_INVALID, "too many atoms");
+  MDX_HIP(hipSetDevice(c->device));
+  invalidate_graph(c);
+  bool resized = c->n != n;
+  c->n = n;
+  c->pos = d_pos;
+  c->vel = d_vel;
+  for (int d = 0; d < 3; ++d) {
+    MDX_CHECK(h_box[d] > 0, MDX_ERR_INVALID, "box edges must be positive");
+    c->box[d] = h_box[d];
+  }
+  int rc;
+  if ((rc = c->ffix.ensure(3 * (size_t)n))) return rc;
+  if (resized) {
+    c->nb_configured = false;
+    c->pme_configured = false;
+    c->int_primed = false;
+  }
+  MDX_HIP(hipMemset(c->ffix.p, 0, 3 * sizeof(long long) * n));
+  int one = 1;
+  MDX_HIP(hipMemcpy(&c->dstate.p->rebuild, &one, sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int mdx_set_box(mdx_ctx* c, const double* h_box) {
+  MDX_CHECK(c && h_box, MDX_ERR_INVALID, "mdx_set_box: bad arguments");
+  for (int d = 0; d < 3; ++d) {
+    MDX_CHECK(h_box[d] > 0, MDX_ERR_INVALID, "box edges must be positive");
+    c->box[d] = h_box[d];
+  }
+  invalidate_graph(c);
+  int rc;
+  if (c->nb_configured && (rc = mdx_nlist_prepare(c))) return rc;
+  if (c->pme_configured) {
+    int K[3] = {c->K[0], c->K[1], c->K[2]};
+    if ((rc = mdx_pme_config(c, K, c->order, c->beta))) return rc;
+  }
+  return 0;
+}
+
+template <typename T>
+static int upload(DevBuf<T>& b, const T* h, size_t count) {
+  int rc = b.ensure(count);
+  if (rc) return rc;
+  if (count) MDX_HIP(hipMemcpy(b.p, h, count * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int mdx_set_masses(mdx_ctx* c, const double* h_mass) {
+  MDX_CHECK(c && c->n > 0 && h_mass, MDX_ERR_INVALID, "mdx_set_masses: bad arguments");
+  std::vector<double> im(c->n);
+  for (int i = 0; i < c->n; ++i) {
+    MDX_CHECK(h_mass[i] > 0, MDX_ERR_INVALID, "mass of atom %d is not positive", i);
+    im[i] = 1.0 / h_mass[i];
+  }
+  int rc;
+  if ((rc = upload(c->mass, h_mass, c->n)) || (rc = upload(c->inv_mass, im.data(), c->n))) return rc;
+  c->has_mass = true;
+  return 0;
+}
+
+extern "C" int mdx_set_charges(mdx_ctx* c, const double* h_q) {
+  MDX_CHECK(c && c->n > 0 && h_q, MDX_ERR_INVALID, "mdx_set_charges: bad arguments");
+  const double sk = sqrt(mdx_coulomb_constant());
+  std::vector<float> qs(c->n);
+  c->qsum = 0;
+  c->q2sum = 0;
+  for (int i = 0; i < c->n; ++i) {
+    qs[i] = (float)(h_q[i] * sk);
+    c->qsum += h_q[i];
+    c->q2sum += h_q[i] * h_q[i];
+  }
+  int rc;
+  if ((rc = upload(c->charge64, h_q, c->n)) || (rc = upload(c->charge_s, qs.data(), c->n))) return rc;
+  c->has_charges = true;
+  return 0;
+}
+
+extern "C" int mdx_set_lj(mdx_ctx* c, const double* h_lj) {
+  MDX_CHECK(c && c->n > 0 && h_lj, MDX_ERR_INVALID, "mdx_set_lj: bad arguments");
+  for (int i = 0; i < 4 * c->n; ++i)
+    MDX_CHECK(!((i % 2 == 0) && h_lj[i] < 0), MDX_ERR_INVALID, "LJ epsilon must be >= 0 (atom %d)", i / 4);
+  int rc = upload(c->lj64, h_lj, 4 * (size_t)c->n);
+  if (rc) return rc;
+  c->has_lj = true;
+  int one = 1;
+  MDX_HIP(hipMemcpy(&c->dstate.p->rebuild, &one, sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int set_csr(mdx_ctx* c, const int* ptr, const int* idx, DevBuf<int>& dp, DevBuf<int>& di, int* maxrow) {
+  MDX_CHECK(ptr && ptr[0] == 0, MDX_ERR_INVALID, "CSR ptr must start at 0");
+  int nnz = ptr[c->n];
+  *maxrow = 0;
+  for (int i = 0; i < c->n; ++i) {
+    MDX_CHECK(ptr[i + 1] >= ptr[i], MDX_ERR_INVALID, "CSR ptr not monotone at %d", i);
+    *maxrow = std::max(*maxrow, ptr[i + 1] - ptr[i]);
+  }
+  for (int k = 0; k < nnz; ++k) MDX_CHECK(idx[k] >= 0 && idx[k] < c->n, MDX_ERR_INVALID, "CSR index out of range");
+  int rc;
+  if ((rc = upload(dp, ptr, c->n + 1)) || (rc = upload(di, idx, std::max(nnz, 1)))) return rc;
+  return 0;
+}
+
+extern "C" int mdx_set_exclusions(mdx_ctx* c, const int* h_ptr, const int* h_idx) {
+  MDX_CHECK(c && c->n > 0, MDX_ERR_INVALID, "mdx_set_exclusions: no system");
+  int rc = set_csr(c, h_ptr, h_idx, c->excl_ptr, c->excl_idx, &c->max_excl);
+  if (rc) return rc;
+  c->has_excl = h_ptr[c->n] > 0;
+  // unique excluded pairs (i < j) for the PME erf correction
+  std::vector<int> pairs;
+  for (int i = 0; i < c->n; ++i)
+    for (int k = h_ptr[i]; k < h_ptr[i + 1]; ++k)
+      if (h_idx[k] > i) {
+        pairs.push_back(i);
+        pairs.push_back(h_idx[k]);
+      }
+  c->n_excl_pairs = (int)pairs.size() / 2;
+  if (pairs.empty()) pairs.assign(2, 0);
+  if ((rc = upload(c->excl_pairs, pairs.data(), pairs.size()))) return rc;
+  int one = 1;
+  MDX_HIP(hipMemcpy(&c->dstate.p->rebuild, &one, sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int mdx_set_scaling(mdx_ctx* c, const int* h_ptr, const int* h_idx) {
+  MDX_CHECK(c && c->n > 0, MDX_ERR_INVALID, "mdx_set_scaling: no system");
+  int rc = set_csr(c, h_ptr, h_idx, c->s14_ptr, c->s14_idx, &c->max_s14);
+  if (rc) return rc;
+  c->has_s14 = h_ptr[c->n] > 0;
+  int one = 1;
+  MDX_HIP(hipMemcpy(&c->dstate.p->rebuild, &one, sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int set_terms(mdx_ctx* c, int count, int width_i, int width_p, const int* idx, const double* par,
+                     DevBuf<int>& di, DevBuf<double>& dp, int* out_count) {
+  MDX_CHECK(c && c->n > 0 && count >= 0, MDX_ERR_INVALID, "bad bonded-term arguments");
+  for (long long k = 0; k < (long long)count * width_i; ++k)
+    MDX_CHECK(idx[k] >= 0 && idx[k] < c->n, MDX_ERR_INVALID, "bonded-term atom index out of range");
+  int rc;
+  if (count > 0) {
+    if ((rc = upload(di, idx, (size_t)count * width_i)) || (rc = upload(dp, par, (size_t)count * width_p))) return rc;
+  }
+  *out_count = count;
+  return 0;
+}
+extern "C" int mdx_set_bonds(mdx_ctx* c, int nb, const int* i, const double* p) { return set_terms(c, nb, 2, 2, i, p, c->bond_idx, c->bond_par, &c->nb); }
+extern "C" int mdx_set_angles(mdx_ctx* c, int na, const int* i, const double* p) { return set_terms(c, na, 3, 4, i, p, c->angle_idx, c->angle_par, &c->na); }
+extern "C" int mdx_set_dihedrals(mdx_ctx* c, int nd, const int* i, const double* p, int refcompat) {
+  int rc = set_terms(c, nd, 4, 3, i, p, c->dih_idx, c->dih_par, &c->nd);
+  if (!rc) c->dih_refcompat = refcompat;
+  return rc;
+}
+extern "C" int mdx_set_impropers(mdx_ctx* c, int ni, const int* i, const double* p) { return set_terms(c, ni, 4, 2, i, p, c->imp_idx, c->imp_par, &c->ni); }
+
+extern "C" int mdx_nonbonded_config(mdx_ctx* c, double cutoff, double skin, int pair_mode, double cell_cutoff) {
+  MDX_CHECK(c && c->n > 0, MDX_ERR_INVALID, "mdx_nonbonded_config: no system");
+  MDX_CHECK(cutoff > 0 && skin >= 0, MDX_ERR_INVALID, "cutoff must be > 0 and skin >= 0");
+  MDX_CHECK(pair_mode == MDX_PAIRS_EXACT || pair_mode == MDX_PAIRS_REFERENCE, MDX_ERR_INVALID, "bad pair mode");
+  for (int d = 0; d < 3; ++d)
+    MDX_CHECK(cutoff <= 0.5 * c->box[d], MDX_ERR_CELL_CUTOFF,
+              "cutoff %.3f exceeds half the box edge %.3f (minimum image)", cutoff, c->box[d]);
+  c->rc = cutoff;
+  // the reference pair set depends on the minimum position, which moves every step
+  c->skin = pair_mode == MDX_PAIRS_REFERENCE ? 0.0 : skin;
+  c->pair_mode = pair_mode;
+  c->cell_cutoff = cell_cutoff > 0 ? cell_cutoff : 12.0;
+  invalidate_graph(c);
+  int rc = mdx_nlist_prepare(c);
+  if (rc) return rc;
+  c->nb_configured = true;
+  return 0;
+}
+
+// ------------------------------------------------------------------ force evaluation
+int mdx_enqueue_zero_accumulators(mdx_ctx* c) {
+  long long n3 = 3LL * c->n;
+  k_zero_acc<<<div_up(n3, 256), 256, 0, c->stream>>>(c->ffix.p, n3, c->efix.p, MDX_NTERMS + 1, nullptr, 0,
+                                                     c->dstate.p, 0, 1);
+  MDX_LAUNCH_CHECK();
+  return 0;
+}
+
+int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy) {
+  int rc;
+  bool lj = (terms & MDX_TERM_LJ) != 0, cd = (terms & MDX_TERM_COUL_DIRECT) != 0;
+  if (lj || cd) {
+    MDX_CHECK(c->nb_configured, MDX_ERR_INVALID, "nonbonded terms requested but mdx_nonbonded_config not called");
+    MDX_CHECK(!lj || c->has_lj, MDX_ERR_INVALID, "LJ requested but no LJ parameters set");
+    MDX_CHECK(!cd || (c->has_charges && c->pme_configured), MDX_ERR_INVALID,
+              "PME direct space requested but charges/PME not configured");
+    if ((rc = mdx_nlist_update(c, false))) return rc;
+    if ((rc = mdx_enqueue_nonbonded(c, lj, cd, energy))) return rc;
+  }
+  if (terms & MDX_TERM_COUL_RECIP) {
+    MDX_CHECK(c->has_charges && c->pme_configured, MDX_ERR_INVALID, "PME requested but not configured");
+    if ((rc = mdx_pme_enqueue(c, energy))) return rc;
+  }
+  if (terms & MDX_TERM_COUL_PLAIN) {
+    MDX_CHECK(c->has_charges, MDX_ERR_INVALID, "Coulomb requested but no charges set");
+    if ((rc = mdx_enqueue_coulomb_plain(c))) return rc;
+  }
+  if ((rc = mdx_bonded_enqueue(c, terms))) return rc;
+  return 0;
+}
+
+extern "C" int mdx_compute_forces(mdx_ctx* c, unsigned terms, double* d_force_out) {
+  MDX_CHECK(c && c->n > 0, MDX_ERR_INVALID, "mdx_compute_forces: no system");
+  ENTER();
+  int rc = mdx_enqueue_zero_accumulators(c);
+  if (!rc) rc = mdx_enqueue_forces(c, terms, true);
+  if (!rc && c->vel && c->has_mass) {
+    k_kinetic<<<div_up(c->n, 256), 256, 0, c->stream>>>(c->n, c->vel, c->mass.p, c->efix.p + MDX_E_KINETIC);
+    if (hipGetLastError() != hipSuccess) rc = MDX_ERR_HIP;
+  }
+  if (!rc && d_force_out) {
+    k_forces_out<<<div_up(c->n, 256), 256, 0, c->stream>>>(c->ffix.p, c->n, d_force_out);
+    if (hipGetLastError() != hipSuccess) rc = MDX_ERR_HIP;
+  }
+  c->evals++;
+  LEAVE_RET(rc);
+}
+
+extern "C" int mdx_energies_fetch(mdx_ctx* c, double* h_out) {
+  MDX_CHECK(c && h_out, MDX_ERR_INVALID, "mdx_energies_fetch: bad arguments");
+  long long e[MDX_NTERMS + 1];
+  MDX_HIP(hipMemcpyAsync(e, c->efix.p, sizeof(e), hipMemcpyDeviceToHost, c->stream));
+  MDX_HIP(hipStreamSynchronize(c->stream));
+  for (int t = 0; t <= MDX_NTERMS; ++t) h_out[t] = (double)e[t] * kFixInv;
+  return 0;
+}
+
+extern "C" int mdx_status_fetch(mdx_ctx* c, unsigned* h_flags) {
+  MDX_CHECK(c && h_flags, MDX_ERR_INVALID, "mdx_status_fetch: bad arguments");
+  MDX_HIP(hipMemcpyAsync(h_flags, &c->dstate.p->flags, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+  MDX_HIP(hipMemsetAsync(&c->dstate.p->flags, 0, sizeof(unsigned), c->stream));
+  MDX_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int mdx_stats_fetch(mdx_ctx* c, int64_t* h_rebuilds, int64_t* h_evals) {
+  MDX_CHECK(c, MDX_ERR_INVALID, "null context");
+  long long nr = 0;
+  MDX_HIP(hipMemcpyAsync(&nr, &c->dstate.p->nrebuild, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+  MDX_HIP(hipStreamSynchronize(c->stream));
+  if (h_rebuilds) *h_rebuilds = nr;
+  if (h_evals) *h_evals = c->evals;
+  return 0;
+}
+
+// ------------------------------------------------------------------ integration
+extern "C" int mdx_integrator_config(mdx_ctx* c, int kind, double dt, double gamma, double kT, uint64_t seed) {
+  MDX_CHECK(c && c->n > 0, MDX_ERR_INVALID, "mdx_integrator_config: no system");
+  MDX_CHECK(kind >= 0 && kind <= 3, MDX_ERR_INVALID, "unknown integrator kind %d", kind);
+  MDX_CHECK(dt > 0 && gamma >= 0 && kT >= 0, MDX_ERR_INVALID, "dt > 0, gamma >= 0, kT >= 0 required");
+  if (kind != c->int_kind) c->int_primed = false;
+  c->int_kind = kind;
+  c->dt = dt;
+  c->gamma = gamma;
+  c->kT = kT;
+  c->seed = seed;
+  c->int_configured = true;
+  invalidate_graph(c);
+  return 0;
+}
+
+extern "C" int mdx_set_energy_interval(mdx_ctx* c, int every) {
+  MDX_CHECK(c && every >= 1, MDX_ERR_INVALID, "energy interval must be >= 1");
+  c->energy_every = every;
+  return 0;
+}
+
+extern "C" int mdx_integrator_reset(mdx_ctx* c) {
+  MDX_CHECK(c, MDX_ERR_INVALID, "null context");
+  c->int_primed = false;
+  return 0;
+}
+
+extern "C" int mdx_set_external_noise(mdx_ctx* c, const double* d_noise, int nsteps) {
+  MDX_CHECK(c, MDX_ERR_INVALID, "null context");
+  long long step = 0;
+  MDX_HIP(hipMemcpy(&step, &c->dstate.p->step, sizeof(long long), hipMemcpyDeviceToHost));
+  c->ext_noise = d_noise;
+  c->ext_noise_steps = d_noise ? nsteps : 0;
+  c->ext_noise_base = step + 1;  // first noise row is used by the next step
+  invalidate_graph(c);
+  return 0;
+}
+
+static IntArgs int_args(mdx_ctx* c) {
+  IntArgs A;
+  A.n = c->n;
+  A.dt = c->dt;
+  A.a = (1 - c->gamma * c->dt / 2) / (1 + c->gamma * c->dt / 2);
+  A.b = 1 / (1 + c->gamma * c->dt / 2);
+  A.sigma = sqrt(2 * c->kT * c->gamma);
+  for (int d = 0; d < 3; ++d) {
+    A.L[d] = c->box[d];
+    A.inv[d] = 1.0 / c->box[d];
+  }
+  A.ns.seed = c->seed;
+  A.ns.ext = c->ext_noise;
+  A.ns.ext_base = c->ext_noise_base;
+  A.ns.ext_steps = c->ext_noise_steps;
+  return A;
+}
+
+static int zero_and_record(mdx_ctx* c) {
+  long long n3 = 3LL * c->n;
+  k_zero_acc<<<div_up(n3, 256), 256, 0, c->stream>>>(c->ffix.p, n3, c->efix.p, MDX_NTERMS + 1, c->ehist.p,
+                                                     c->ehist_cap, c->dstate.p, 1, c->energy_every);
+  MDX_LAUNCH_CHECK();
+  return 0;
+}
+
+// One MD step; the launch sequence depends only on (kind, terms), so it can be captured.
+int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy) {
+  IntArgs A = int_args(c);
+  dim3 g(div_up(c->n, 256)), b(256);
+  int rc;
+  long long* ke = c->efix.p + MDX_E_KINETIC;
+  switch (c->int_kind) {
+    case MDX_INT_VELOCITY_VERLET:
+      k_vv_a<<<g, b, 0, c->stream>>>(A, c->pos, c->vel, c->inv_mass.p, c->ffix.p);
+      MDX_LAUNCH_CHECK();
+      if ((rc = zero_and_record(c)) || (rc = mdx_enqueue_forces(c, terms, energy))) return rc;
+      k_vv_b<<<g, b, 0, c->stream>>>(A, c->vel, c->inv_mass.p, c->mass.p, c->ffix.p, ke);
+      MDX_LAUNCH_CHECK();
+      break;
+    case MDX_INT_LANGEVIN:
+      k_gjf_a<<<g, b, 0, c->stream>>>(A, c->pos, c->vel, c->inv_mass.p, c->ffix.p, c->acc_prev.p, c->dstate.p);
+      MDX_LAUNCH_CHECK();
+      if ((rc = zero_and_record(c)) || (rc = mdx_enqueue_forces(c, terms, energy))) return rc;
+      k_gjf_b<<<g, b, 0, c->stream>>>(A, c->vel, c->inv_mass.p, c->mass.p, c->ffix.p, c->acc_prev.p, c->dstate.p, ke);
+      MDX_LAUNCH_CHECK();
+      break;
+    case MDX_INT_VERLET_REF:
+      k_vref<<<g, b, 0, c->stream>>>(A, c->xa.p, c->xb.p, c->pos, c->inv_mass.p, c->ffix.p, c->dstate.p);
+      MDX_LAUNCH_CHECK();
+      if ((rc = zero_and_record(c)) || (rc = mdx_enqueue_forces(c, terms, energy))) return rc;
+      k_kinetic<<<g, b, 0, c->stream>>>(c->n, c->vel, c->mass.p, ke);
+      MDX_LAUNCH_CHECK();
+      break;
+    case MDX_INT_LANGEVIN_REF:
+      k_lref_a<<<g, b, 0, c->stream>>>(A, c->xa.p, c->xb.p, c->vb.p, c->pos, c->inv_mass.p, c->acc_cur.p,
+                                       c->acc_prev.p, c->dstate.p);
+      MDX_LAUNCH_CHECK();
+      if ((rc = zero_and_record(c)) || (rc = mdx_enqueue_forces(c, terms, energy))) return rc;
+      k_lref_b<<<g, b, 0, c->stream>>>(A, c->va.p, c->vb.p, c->inv_mass.p, c->mass.p, c->ffix.p, c->acc_cur.p,
+                                       c->acc_prev.p, c->dstate.p, ke);
+      MDX_LAUNCH_CHECK();
+      break;
+  }
+  return 0;
+}
+
+extern "C" int mdx_integrate(mdx_ctx* c, unsigned terms, int nsteps, int use_graph) {
+  MDX_CHECK(c && c->n > 0 && c->vel, MDX_ERR_INVALID, "mdx_integrate: system with velocities required");
+  MDX_CHECK(c->int_configured && c->has_mass, MDX_ERR_INVALID, "integrator or masses not configured");
+  MDX_CHECK(nsteps >= 0, MDX_ERR_INVALID, "nsteps must be >= 0");
+  ENTER();
+  int rc = 0;
+  size_t n3 = 3 * (size_t)c->n;
+  if ((rc = c->acc_prev.ensure(n3)) || (rc = c->acc_cur.ensure(n3))) LEAVE_RET(rc);
+  if (c->int_kind >= MDX_INT_VERLET_REF &&
+      ((rc = c->xa.ensure(n3)) || (rc = c->xb.ensure(n3)) || (rc = c->va.ensure(n3)) || (rc = c->vb.ensure(n3))))
+    LEAVE_RET(rc);
+  const int ne = MDX_NTERMS + 1;
+  const int every = c->energy_every;
+  const int nrows = nsteps / every + 1;
+  if (nrows > c->ehist_cap) {
+    if ((rc = c->ehist.ensure((size_t)nrows * ne))) LEAVE_RET(rc);
+    c->ehist_cap = nrows;
+  }
+  IntArgs A = int_args(c);
+  dim3 g(div_up(c->n, 256)), b(256);
+  // energy-history rows are (device step counter - hist_base); set the base for this call
+  long long s0 = 0;
+  MDX_HIP(hipMemcpyAsync(&s0, &c->dstate.p->step, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+  MDX_HIP(hipStreamSynchronize(c->stream));
+  MDX_HIP(hipMemcpyAsync(&c->dstate.p->hist_base, &s0, sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  MDX_HIP(hipStreamSynchronize(c->stream));
+  c->ehist_count = 0;
+  // prime: forces at the current positions (every mdpy integrate call starts with ensemble.update)
+  if ((rc = mdx_enqueue_zero_accumulators(c)) || (rc = mdx_enqueue_forces(c, terms, true))) LEAVE_RET(rc);
+  k_kinetic<<<g, b, 0, c->stream>>>(c->n, c->vel, c->mass.p, c->efix.p + MDX_E_KINETIC);
+  if (c->int_kind == MDX_INT_VERLET_REF) {
+    k_vref_prime<<<g, b, 0, c->stream>>>(A, c->pos, c->vel, c->inv_mass.p, c->ffix.p, c->xa.p, c->xb.p,
+                                         c->int_primed ? 1 : 0);
+  } else if (c->int_kind == MDX_INT_LANGEVIN_REF) {
+    k_lref_prime<<<g, b, 0, c->stream>>>(A, c->pos, c->vel, c->inv_mass.p, c->ffix.p, c->xa.p, c->xb.p, c->va.p,
+                                         c->vb.p, c->acc_cur.p, c->int_primed ? 1 : 0);
+  }
+  MDX_LAUNCH_CHECK();
+  c->int_primed = true;
+  // potential energies are evaluated on steps that are multiples of `every` (the prime counts
+  // as step 0); the graph holds a whole number of energy intervals
+  int local = 0;
+  if (use_graph && nsteps > 0) {
+    const int gsteps = std::min(nsteps, every >= 10 ? every : every * ((10 + every - 1) / every));
+    if (!c->graph_exec || c->graph_terms != terms || c->graph_steps != gsteps || c->graph_kind != c->int_kind ||
+        c->graph_every != every) {
+      invalidate_graph(c);
+      hipGraph_t graph;
+      MDX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < gsteps && !rc; ++k) rc = mdx_integrator_enqueue_step(c, terms, (k + 1) % every == 0);
+      hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+      if (rc) LEAVE_RET(rc);
+      if (e2 != hipSuccess) { mdx_set_error("graph capture failed: %s", hipGetErrorString(e2)); LEAVE_RET(MDX_ERR_HIP); }
+      hipError_t e3 = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (e3 != hipSuccess) { mdx_set_error("graph instantiate failed: %s", hipGetErrorString(e3)); LEAVE_RET(MDX_ERR_HIP); }
+      c->graph_terms = terms;
+      c->graph_steps = gsteps;
+      c->graph_every = every;
+      c->graph_kind = c->int_kind;
+    }
+    int full = nsteps / gsteps, rem = nsteps % gsteps;
+    for (int k = 0; k < full; ++k) {
+      hipError_t e = hipGraphLaunch(c->graph_exec, c->stream);
+      if (e != hipSuccess) { mdx_set_error("hipGraphLaunch: %s", hipGetErrorString(e)); LEAVE_RET(MDX_ERR_HIP); }
+    }
+    local = full * gsteps;
+    for (int k = 0; k < rem && !rc; ++k, ++local) rc = mdx_integrator_enqueue_step(c, terms, (local + 1) % every == 0);
+  } else {
+    for (int k = 0; k < nsteps && !rc; ++k, ++local) rc = mdx_integrator_enqueue_step(c, terms, (local + 1) % every == 0);
+  }
+  if (!rc) {
+    k_record<<<1, 1, 0, c->stream>>>(c->efix.p, ne, c->ehist.p, c->ehist_cap, c->dstate.p, every);
+    if (hipGetLastError() != hipSuccess) rc = MDX_ERR_HIP;
+  }
+  c->ehist_count = nrows;
+  if (rc) LEAVE_RET(rc);
+  // finish: state velocities / wrapped positions as the reference leaves them
+  if (c->int_kind == MDX_INT_VERLET_REF) {
+    k_vref_finish<<<g, b, 0, c->stream>>>(A, c->xa.p, c->xb.p, c->vel);
+  } else if (c->int_kind == MDX_INT_LANGEVIN_REF) {
+    k_copy3<<<div_up(3LL * c->n, 256), 256, 0, c->stream>>>(c->va.p, c->vel, 3LL * c->n);
+  } else {
+    k_wrap_state<<<g, b, 0, c->stream>>>(A, c->pos, c->dstate.p);
+  }
+  MDX_LAUNCH_CHECK();
+  c->evals += nsteps + 1;
+  // positions were wrapped: the neighbour list must not trust its displacement reference
+  LEAVE_RET(mdx_enqueue_request_rebuild(c));
+}
+
+extern "C" int mdx_energy_history_fetch(mdx_ctx* c, double* h_out, int h_cap, int* h_count) {
+  MDX_CHECK(c && h_out && h_count, MDX_ERR_INVALID, "mdx_energy_history_fetch: bad arguments");
+  int cnt = std::min(h_cap, c->ehist_count);
+  const int ne = MDX_NTERMS + 1;
+  if (cnt > 0)
+    MDX_HIP(hipMemcpyAsync(h_out, c->ehist.p, (size_t)cnt * ne * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  MDX_HIP(hipStreamSynchronize(c->stream));
+  *h_count = cnt;
+  return 0;
+}
+
+__global__ void k_count_pairs(int n, int cap, const int* __restrict__ nbr_count, const unsigned* __restrict__ nbr_list,
+                              const float4* __restrict__ xq, float Lx, float Ly, float Lz, float rc2,
+                              unsigned long long* __restrict__ out) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long slots = 0, inside = 0;
+  if (s < n) {
+    int cnt = nbr_count[s];
+    slots = cnt;
+    float4 xi = xq[s];
+    for (int k = 0; k < cnt; ++k) {
+      int t = (int)(nbr_list[(long long)s * cap + k] & 0x7fffffffu);
+      float dx = xq[t].x - xi.x, dy = xq[t].y - xi.y, dz = xq[t].z - xi.z;
+      dx -= Lx * rintf(dx / Lx);
+      dy -= Ly * rintf(dy / Ly);
+      dz -= Lz * rintf(dz / Lz);
+      if (dx * dx + dy * dy + dz * dz <= rc2) inside++;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    slots += __shfl_xor(slots, off, 64);
+    inside += __shfl_xor(inside, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&out[0], slots);
+    atomicAdd(&out[1], inside);
+  }
+}
+
+extern "C" int mdx_pair_counts(mdx_ctx* c, int64_t* h_list_slots, int64_t* h_unique_in_cutoff) {
+  MDX_CHECK(c && c->nb_configured, MDX_ERR_INVALID, "mdx_pair_counts: nonbonded not configured");
+  int rc;
+  if ((rc = c->tmp64.ensure(2))) return rc;
+  MDX_HIP(hipMemsetAsync(c->tmp64.p, 0, 2 * sizeof(long long), c->stream));
+  k_count_pairs<<<div_up(c->n, 256), 256, 0, c->stream>>>(c->n, c->nbr_cap, c->nbr_count.p, c->nbr_list.p, c->xq.p,
+                                                         (float)c->box[0], (float)c->box[1], (float)c->box[2],
+                                                         (float)(c->rc * c->rc), (unsigned long long*)c->tmp64.p);
+  MDX_LAUNCH_CHECK();
+  long long h[2];
+  MDX_HIP(hipMemcpyAsync(h, c->tmp64.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  MDX_HIP(hipStreamSynchronize(c->stream));
+  *h_list_slots = h[0];
+  *h_unique_in_cutoff = h[1] / 2;
+  return 0;
+}
+
+// ------------------------------------------------------------------ kernel timing
+// Average duration (ms) of one launch of a hot kernel, timed with HIP events on the context's
+// own stream (the stream the kernel runs on). which: 0 nonbonded (LJ + erfc Coulomb, forces
+// only, on the current list), 1 full force evaluation of `terms`, 2 PME reciprocal chain,
+// 3 one integrator step of `terms`.
+extern "C" int mdx_time_kernel(mdx_ctx* c, int which, unsigned terms, int reps, double* h_avg_ms) {
+  MDX_CHECK(c && h_avg_ms && reps > 0, MDX_ERR_INVALID, "mdx_time_kernel: bad arguments");
+  MDX_ENTER();
+  int rc = 0;
+  bool lj = (terms & MDX_TERM_LJ) != 0, cd = (terms & MDX_TERM_COUL_DIRECT) != 0;
+  if (which == 0) {  // make sure the list is current
+    MDX_CHECK(c->nb_configured && (lj || cd), MDX_ERR_INVALID, "nonbonded timing needs LJ or direct Coulomb");
+    if ((rc = mdx_nlist_update(c, false)) || (rc = mdx_enqueue_nonbonded(c, lj, cd, true))) MDX_LEAVE_RET(rc);
+  }
+  hipEvent_t e0, e1;
+  MDX_HIP(hipEventCreate(&e0));
+  MDX_HIP(hipEventCreate(&e1));
+  MDX_HIP(hipEventRecord(e0, c->stream));
+  for (int k = 0; k < reps && !rc; ++k) {
+    if (which == 0) rc = mdx_enqueue_nonbonded(c, lj, cd, false);
+    else if (which == 1) {
+      rc = mdx_enqueue_zero_accumulators(c);
+      if (!rc) rc = mdx_enqueue_forces(c, terms, true);
+    } else if (which == 2) rc = mdx_pme_enqueue(c, true);
+    else if (which == 3) rc = mdx_integrator_enqueue_step(c, terms, false);
+    else rc = MDX_ERR_INVALID;
+  }
+  MDX_HIP(hipEventRecord(e1, c->stream));
+  MDX_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  MDX_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *h_avg_ms = ms / reps;
+  MDX_LEAVE_RET(rc);
+}

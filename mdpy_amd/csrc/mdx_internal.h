@@ -1,0 +1,244 @@
+// Internal declarations shared by the libmdx translation units (gfx950 / CDNA4 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include <cstring>
+#include "../../include/mdx.h"
+
+// ------------------------------------------------------------------ fixed point
+// Forces and energies are accumulated as int64 with 40 fractional bits: integer addition is
+// associative, so every sum is bitwise reproducible whatever the order of the atomics.
+// Range +-8.4e6 internal units (forces ~1e-2, energies ~1e0 in practice), resolution 9.1e-13.
+#define MDX_FIX_SHIFT 40
+static constexpr double kFixScale = 1099511627776.0;          // 2^40
+static constexpr double kFixInv = 1.0 / 1099511627776.0;
+
+__device__ __forceinline__ long long to_fix(double v) { return __double2ll_rn(v * kFixScale); }
+__device__ __forceinline__ long long to_fix_f(float v) { return __double2ll_rn((double)v * kFixScale); }
+
+// status flag bits (device word)
+#define MDX_FLAG_OVERFLOW 1u
+#define MDX_FLAG_LOSS 2u
+#define MDX_FLAG_NONFINITE 4u
+
+// ------------------------------------------------------------------ error handling
+void mdx_set_error(const char* fmt, ...);
+#define MDX_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (call);                                                            \
+    if (_e != hipSuccess) {                                                            \
+      mdx_set_error("%s failed at %s:%d: %s", #call, __FILE__, __LINE__,               \
+                    hipGetErrorString(_e));                                            \
+      return MDX_ERR_HIP;                                                              \
+    }                                                                                  \
+  } while (0)
+#define MDX_LAUNCH_CHECK()                                                             \
+  do {                                                                                 \
+    hipError_t _e = hipGetLastError();                                                 \
+    if (_e != hipSuccess) {                                                            \
+      mdx_set_error("kernel launch failed at %s:%d: %s", __FILE__, __LINE__,           \
+                    hipGetErrorString(_e));                                            \
+      return MDX_ERR_HIP;                                                              \
+    }                                                                                  \
+  } while (0)
+#define MDX_CHECK(cond, code, ...)                                                     \
+  do {                                                                                 \
+    if (!(cond)) {                                                                     \
+      mdx_set_error(__VA_ARGS__);                                                      \
+      return code;                                                                     \
+    }                                                                                  \
+  } while (0)
+
+// ------------------------------------------------------------------ device buffers
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  int ensure(size_t count) {  // grow-only
+    if (count <= n && p) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (count == 0) return 0;
+    if (hipMalloc((void**)&p, count * sizeof(T)) != hipSuccess) {
+      mdx_set_error("hipMalloc of %zu bytes failed", count * sizeof(T));
+      return MDX_ERR_HIP;
+    }
+    n = count;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// Per-context counters and flags kept in one small device block (zeroed at creation).
+struct DevState {
+  unsigned flags;           // MDX_FLAG_*
+  int rebuild;              // 1 when the neighbour list must be rebuilt before use
+  unsigned long long max_disp2_bits;  // max squared displacement since build (double bits)
+  long long step;           // integrator step counter (drives the Philox stream)
+  long long nrebuild;       // number of list rebuilds
+  long long maxcount;       // largest neighbour count of the last build
+  unsigned long long minpos[3];  // order-preserving encoded float64 minima (reference binning)
+  int maxpc;                // largest cell population (reference binning)
+  int pad;
+  long long hist_base;      // step counter value at the start of the current integrate call
+};
+
+struct mdx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // ---- system (borrowed)
+  int n = 0;
+  double* pos = nullptr;
+  double* vel = nullptr;
+  double box[3] = {0, 0, 0};
+  // ---- per-atom data (owned)
+  DevBuf<double> inv_mass, mass, charge64;
+  DevBuf<float> charge_s;         // q * sqrt(ke), float
+  DevBuf<double> lj64;            // N x 4 (eps, sigma, eps14, sigma14)
+  DevBuf<int> excl_ptr, excl_idx, s14_ptr, s14_idx;
+  int max_excl = 0, max_s14 = 0;
+  bool has_excl = false, has_s14 = false;
+  std::vector<int> h_excl_ptr, h_excl_idx;
+  double qsum = 0, q2sum = 0;
+  bool has_charges = false, has_lj = false, has_mass = false;
+  // ---- bonded (owned)
+  int nb = 0, na = 0, nd = 0, ni = 0;
+  DevBuf<int> bond_idx, angle_idx, dih_idx, imp_idx;
+  DevBuf<double> bond_par, angle_par, dih_par, imp_par;
+  int dih_refcompat = 0;
+  // ---- fixed-point accumulators
+  DevBuf<long long> ffix;        // 3 x N (SoA)
+  DevBuf<long long> efix;        // MDX_NTERMS + 1
+  DevBuf<DevState> dstate;
+  // ---- reference cell list
+  DevBuf<int> ref_cell_lin;      // N (linear cell id of the last reference binning)
+  DevBuf<int> ref_counts, ref_cursor;
+  int ref_n[3] = {0, 0, 0};
+  double ref_w[3] = {0, 0, 0};
+  double ref_inv[3] = {0, 0, 0};
+  // ---- nonbonded configuration
+  bool nb_configured = false;
+  double rc = 12.0, skin = 0.0, cell_cutoff = 12.0;
+  int pair_mode = MDX_PAIRS_EXACT;
+  // list grid
+  int gn[3] = {0, 0, 0};
+  int reach[3] = {0, 0, 0};
+  int ncell = 0;
+  DevBuf<int> cell_count, cell_start, cell_cursor, atom_cell;
+  DevBuf<int> sorted_orig;       // sorted slot -> original atom
+  DevBuf<int> orig_sorted;       // original atom -> sorted slot
+  DevBuf<float4> xq;             // sorted: x, y, z (centred, float), q*sqrt(ke)
+  DevBuf<float2> ljs, ljs14;     // sorted: sigma/2, sqrt(eps)
+  DevBuf<int> ref_cell_sorted;   // sorted: packed reference cell coords (REFERENCE mode)
+  DevBuf<double> pos_at_build;   // original order, 3N
+  DevBuf<int> nbr_count;         // sorted
+  DevBuf<unsigned> nbr_list;     // column major: [k * stride + s]
+  int nbr_cap = 0;               // capacity per atom
+  int list_stride = 0;
+  bool list_valid = false;
+  std::vector<int> stencil;      // packed stencil offsets (dx,dy,dz) of the list grid
+  DevBuf<int> d_stencil;
+  // ---- PME
+  bool pme_configured = false;
+  int K[3] = {0, 0, 0};
+  int order = 4;
+  double beta = 0.0;
+  DevBuf<long long> qfix;        // spread grid (fixed point)
+  DevBuf<float2> cgrid;          // complex work grid
+  DevBuf<float> conv;            // real convolved grid
+  DevBuf<float> bc;              // influence function B*C (float)
+  DevBuf<float2> twiddle;        // per axis, K entries each (3 * maxK)
+  DevBuf<int> excl_pairs;        // unique excluded pairs (i<j), 2 x n_excl_pairs
+  int n_excl_pairs = 0;
+  // ---- integrator
+  bool int_configured = false;
+  int int_kind = MDX_INT_VELOCITY_VERLET;
+  double dt = 1.0, gamma = 0.0, kT = 0.0;
+  uint64_t seed = 0;
+  bool int_primed = false;       // history arrays valid
+  DevBuf<double> xa, xb, va, vb, acc_prev, acc_cur;   // integrator history (3N each)
+  const double* ext_noise = nullptr;
+  int ext_noise_steps = 0;
+  long long ext_noise_base = 0;
+  DevBuf<double> ehist;          // per-step energies
+  int ehist_cap = 0;
+  int ehist_count = 0;
+  long long evals = 0;
+  // graph
+  hipGraphExec_t graph_exec = nullptr;
+  unsigned graph_terms = 0;
+  int graph_steps = 0;
+  int graph_kind = -1;
+  int graph_every = 0;
+  int energy_every = 1;          // potential energies every this many steps during integrate
+  // scratch
+  DevBuf<long long> tmp64;
+};
+
+// ------------------------------------------------------------------ shared helpers
+__device__ __forceinline__ unsigned long long enc_double_order(double v) {
+  unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double dec_double_order(unsigned long long e) {
+  unsigned long long b = (e & 0x8000000000000000ull) ? (e & 0x7fffffffffffffffull) : ~e;
+  return __longlong_as_double((long long)b);
+}
+static inline unsigned long long enc_double_order_host(double v) {
+  unsigned long long b;
+  std::memcpy(&b, &v, 8);
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+static inline double dec_double_order_host(unsigned long long e) {
+  unsigned long long b = (e & 0x8000000000000000ull) ? (e & 0x7fffffffffffffffull) : ~e;
+  double v;
+  std::memcpy(&v, &b, 8);
+  return v;
+}
+
+// lanes cooperating on one atom's neighbour list (nonbonded kernel and list build)
+#define NB_G 32
+
+static inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ module entry points
+// join the caller's stream on entry / make it wait for the context stream on exit
+int mdx_enter(mdx_ctx* c);
+int mdx_leave(mdx_ctx* c);
+#define MDX_ENTER()                   \
+  do {                                \
+    int _r = mdx_enter(c);            \
+    if (_r) return _r;                \
+  } while (0)
+#define MDX_LEAVE_RET(v)              \
+  do {                                \
+    int _v = (v);                     \
+    int _r = mdx_leave(c);            \
+    return _v ? _v : _r;              \
+  } while (0)
+// celllist / neighbour list
+int mdx_nlist_prepare(mdx_ctx* c);                    // allocate grid/list buffers for current config
+int mdx_nlist_update(mdx_ctx* c, bool force_rebuild);  // enqueue displacement check + conditional rebuild + pack
+int mdx_nlist_enqueue_pack(mdx_ctx* c);
+int mdx_nlist_enqueue_clear(mdx_ctx* c);
+int mdx_enqueue_request_rebuild(mdx_ctx* c);
+// forces
+int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy);
+int mdx_enqueue_coulomb_plain(mdx_ctx* c);
+int mdx_pme_enqueue(mdx_ctx* c, bool energy);
+int mdx_pme_enqueue_corrections(mdx_ctx* c);
+int mdx_bonded_enqueue(mdx_ctx* c, unsigned terms);
+int mdx_enqueue_zero_accumulators(mdx_ctx* c);
+int mdx_enqueue_forces_out(mdx_ctx* c, double* d_out);
+int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy);
+// integrator
+int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy);
+int mdx_integrator_prime(mdx_ctx* c, unsigned terms);
+int mdx_integrator_finish(mdx_ctx* c);

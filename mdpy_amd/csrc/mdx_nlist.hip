@@ -1,0 +1,880 @@
+// Wrapping, mdpy-compatible cell binning, and the per-atom neighbour list (gfx950).
+//
+// Reference behaviour restated here:
+//   wrap_positions            mdpy/utils/pbc.py:28-36
+//   CellList grid + binning   mdpy/core/cell_list.py:56-116
+//   27-cell stencil           mdpy/constraint/__init__.py:12-19 and
+//                             mdpy/constraint/charmm_nonbonded_constraint.py:75-100
+// The force kernels do not walk the reference's dense cell table: atoms are binned into a
+// periodic grid of cells at least (rc + skin)/2 wide, sorted by cell (ascending original id
+// inside a cell, so every order is deterministic), and each atom gets a full neighbour list
+// (column-major, one lane per atom in the force kernel). In REFERENCE mode the list also drops
+// pairs whose mdpy cells are not stencil neighbours, reproducing the reference's pair set.
+#include "mdx_internal.h"
+#include <algorithm>
+#include <cmath>
+
+#define NL_BLOCK 256
+#define SORT_MAX 1024
+
+// ============================================================== wrap / reference binning
+__global__ void k_wrap(double* __restrict__ pos, int n, double Lx, double Ly, double Lz, double ix,
+                       double iy, double iz, DevState* st) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double L[3] = {Lx, Ly, Lz}, inv[3] = {ix, iy, iz};
+  bool lost = false;
+  double out[3];
+  for (int d = 0; d < 3; ++d) {
+    double x = pos[3 * i + d];
+    double m = -rint(x * inv[d]);  // np.round: half to even
+    if (fabs(m) >= 2.0) lost = true;
+    out[d] = x + m * L[d];
+  }
+  if (lost) {
+    atomicOr(&st->flags, MDX_FLAG_LOSS);
+    return;
+  }
+  for (int d = 0; d < 3; ++d) pos[3 * i + d] = out[d];
+}
+
+__global__ void k_minpos(const double* __restrict__ pos, int n, DevState* st) {
+  __shared__ unsigned long long sm[3][NL_BLOCK / 64];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long e[3] = {~0ull, ~0ull, ~0ull};
+  if (i < n)
+    for (int d = 0; d < 3; ++d) e[d] = enc_double_order(pos[3 * i + d]);
+  for (int d = 0; d < 3; ++d) {
+    unsigned long long v = e[d];
+    for (int off = 32; off > 0; off >>= 1) {
+      unsigned long long o = __shfl_xor(v, off, 64);
+      v = o < v ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0) sm[d][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    unsigned long long v = ~0ull;
+    for (int w = 0; w < NL_BLOCK / 64; ++w) v = sm[threadIdx.x][w] < v ? sm[threadIdx.x][w] : v;
+    atomicMin(&st->minpos[threadIdx.x], v);
+  }
+}
+
+__global__ void k_ref_bin(const double* __restrict__ pos, int n, int nx, int ny, int nz, double ix,
+                          double iy, double iz, DevState* st, int* __restrict__ cell_index,
+                          int* __restrict__ cell_lin, int* __restrict__ counts) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double mn[3] = {dec_double_order(st->minpos[0]), dec_double_order(st->minpos[1]),
+                  dec_double_order(st->minpos[2])};
+  double inv[3] = {ix, iy, iz};
+  int nn[3] = {nx, ny, nz};
+  int c[3];
+  for (int d = 0; d < 3; ++d) {
+    double s = (pos[3 * i + d] - mn[d]) * inv[d];  // (x - min x) . cell_inv, diagonal
+    int k = (int)floor(s);
+    if (k < 0 || k >= nn[d]) {
+      atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
+      k = k < 0 ? 0 : nn[d] - 1;
+    }
+    c[d] = k;
+    if (cell_index) cell_index[3 * i + d] = k;
+  }
+  int lin = (c[0] * ny + c[1]) * nz + c[2];
+  cell_lin[i] = lin;
+  int cnt = atomicAdd(&counts[lin], 1) + 1;
+  atomicMax(&st->maxpc, cnt);
+}
+
+__global__ void k_ref_scatter(const int* __restrict__ cell_lin, int n, int* __restrict__ cursor,
+                              int* __restrict__ table, int maxpc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = cell_lin[i];
+  int slot = atomicAdd(&cursor[c], 1);
+  if (slot < maxpc) table[(long long)c * maxpc + slot] = i;
+}
+
+// Sort each row ascending (bitonic sort in LDS); entries >= valid are padding -1.
+__global__ void k_sort_rows(int* __restrict__ table, const int* __restrict__ counts, int width) {
+  __shared__ int buf[SORT_MAX];
+  int row = blockIdx.x;
+  int cnt = counts[row];
+  int P = 1;
+  while (P < cnt) P <<= 1;
+  for (int t = threadIdx.x; t < P; t += blockDim.x)
+    buf[t] = t < cnt ? table[(long long)row * width + t] : 0x7fffffff;
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < P; t += blockDim.x) {
+        int u = t ^ j;
+        if (u > t) {
+          bool up = (t & k) == 0;
+          int a = buf[t], b = buf[u];
+          if ((a > b) == up) {
+            buf[t] = b;
+            buf[u] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int t = threadIdx.x; t < width; t += blockDim.x)
+    table[(long long)row * width + t] = t < cnt ? buf[t] : -1;
+}
+
+__global__ void k_fill_i32(int* __restrict__ p, int v, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// reset the reference-binning reductions (minimum position per axis, largest population)
+__global__ void k_ref_reset(DevState* st, int* __restrict__ counts, int nc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nc) counts[i] = 0;
+  if (i == 0) {
+    st->minpos[0] = st->minpos[1] = st->minpos[2] = ~0ull;
+    st->maxpc = 0;
+  }
+}
+
+extern "C" int mdx_wrap_positions(mdx_ctx* c) {
+  MDX_CHECK(c && c->pos, MDX_ERR_INVALID, "mdx_wrap_positions: no system bound");
+  MDX_ENTER();
+  double inv[3];
+  for (int d = 0; d < 3; ++d) inv[d] = 1.0 / c->box[d];
+  MDX_HIP(hipMemsetAsync(&c->dstate.p->flags, 0, sizeof(unsigned), c->stream));
+  k_wrap<<<div_up(c->n, NL_BLOCK), NL_BLOCK, 0, c->stream>>>(c->pos, c->n, c->box[0], c->box[1], c->box[2],
+                                                            inv[0], inv[1], inv[2], c->dstate.p);
+  MDX_LAUNCH_CHECK();
+  unsigned flags = 0;
+  MDX_HIP(hipMemcpyAsync(&flags, &c->dstate.p->flags, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+  MDX_HIP(hipStreamSynchronize(c->stream));
+  int h1 = 1;
+  MDX_HIP(hipMemcpy(&c->dstate.p->rebuild, &h1, sizeof(int), hipMemcpyHostToDevice));
+  MDX_CHECK(!(flags & MDX_FLAG_LOSS), MDX_ERR_PARTICLE_LOSS, "atom(s) moved beyond 2 PBC images");
+  MDX_LEAVE_RET(0);
+}
+
+// Reference grid for a cell-list cutoff (cell_list.py:74-81).
+static void ref_grid(const double* box, double rc, int* n, double* w, double* inv) {
+  for (int d = 0; d < 3; ++d) {
+    int k = (int)std::floor((box[d] + 2.0) / rc);
+    if (k < 3) k = 3;
+    n[d] = k;
+    w[d] = (box[d] + 2.0) / (double)k;
+    inv[d] = 1.0 / w[d];
+  }
+}
+
+static int ref_bin_enqueue(mdx_ctx* c, double cell_cutoff, int* d_cell_index) {
+  ref_grid(c->box, cell_cutoff, c->ref_n, c->ref_w, c->ref_inv);
+  int nc = c->ref_n[0] * c->ref_n[1] * c->ref_n[2];
+  int rc;
+  if ((rc = c->ref_cell_lin.ensure(c->n))) return rc;
+  if ((rc = c->ref_counts.ensure(nc))) return rc;
+  if ((rc = c->ref_cursor.ensure(nc))) return rc;
+  k_ref_reset<<<div_up(nc, NL_BLOCK), NL_BLOCK, 0, c->stream>>>(c->dstate.p, c->ref_counts.p, nc);
+  MDX_LAUNCH_CHECK();
+  k_minpos<<<div_up(c->n, NL_BLOCK), NL_BLOCK, 0, c->stream>>>(c->pos, c->n, c->dstate.p);
+  MDX_LAUNCH_CHECK();
+  k_ref_bin<<<div_up(c->n, NL_BLOCK), NL_BLOCK, 0, c->stream>>>(
+      c->pos, c->n, c->ref_n[0], c->ref_n[1], c->ref_n[2], c->ref_inv[0], c->ref_inv[1], c->ref_inv[2],
+      c->dstate.p, d_cell_index, c->ref_cell_lin.p, c->ref_counts.p);
+  MDX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int mdx_celllist_reference(mdx_ctx* c, double cell_cutoff, int* d_cell_index, int* h_n, int* h_maxpc) {
+  MDX_CHECK(c && c->pos, MDX_ERR_INVALID, "mdx_celllist_reference: no system bound");
+  MDX_CHECK(cell_cutoff > 0, MDX_ERR_CELL_CUTOFF, "cutoff radius is poor defined");
+  for (int d = 0; d < 3; ++d)
+    MDX_CHECK(std::floor(c->box[d] / cell_cutoff) >= 2, MDX_ERR_CELL_CUTOFF,
+              "the cutoff_radius is too large to create cell list");
+  MDX_ENTER();
+  MDX_HIP(hipMemsetAsync(&c->dstate.p->flags, 0, sizeof(unsigned), c->stream));
+  int rc = ref_bin_enqueue(c, cell_cutoff, d_cell_index);
+  if (rc) return rc;
+  int maxpc = 0;
+  unsigned flags = 0;
+  MDX_HIP(hipMemcpyAsync(&maxpc, &c->dstate.p->maxpc, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  MDX_HIP(hipMemcpyAsync(&flags, &c->dstate.p->flags, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+  MDX_HIP(hipStreamSynchronize(c->stream));
+  MDX_CHECK(!(flags & MDX_FLAG_OVERFLOW), MDX_ERR_OVERFLOW, "reference binning produced an out-of-range cell");
+  for (int d = 0; d < 3; ++d) h_n[d] = c->ref_n[d];
+  *h_maxpc = maxpc;
+  MDX_LEAVE_RET(0);
+}
+
+extern "C" int mdx_celllist_table(mdx_ctx* c, int* d_table, int maxpc) {
+  MDX_CHECK(c && d_table && maxpc > 0, MDX_ERR_INVALID, "mdx_celllist_table: bad arguments");
+  MDX_CHECK(maxpc <= SORT_MAX * 8, MDX_ERR_OVERFLOW, "cell population %d too large", maxpc);
+  int nc = c->ref_n[0] * c->ref_n[1] * c->ref_n[2];
+  MDX_CHECK(nc > 0, MDX_ERR_INVALID, "mdx_celllist_table before mdx_celllist_reference");
+  MDX_CHECK(maxpc <= SORT_MAX, MDX_ERR_OVERFLOW, "cell population %d exceeds %d", maxpc, SORT_MAX);
+  MDX_ENTER();
+  MDX_HIP(hipMemsetAsync(c->ref_cursor.p, 0, nc * sizeof(int), c->stream));
+  MDX_HIP(hipMemsetAsync(d_table, 0xff, (size_t)nc * maxpc * sizeof(int), c->stream));
+  k_ref_scatter<<<div_up(c->n, NL_BLOCK), NL_BLOCK, 0, c->stream>>>(c->ref_cell_lin.p, c->n, c->ref_cursor.p,
+                                                                   d_table, maxpc);
+  MDX_LAUNCH_CHECK();
+  k_sort_rows<<<nc, 256, 0, c->stream>>>(d_table, c->ref_counts.p, maxpc);
+  MDX_LAUNCH_CHECK();
+  MDX_LEAVE_RET(0);
+}
+
+// ============================================================== neighbour-list grid
+// decision shared by every rebuild kernel: rebuild requested, or an atom moved > skin/2
+__device__ __forceinline__ bool need_rebuild(const DevState* st, double thr2) {
+  return st->rebuild != 0 || dec_double_order(st->max_disp2_bits) > thr2;
+}
+
+__global__ void k_check_disp(const double* __restrict__ pos, const double* __restrict__ pos_build, int n,
+                             double Lx, double Ly, double Lz, DevState* st) {
+  __shared__ double sm[NL_BLOCK / 64];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  bool bad = false;
+  if (i < n) {
+    double L[3] = {Lx, Ly, Lz};
+    for (int d = 0; d < 3; ++d) {
+      double x = pos[3 * i + d];
+      if (!isfinite(x)) bad = true;
+      double dd = x - pos_build[3 * i + d];
+      dd -= L[d] * rint(dd / L[d]);
+      d2 += dd * dd;
+    }
+  }
+  if (bad) atomicOr(&st->flags, MDX_FLAG_NONFINITE);
+  for (int off = 32; off > 0; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off, 64));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = d2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0;
+    for (int w = 0; w < NL_BLOCK / 64; ++w) v = fmax(v, sm[w]);
+    atomicMax(&st->max_disp2_bits, enc_double_order(v));
+  }
+}
+
+struct GridArgs {
+  int nx, ny, nz;
+  double Lx, Ly, Lz;
+  double wx, wy, wz;  // cell widths
+};
+
+__global__ void k_grid_bin(const double* __restrict__ pos, int n, GridArgs g, const DevState* st, double thr2,
+                           int* __restrict__ atom_cell, int* __restrict__ cell_count) {
+  if (!need_rebuild(st, thr2)) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double L[3] = {g.Lx, g.Ly, g.Lz}, w[3] = {g.wx, g.wy, g.wz};
+  int nn[3] = {g.nx, g.ny, g.nz};
+  int c[3];
+  for (int d = 0; d < 3; ++d) {
+    double x = pos[3 * i + d];
+    x -= L[d] * floor(x / L[d]);
+    int k = (int)(x / w[d]);
+    c[d] = k < 0 ? 0 : (k >= nn[d] ? nn[d] - 1 : k);
+  }
+  int lin = (c[0] * g.ny + c[1]) * g.nz + c[2];
+  atom_cell[i] = lin;
+  atomicAdd(&cell_count[lin], 1);
+}
+
+// single-block exclusive scan over cells; also resets cursors and checks the sort limit
+__global__ void k_cell_scan(const int* __restrict__ cell_count, int* __restrict__ cell_start,
+                            int* __restrict__ cursor, int ncell, const DevState* st_c, DevState* st, double thr2) {
+  if (!need_rebuild(st_c, thr2)) return;
+  __shared__ int part[1024];
+  int per = (ncell + blockDim.x - 1) / blockDim.x;
+  int b = threadIdx.x * per, e = min(ncell, b + per);
+  int s = 0;
+  for (int k = b; k < e; ++k) {
+    int v = cell_count[k];
+    if (v > SORT_MAX) atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
+    s += v;
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int t = 0; t < (int)blockDim.x; ++t) {
+      int v = part[t];
+      part[t] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  int acc = part[threadIdx.x];
+  for (int k = b; k < e; ++k) {
+    cell_start[k] = acc;
+    cursor[k] = 0;
+    acc += cell_count[k];
+  }
+}
+
+__global__ void k_grid_scatter(const int* __restrict__ atom_cell, int n, const int* __restrict__ cell_start,
+                               int* __restrict__ cursor, int* __restrict__ sorted_orig, const DevState* st,
+                               double thr2) {
+  if (!need_rebuild(st, thr2)) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = atom_cell[i];
+  int slot = atomicAdd(&cursor[c], 1);
+  sorted_orig[cell_start[c] + slot] = i;
+}
+
+// per cell: sort members ascending (deterministic order), then fill the per-slot arrays
+__global__ void k_grid_sort_fill(const int* __restrict__ cell_start, const int* __restrict__ cell_count,
+                                 int* __restrict__ sorted_orig, int* __restrict__ orig_sorted,
+                                 const double* __restrict__ pos, double* __restrict__ pos_build,
+                                 const double* __restrict__ lj64, float2* __restrict__ ljs,
+                                 float2* __restrict__ ljs14, const int* __restrict__ ref_lin,
+                                 int* __restrict__ ref_cell_sorted, int rny, int rnz, const DevState* st,
+                                 double thr2) {
+  if (!need_rebuild(st, thr2)) return;
+  __shared__ int buf[SORT_MAX];
+  int cell = blockIdx.x;
+  int start = cell_start[cell], cnt = cell_count[cell];
+  if (cnt > SORT_MAX) return;  // flagged in k_cell_scan
+  int P = 1;
+  while (P < cnt) P <<= 1;
+  for (int t = threadIdx.x; t < P; t += blockDim.x) buf[t] = t < cnt ? sorted_orig[start + t] : 0x7fffffff;
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < P; t += blockDim.x) {
+        int u = t ^ j;
+        if (u > t) {
+          bool up = (t & k) == 0;
+          int a = buf[t], b = buf[u];
+          if ((a > b) == up) {
+            buf[t] = b;
+            buf[u] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+    int o = buf[t];
+    int s = start + t;
+    sorted_orig[s] = o;
+    orig_sorted[o] = s;
+    for (int d = 0; d < 3; ++d) pos_build[3 * o + d] = pos[3 * o + d];
+    if (lj64) {
+      double e = lj64[4 * o + 0], sg = lj64[4 * o + 1], e14 = lj64[4 * o + 2], s14 = lj64[4 * o + 3];
+      ljs[s] = make_float2((float)(0.5 * sg), (float)sqrt(e));
+      ljs14[s] = make_float2((float)(0.5 * s14), (float)sqrt(e14));
+    }
+    if (ref_lin) {
+      int lin = ref_lin[o];
+      int cz = lin % rnz, cy = (lin / rnz) % rny, cx = lin / (rnz * rny);
+      ref_cell_sorted[s] = (cx << 20) | (cy << 10) | cz;
+    }
+  }
+}
+
+// every evaluation: float32 positions (wrapped, centred) and scaled charges in sorted order
+__global__ void k_pack(const double* __restrict__ pos, const int* __restrict__ sorted_orig,
+                       const float* __restrict__ qs, int n, double Lx, double Ly, double Lz,
+                       float4* __restrict__ xq) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int o = sorted_orig[s];
+  double x = pos[3 * o + 0], y = pos[3 * o + 1], z = pos[3 * o + 2];
+  x -= Lx * floor(x / Lx);
+  y -= Ly * floor(y / Ly);
+  z -= Lz * floor(z / Lz);
+  xq[s] = make_float4((float)(x - 0.5 * Lx), (float)(y - 0.5 * Ly), (float)(z - 0.5 * Lz), qs ? qs[o] : 0.f);
+}
+
+struct ListArgs {
+  GridArgs g;
+  int n;
+  int cap;
+  float rl2;          // list radius squared (float pre-filter)
+  int exact64;        // decide r <= rc in float64 (skin == 0)
+  double rc;
+  int ref_mode;       // REFERENCE stencil filter
+  int rnx, rny, rnz;  // reference grid
+  int nstencil;
+};
+
+__device__ __forceinline__ int cyc_dist(int a, int b, int n) {
+  int d = a > b ? a - b : b - a;
+  return min(d, n - d);
+}
+
+// NB_G lanes build one atom's list: for every stencil cell the group scans the cell's atoms
+// NB_G at a time (contiguous loads), and a wave ballot turns the accepted lanes into
+// consecutive, deterministic slots of the atom's row-major list.
+__global__ __launch_bounds__(NL_BLOCK) void k_build_list(
+    ListArgs a, const float4* __restrict__ xq, const int* __restrict__ sorted_orig,
+    const int* __restrict__ cell_start, const int* __restrict__ cell_count, const int* __restrict__ stencil,
+    const int* __restrict__ excl_ptr, const int* __restrict__ excl_idx, const int* __restrict__ s14_ptr,
+    const int* __restrict__ s14_idx, const int* __restrict__ ref_cell_sorted, const double* __restrict__ pos,
+    int* __restrict__ nbr_count, unsigned* __restrict__ nbr_list, DevState* st, double thr2) {
+  if (!need_rebuild(st, thr2)) return;
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = gt / NB_G;
+  const int g = gt & (NB_G - 1);
+  if (s >= a.n) return;  // whole groups leave together
+  const int lane = threadIdx.x & 63;
+  const unsigned long long gmask = (NB_G == 64 ? ~0ull : ((1ull << NB_G) - 1ull)) << (lane & ~(NB_G - 1));
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const GridArgs& gr = a.g;
+  const int o = sorted_orig[s];
+  const float4 xi = xq[s];
+  const float Lx = (float)gr.Lx, Ly = (float)gr.Ly, Lz = (float)gr.Lz;
+  const float ix = 1.0f / Lx, iy = 1.0f / Ly, iz = 1.0f / Lz;
+  double px = pos[3 * o + 0], py = pos[3 * o + 1], pz = pos[3 * o + 2];
+  double wx = px - gr.Lx * floor(px / gr.Lx), wy = py - gr.Ly * floor(py / gr.Ly), wz = pz - gr.Lz * floor(pz / gr.Lz);
+  int cx = min(gr.nx - 1, (int)(wx / gr.wx)), cy = min(gr.ny - 1, (int)(wy / gr.wy)), cz = min(gr.nz - 1, (int)(wz / gr.wz));
+  const int e0 = excl_ptr ? excl_ptr[o] : 0, e1 = excl_ptr ? excl_ptr[o + 1] : 0;
+  const int t0s = s14_ptr ? s14_ptr[o] : 0, t1s = s14_ptr ? s14_ptr[o + 1] : 0;
+  const int rci = a.ref_mode ? ref_cell_sorted[s] : 0;
+  unsigned* row = nbr_list + (long long)s * a.cap;
+  // the first few excluded / 1-4 partners live in registers (water: 2 and 0)
+  constexpr int NREG = 6;
+  int exr[NREG], s4r[NREG];
+#pragma unroll
+  for (int q = 0; q < NREG; ++q) {
+    exr[q] = e0 + q < e1 ? excl_idx[e0 + q] : -1;
+    s4r[q] = t0s + q < t1s ? s14_idx[t0s + q] : -1;
+  }
+  int cnt = 0;
+  for (int k = 0; k < a.nstencil; ++k) {
+    int p = stencil[k];
+    int dx = ((p >> 16) & 0xff) - 128, dy = ((p >> 8) & 0xff) - 128, dz = (p & 0xff) - 128;
+    int nx = cx + dx, ny = cy + dy, nz = cz + dz;
+    nx += nx < 0 ? gr.nx : (nx >= gr.nx ? -gr.nx : 0);
+    ny += ny < 0 ? gr.ny : (ny >= gr.ny ? -gr.ny : 0);
+    nz += nz < 0 ? gr.nz : (nz >= gr.nz ? -gr.nz : 0);
+    const int cell = (nx * gr.ny + ny) * gr.nz + nz;
+    const int b = cell_start[cell], e = b + cell_count[cell];
+    for (int tb = b; tb < e; tb += NB_G) {
+      const int t = tb + g;
+      bool acc = false, s14 = false;
+      if (t < e && t != s) {
+        const float4 xj = xq[t];
+        const int jo = sorted_orig[t];
+        float ddx = xj.x - xi.x, ddy = xj.y - xi.y, ddz = xj.z - xi.z;
+        ddx -= Lx * rintf(ddx * ix);
+        ddy -= Ly * rintf(ddy * iy);
+        ddz -= Lz * rintf(ddz * iz);
+        float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        if (r2 < a.rl2) {
+          acc = true;
+          if (a.exact64) {  // float64 decision with the reference's sequence (pbc.py:40-44)
+            double d[3];
+            double L[3] = {gr.Lx, gr.Ly, gr.Lz};
+            for (int q = 0; q < 3; ++q) {
+              double v = pos[3 * jo + q] - pos[3 * o + q];
+              double sv = v * (1.0 / L[q]);
+              sv -= rint(sv);
+              d[q] = sv * L[q];
+            }
+            acc = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) <= a.rc;
+          }
+#pragma unroll
+          for (int q = 0; q < NREG; ++q) acc = acc && (exr[q] != jo);
+          for (int q = e0 + NREG; q < e1; ++q) acc = acc && (excl_idx[q] != jo);
+          if (acc && a.ref_mode) {
+            int rcj = ref_cell_sorted[t];
+            int ax = cyc_dist(rci >> 20, rcj >> 20, a.rnx);
+            int ay = cyc_dist((rci >> 10) & 1023, (rcj >> 10) & 1023, a.rny);
+            int az = cyc_dist(rci & 1023, rcj & 1023, a.rnz);
+            acc = ax <= 1 && ay <= 1 && az <= 1;
+          }
+          if (acc) {
+#pragma unroll
+            for (int q = 0; q < NREG; ++q) s14 |= (s4r[q] == jo);
+            for (int q = t0s + NREG; q < t1s; ++q) s14 |= (s14_idx[q] == jo);
+          }
+        }
+      }
+      const unsigned long long m = __ballot(acc) & gmask;
+      const int slot = cnt + __popcll(m & below);
+      if (acc && slot < a.cap) row[slot] = (unsigned)t | (s14 ? 0x80000000u : 0u);
+      cnt += __popcll(m);
+    }
+  }
+  if (g == 0) {
+    nbr_count[s] = min(cnt, a.cap);
+    if (cnt > a.cap) atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
+  }
+}
+
+// One workgroup per home cell: the candidates of every stencil cell (identical for all atoms of
+// the home cell) are staged in LDS in chunks, then each NB_G-lane group scans them for one home
+// atom at a time and compacts the accepted ones with a wave ballot. Same candidate order, hence
+// the same lists, as a per-atom walk of the stencil; global memory is read once per cell.
+#define BL_CH 1536
+#define BL_MAXST 512
+__global__ __launch_bounds__(NL_BLOCK) void k_build_list_cell(
+    ListArgs a, const float4* __restrict__ xq, const int* __restrict__ sorted_orig,
+    const int* __restrict__ cell_start, const int* __restrict__ cell_count, const int* __restrict__ stencil,
+    const int* __restrict__ excl_ptr, const int* __restrict__ excl_idx, const int* __restrict__ s14_ptr,
+    const int* __restrict__ s14_idx, const int* __restrict__ ref_cell_sorted, const double* __restrict__ pos,
+    int* __restrict__ nbr_count, unsigned* __restrict__ nbr_list, DevState* st, double thr2) {
+  if (!need_rebuild(st, thr2)) return;
+  __shared__ float4 cxq[BL_CH];
+  __shared__ int ct[BL_CH], co[BL_CH], cr[BL_CH];
+  __shared__ int sb[BL_MAXST], sp[BL_MAXST + 1];
+  __shared__ int acnt[SORT_MAX];
+  const GridArgs& gr = a.g;
+  const int cell = blockIdx.x;
+  const int hb = cell_start[cell], hn = cell_count[cell];
+  if (hn == 0 || hn > SORT_MAX) return;  // oversize cells are flagged by k_cell_scan
+  const int tid = threadIdx.x;
+  const int hz = cell % gr.nz, hy = (cell / gr.nz) % gr.ny, hx = cell / (gr.nz * gr.ny);
+  for (int k = tid; k < a.nstencil; k += blockDim.x) {
+    int p = stencil[k];
+    int nx = hx + ((p >> 16) & 0xff) - 128, ny = hy + ((p >> 8) & 0xff) - 128, nz = hz + (p & 0xff) - 128;
+    nx += nx < 0 ? gr.nx : (nx >= gr.nx ? -gr.nx : 0);
+    ny += ny < 0 ? gr.ny : (ny >= gr.ny ? -gr.ny : 0);
+    nz += nz < 0 ? gr.nz : (nz >= gr.nz ? -gr.nz : 0);
+    const int nc = (nx * gr.ny + ny) * gr.nz + nz;
+    sb[k] = cell_start[nc];
+    sp[k + 1] = cell_count[nc];
+  }
+  for (int j = tid; j < hn; j += blockDim.x) acnt[j] = 0;
+  __syncthreads();
+  if (tid == 0) {
+    sp[0] = 0;
+    for (int k = 0; k < a.nstencil; ++k) sp[k + 1] += sp[k];
+  }
+  __syncthreads();
+  const int total = sp[a.nstencil];
+  const int lane = tid & 63;
+  const int q = tid / NB_G, g = tid & (NB_G - 1);
+  const int ngroups = blockDim.x / NB_G;
+  const unsigned long long gmask = (NB_G == 64 ? ~0ull : ((1ull << NB_G) - 1ull)) << (lane & ~(NB_G - 1));
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const float Lx = (float)gr.Lx, Ly = (float)gr.Ly, Lz = (float)gr.Lz;
+  const float ix = 1.0f / Lx, iy = 1.0f / Ly, iz = 1.0f / Lz;
+  for (int base = 0; base < total; base += BL_CH) {
+    const int nch = min(BL_CH, total - base);
+    for (int c = tid; c < nch; c += blockDim.x) {
+      const int idx = base + c;
+      int lo = 0, hi = a.nstencil - 1;  // last stencil cell whose prefix <= idx
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sp[mid] <= idx) lo = mid;
+        else hi = mid - 1;
+      }
+      const int t = sb[lo] + (idx - sp[lo]);
+      cxq[c] = xq[t];
+      ct[c] = t;
+      co[c] = sorted_orig[t];
+      if (a.ref_mode) cr[c] = ref_cell_sorted[t];
+    }
+    __syncthreads();
+    for (int h = q; h < hn; h += ngroups) {
+      const int s = hb + h;
+      const int o = sorted_orig[s];
+      const float4 xi = xq[s];
+      const int e0 = excl_ptr ? excl_ptr[o] : 0, e1 = excl_ptr ? excl_ptr[o + 1] : 0;
+      const int t0s = s14_ptr ? s14_ptr[o] : 0, t1s = s14_ptr ? s14_ptr[o + 1] : 0;
+      const int rci = a.ref_mode ? ref_cell_sorted[s] : 0;
+      constexpr int NREG = 6;
+      int exr[NREG], s4r[NREG];
+#pragma unroll
+      for (int r = 0; r < NREG; ++r) {
+        exr[r] = e0 + r < e1 ? excl_idx[e0 + r] : -1;
+        s4r[r] = t0s + r < t1s ? s14_idx[t0s + r] : -1;
+      }
+      unsigned* row = nbr_list + (long long)s * a.cap;
+      int cnt = acnt[h];
+      for (int cb = 0; cb < nch; cb += NB_G) {
+        const int c = cb + g;
+        bool acc = false, s14 = false;
+        int t = 0;
+        if (c < nch) {
+          t = ct[c];
+          if (t != s) {
+            const float4 xj = cxq[c];
+            const int jo = co[c];
+            float ddx = xj.x - xi.x, ddy = xj.y - xi.y, ddz = xj.z - xi.z;
+            ddx -= Lx * rintf(ddx * ix);
+            ddy -= Ly * rintf(ddy * iy);
+            ddz -= Lz * rintf(ddz * iz);
+            const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            if (r2 < a.rl2) {
+              acc = true;
+              if (a.exact64) {  // float64 decision with the reference's sequence (pbc.py:40-44)
+                double d[3];
+                const double L[3] = {gr.Lx, gr.Ly, gr.Lz};
+                for (int r = 0; r < 3; ++r) {
+                  double v = pos[3 * jo + r] - pos[3 * o + r];
+                  double sv = v * (1.0 / L[r]);
+                  sv -= rint(sv);
+                  d[r] = sv * L[r];
+                }
+                acc = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) <= a.rc;
+              }
+#pragma unroll
+              for (int r = 0; r < NREG; ++r) acc = acc && (exr[r] != jo);
+              for (int r = e0 + NREG; r < e1; ++r) acc = acc && (excl_idx[r] != jo);
+              if (acc && a.ref_mode) {
+                const int rcj = cr[c];
+                const int ax = cyc_dist(rci >> 20, rcj >> 20, a.rnx);
+                const int ay = cyc_dist((rci >> 10) & 1023, (rcj >> 10) & 1023, a.rny);
+                const int az = cyc_dist(rci & 1023, rcj & 1023, a.rnz);
+                acc = ax <= 1 && ay <= 1 && az <= 1;
+              }
+              if (acc) {
+#pragma unroll
+                for (int r = 0; r < NREG; ++r) s14 |= (s4r[r] == jo);
+                for (int r = t0s + NREG; r < t1s; ++r) s14 |= (s14_idx[r] == jo);
+              }
+            }
+          }
+        }
+        const unsigned long long m = __ballot(acc) & gmask;
+        const int slot = cnt + __popcll(m & below);
+        if (acc && slot < a.cap) row[slot] = (unsigned)t | (s14 ? 0x80000000u : 0u);
+        cnt += __popcll(m);
+      }
+      if (g == 0) acnt[h] = cnt;
+    }
+    __syncthreads();
+  }
+  for (int h = tid; h < hn; h += blockDim.x) {
+    const int cnt = acnt[h];
+    nbr_count[hb + h] = min(cnt, a.cap);
+    if (cnt > a.cap) atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
+  }
+}
+
+__global__ void k_request_rebuild(DevState* st) { st->rebuild = 1; }
+
+int mdx_enqueue_request_rebuild(mdx_ctx* c) {
+  k_request_rebuild<<<1, 1, 0, c->stream>>>(c->dstate.p);
+  MDX_LAUNCH_CHECK();
+  return 0;
+}
+
+__global__ void k_clear_rebuild(DevState* st) {
+  st->rebuild = 0;
+  st->max_disp2_bits = 0;
+}
+
+// zero the per-cell counts of the list grid when a rebuild is due (a kernel, not a memset, so
+// the step stays safe to capture in a hipGraph) and count rebuilds
+__global__ void k_prep_rebuild(DevState* st, double thr2, int* __restrict__ cell_count, int ncell) {
+  if (!need_rebuild(st, thr2)) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ncell) cell_count[i] = 0;
+  if (i == 0) st->nrebuild += 1;
+}
+
+
+// ============================================================== host side
+static void make_grid(mdx_ctx* c, GridArgs* g) {
+  g->nx = c->gn[0];
+  g->ny = c->gn[1];
+  g->nz = c->gn[2];
+  g->Lx = c->box[0];
+  g->Ly = c->box[1];
+  g->Lz = c->box[2];
+  g->wx = c->box[0] / c->gn[0];
+  g->wy = c->box[1] / c->gn[1];
+  g->wz = c->box[2] / c->gn[2];
+}
+
+int mdx_nlist_prepare(mdx_ctx* c) {
+  double rl = c->rc + c->skin;
+  for (int d = 0; d < 3; ++d) {
+    int k = (int)std::floor(c->box[d] / (0.5 * rl));
+    if (k < 1) k = 1;
+    c->gn[d] = k;
+  }
+  c->ncell = c->gn[0] * c->gn[1] * c->gn[2];
+  // stencil: offsets whose cells can hold an atom within rl of the home cell, each distinct
+  // periodic cell once
+  c->stencil.clear();
+  int lo[3], hi[3];
+  double w[3];
+  for (int d = 0; d < 3; ++d) {
+    w[d] = c->box[d] / c->gn[d];
+    int r = (int)std::ceil(rl / w[d]);
+    if (2 * r + 1 >= c->gn[d]) {
+      lo[d] = -((c->gn[d] - 1) / 2);
+      hi[d] = c->gn[d] / 2;
+    } else {
+      lo[d] = -r;
+      hi[d] = r;
+    }
+    c->reach[d] = r;
+  }
+  for (int dx = lo[0]; dx <= hi[0]; ++dx)
+    for (int dy = lo[1]; dy <= hi[1]; ++dy)
+      for (int dz = lo[2]; dz <= hi[2]; ++dz) {
+        double mx = std::max(0, std::abs(dx) - 1) * w[0];
+        double my = std::max(0, std::abs(dy) - 1) * w[1];
+        double mz = std::max(0, std::abs(dz) - 1) * w[2];
+        if (mx * mx + my * my + mz * mz > rl * rl) continue;
+        c->stencil.push_back(((dx + 128) << 16) | ((dy + 128) << 8) | (dz + 128));
+      }
+  int rc;
+  if ((rc = c->d_stencil.ensure(c->stencil.size()))) return rc;
+  MDX_HIP(hipMemcpy(c->d_stencil.p, c->stencil.data(), c->stencil.size() * sizeof(int), hipMemcpyHostToDevice));
+  if ((rc = c->cell_count.ensure(c->ncell)) || (rc = c->cell_start.ensure(c->ncell)) ||
+      (rc = c->cell_cursor.ensure(c->ncell)) || (rc = c->atom_cell.ensure(c->n)) ||
+      (rc = c->sorted_orig.ensure(c->n)) || (rc = c->orig_sorted.ensure(c->n)) || (rc = c->xq.ensure(c->n)) ||
+      (rc = c->ljs.ensure(c->n)) || (rc = c->ljs14.ensure(c->n)) || (rc = c->ref_cell_sorted.ensure(c->n)) ||
+      (rc = c->pos_at_build.ensure(3 * (size_t)c->n)) || (rc = c->nbr_count.ensure(c->n)))
+    return rc;
+  // capacity: expected neighbours in a sphere of radius rl at the system density, with margin
+  double vol = c->box[0] * c->box[1] * c->box[2];
+  double dens = c->n / vol;
+  double expect = 4.0 / 3.0 * M_PI * rl * rl * rl * dens;
+  int cap = (int)(expect * 1.5 + 64);
+  if (cap > c->n) cap = c->n;
+  if (cap < c->nbr_cap) cap = c->nbr_cap;
+  c->nbr_cap = cap;
+  c->list_stride = cap;
+  if ((rc = c->nbr_list.ensure((size_t)cap * c->n))) return rc;
+  MDX_HIP(hipMemset(c->pos_at_build.p, 0, 3 * sizeof(double) * c->n));
+  int one = 1;
+  MDX_HIP(hipMemcpy(&c->dstate.p->rebuild, &one, sizeof(int), hipMemcpyHostToDevice));
+  c->list_valid = true;
+  return 0;
+}
+
+int mdx_nlist_enqueue_pack(mdx_ctx* c) {
+  k_pack<<<div_up(c->n, NL_BLOCK), NL_BLOCK, 0, c->stream>>>(c->pos, c->sorted_orig.p,
+                                                            c->has_charges ? c->charge_s.p : nullptr, c->n,
+                                                            c->box[0], c->box[1], c->box[2], c->xq.p);
+  MDX_LAUNCH_CHECK();
+  return 0;
+}
+
+// Enqueue: displacement check, conditional rebuild, pack. Leaves the rebuild flag set for the
+// consumer (k_clear_rebuild is enqueued by the caller after the last reader).
+int mdx_nlist_update(mdx_ctx* c, bool force_rebuild) {
+  MDX_CHECK(c->nb_configured, MDX_ERR_INVALID, "nonbonded not configured");
+  double half = 0.5 * c->skin;
+  // skin == 0 (reference pair set or no buffer): rebuild every evaluation
+  double thr2 = c->skin > 0 ? half * half : -1.0;
+  if (force_rebuild) {
+    int rc0 = mdx_enqueue_request_rebuild(c);
+    if (rc0) return rc0;
+  }
+  GridArgs g;
+  make_grid(c, &g);
+  int nb = div_up(c->n, NL_BLOCK);
+  DevState* st = c->dstate.p;
+  k_check_disp<<<nb, NL_BLOCK, 0, c->stream>>>(c->pos, c->pos_at_build.p, c->n, c->box[0], c->box[1], c->box[2], st);
+  MDX_LAUNCH_CHECK();
+  k_prep_rebuild<<<div_up(c->ncell, NL_BLOCK), NL_BLOCK, 0, c->stream>>>(st, thr2, c->cell_count.p, c->ncell);
+  MDX_LAUNCH_CHECK();
+  // reference cells are needed for the REFERENCE stencil filter; they depend on the minimum
+  // position, so REFERENCE mode runs with skin 0 (rebuild every evaluation) and re-bins here.
+  if (c->pair_mode == MDX_PAIRS_REFERENCE) {
+    int rc = ref_bin_enqueue(c, c->cell_cutoff, nullptr);
+    if (rc) return rc;
+  }
+  k_grid_bin<<<nb, NL_BLOCK, 0, c->stream>>>(c->pos, c->n, g, st, thr2, c->atom_cell.p, c->cell_count.p);
+  MDX_LAUNCH_CHECK();
+  k_cell_scan<<<1, 1024, 0, c->stream>>>(c->cell_count.p, c->cell_start.p, c->cell_cursor.p, c->ncell, st, st, thr2);
+  MDX_LAUNCH_CHECK();
+  k_grid_scatter<<<nb, NL_BLOCK, 0, c->stream>>>(c->atom_cell.p, c->n, c->cell_start.p, c->cell_cursor.p,
+                                                 c->sorted_orig.p, st, thr2);
+  MDX_LAUNCH_CHECK();
+  bool ref = c->pair_mode == MDX_PAIRS_REFERENCE;
+  k_grid_sort_fill<<<c->ncell, 256, 0, c->stream>>>(
+      c->cell_start.p, c->cell_count.p, c->sorted_orig.p, c->orig_sorted.p, c->pos, c->pos_at_build.p,
+      c->has_lj ? c->lj64.p : nullptr, c->ljs.p, c->ljs14.p, ref ? c->ref_cell_lin.p : nullptr,
+      c->ref_cell_sorted.p, c->ref_n[1], c->ref_n[2], st, thr2);
+  MDX_LAUNCH_CHECK();
+  int rc = mdx_nlist_enqueue_pack(c);
+  if (rc) return rc;
+  ListArgs a;
+  a.g = g;
+  a.n = c->n;
+  a.cap = c->nbr_cap;
+  double rl = c->rc + c->skin;
+  a.rl2 = (float)(rl * rl) * (1.0f + 1e-5f);
+  a.exact64 = c->skin > 0 ? 0 : 1;
+  a.rc = c->rc;
+  a.ref_mode = ref ? 1 : 0;
+  a.rnx = c->ref_n[0];
+  a.rny = c->ref_n[1];
+  a.rnz = c->ref_n[2];
+  a.nstencil = (int)c->stencil.size();
+  MDX_CHECK(a.nstencil <= BL_MAXST, MDX_ERR_INVALID, "stencil of %d cells exceeds %d", a.nstencil, BL_MAXST);
+  k_build_list_cell<<<c->ncell, NL_BLOCK, 0, c->stream>>>(
+      a, c->xq.p, c->sorted_orig.p, c->cell_start.p, c->cell_count.p, c->d_stencil.p,
+      c->has_excl ? c->excl_ptr.p : nullptr, c->excl_idx.p, c->has_s14 ? c->s14_ptr.p : nullptr, c->s14_idx.p,
+      c->ref_cell_sorted.p, c->pos, c->nbr_count.p, c->nbr_list.p, st, thr2);
+  MDX_LAUNCH_CHECK();
+  return 0;
+}
+
+int mdx_nlist_enqueue_clear(mdx_ctx* c) {
+  k_clear_rebuild<<<1, 1, 0, c->stream>>>(c->dstate.p);
+  MDX_LAUNCH_CHECK();
+  return 0;
+}
+
+// ---------------------------------------------------------------- pair enumeration (parity)
+__global__ void k_emit_pairs(int n, int cap, const int* __restrict__ nbr_count, const unsigned* __restrict__ nbr_list,
+                             const int* __restrict__ sorted_orig, const float4* __restrict__ xq, const double* __restrict__ pos,
+                             double Lx, double Ly, double Lz, double rc, int* __restrict__ out,
+                             long long out_cap, unsigned long long* __restrict__ counter) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int o = sorted_orig[s];
+  int cnt = nbr_count[s];
+  double L[3] = {Lx, Ly, Lz};
+  for (int k = 0; k < cnt; ++k) {
+    unsigned e = nbr_list[(long long)s * cap + k];
+    int t = (int)(e & 0x7fffffffu);
+    int jo = sorted_orig[t];
+    if (jo <= o) continue;  // unique pairs, i < j (original ids)
+    double d[3];
+    for (int q = 0; q < 3; ++q) {
+      double v = pos[3 * jo + q] - pos[3 * o + q];
+      double sv = v * (1.0 / L[q]);
+      sv -= rint(sv);
+      d[q] = sv * L[q];
+    }
+    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (!(r <= rc)) continue;
+    unsigned long long slot = atomicAdd(counter, 1ull);
+    if ((long long)slot < out_cap) {
+      out[2 * slot] = o;
+      out[2 * slot + 1] = jo;
+    }
+  }
+}
+
+extern "C" int mdx_neighbor_pairs(mdx_ctx* c, int* d_pairs, int64_t capacity, int64_t* h_count) {
+  MDX_CHECK(c && c->pos && c->nb_configured, MDX_ERR_INVALID, "mdx_neighbor_pairs: system/nonbonded not set");
+  MDX_ENTER();
+  MDX_HIP(hipMemsetAsync(&c->dstate.p->flags, 0, sizeof(unsigned), c->stream));
+  int rc = mdx_nlist_update(c, true);
+  if (rc) return rc;
+  if ((rc = c->tmp64.ensure(1))) return rc;
+  MDX_HIP(hipMemsetAsync(c->tmp64.p, 0, sizeof(long long), c->stream));
+  k_emit_pairs<<<div_up(c->n, NL_BLOCK), NL_BLOCK, 0, c->stream>>>(
+      c->n, c->nbr_cap, c->nbr_count.p, c->nbr_list.p, c->sorted_orig.p, c->xq.p, c->pos, c->box[0], c->box[1],
+      c->box[2], c->rc, d_pairs, capacity, (unsigned long long*)c->tmp64.p);
+  MDX_LAUNCH_CHECK();
+  rc = mdx_nlist_enqueue_clear(c);
+  if (rc) return rc;
+  long long cnt = 0;
+  unsigned flags = 0;
+  MDX_HIP(hipMemcpyAsync(&cnt, c->tmp64.p, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+  MDX_HIP(hipMemcpyAsync(&flags, &c->dstate.p->flags, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+  MDX_HIP(hipStreamSynchronize(c->stream));
+  MDX_CHECK(!(flags & MDX_FLAG_OVERFLOW), MDX_ERR_OVERFLOW, "neighbour list or cell capacity exceeded");
+  *h_count = cnt;
+  MDX_LEAVE_RET(0);
+}

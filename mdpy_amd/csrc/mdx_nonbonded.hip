@@ -1,0 +1,303 @@
+// Nonbonded direct-space kernel (CHARMM LJ + erfc-screened Coulomb) and the plain all-pairs
+// Coulomb kernel, gfx950.
+//
+// LJ physics follows mdpy/constraint/charmm_nonbonded_constraint.py:88-107:
+//   eps = sqrt(eps_i eps_j), sigma = (sigma_i + sigma_j)/2 ((eps14, sigma14) for 1-4 partners),
+//   F_i = -24 eps (2 (s/r)^12 - (s/r)^6) / r^2 * d,  E = 4 eps ((s/r)^12 - (s/r)^6),  r <= rc,
+//   with d = minimum-image x_j - x_i.
+// The Coulomb part is the PME direct space q_i q_j erfc(beta r)/r over the same pair set.
+// Plain Coulomb follows mdpy/constraint/electrostatic_constraint.py:52-79.
+//
+// Layout: a group of NB_G lanes per atom in cell-sorted order walks the atom's contiguous
+// neighbour list (coalesced index loads), gathers each neighbour's float4 (x, y, z, q*sqrt(ke))
+// and float2 (sigma/2, sqrt(eps)), and keeps partial forces in registers. Every pair is
+// evaluated from both ends (full list), so there are no atomics in the pair loop; the group
+// reduces with xor shuffles and adds each atom's force once to the int64 fixed-point
+// accumulator (deterministic).
+#include "mdx_internal.h"
+
+#define NB_BLOCK 256
+#define NB_U 4  // list entries per lane per pass
+
+struct NbArgs {
+  int n, cap;
+  float Lx, Ly, Lz, ix, iy, iz;
+  float rc2, guard2;
+  float beta, twob_sqrtpi;
+  double Lxd, Lyd, Lzd, rcd;
+};
+
+// erfc(x) = exp(-x^2) t P(t), t = 1/(1 + p x): the form of Abramowitz & Stegun 7.1.26 refit
+// (mdpy_amd/tools/fit_erfc.py) to a 7-term polynomial with max relative error 4.5e-8 on
+// [0, 3.6] (beta * rc = 3.29 for the 1e-5 Ewald tolerance), so pairs near the cutoff, where
+// erfc ~ 1e-5, keep float32 relative accuracy.
+#define ERFC_P 0.355f
+#define ERFC_C1 1.9758294902e-01f
+#define ERFC_C2 2.3220149337e-01f
+#define ERFC_C3 2.8235364196e-02f
+#define ERFC_C4 5.9699421939e-01f
+#define ERFC_C5 -5.6119238634e-01f
+#define ERFC_C6 6.9343106043e-01f
+#define ERFC_C7 -1.8725274521e-01f
+
+// float64 cutoff decision with the reference's minimum-image sequence (pbc.py:40-44)
+__device__ __noinline__ bool within_cutoff64(const double* __restrict__ pos, int oi, int oj, double Lx, double Ly,
+                                              double Lz, double rc) {
+  double L[3] = {Lx, Ly, Lz};
+  double d[3];
+  for (int q = 0; q < 3; ++q) {
+    double v = pos[3 * oj + q] - pos[3 * oi + q];
+    double sv = v * (1.0 / L[q]);
+    sv -= rint(sv);
+    d[q] = sv * L[q];
+  }
+  return sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) <= rc;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// NB_G lanes share one atom: lane g walks list entries g, g + NB_G, ... of the atom's
+// contiguous (row-major) list, so a group's index loads and its neighbours' position loads
+// are contiguous; the group reduces its partial forces with xor shuffles at the end.
+template <bool LJ, bool COUL, bool ENERGY>
+__global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, const float4* __restrict__ xq,
+                                                      const float2* __restrict__ ljs,
+                                                      const float2* __restrict__ ljs14,
+                                                      const int* __restrict__ nbr_count,
+                                                      const unsigned* __restrict__ nbr_list,
+                                                      const int* __restrict__ sorted_orig,
+                                                      const double* __restrict__ pos, long long* __restrict__ ffix,
+                                                      long long* __restrict__ efix, DevState* st) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = gt / NB_G;
+  const int g = gt & (NB_G - 1);
+  if (gt == 0) {  // every rebuild kernel has finished reading these
+    st->rebuild = 0;
+    st->max_disp2_bits = 0;
+  }
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  double elj = 0.0, ec = 0.0;
+  if (s < a.n) {
+    const float4 xi = xq[s];
+    float2 pi = make_float2(0.f, 0.f), pi14 = pi;
+    if (LJ) {
+      pi = ljs[s];
+      pi14 = ljs14[s];
+    }
+    const int cnt = nbr_count[s];
+    const unsigned* lp = nbr_list + (long long)s * a.cap;
+    int oi = -1;
+    // NB_U list entries per lane per pass: all index loads, then all neighbour loads, are
+    // issued before any of them is consumed, so each pass waits for memory once, not 3*NB_U times
+    for (int kb = g; kb < cnt; kb += NB_G * NB_U) {
+      unsigned e[NB_U];
+#pragma unroll
+      for (int u = 0; u < NB_U; ++u) {
+        const int k = kb + u * NB_G;
+        e[u] = k < cnt ? lp[k] : (unsigned)s | 0x40000000u;  // bit 30: padding slot
+      }
+      float4 xj[NB_U];
+      float2 pj[NB_U];
+#pragma unroll
+      for (int u = 0; u < NB_U; ++u) {
+        const int t = (int)(e[u] & 0x3fffffffu);
+        xj[u] = xq[t];
+        if (LJ) pj[u] = ((e[u] >> 31) ? ljs14 : ljs)[t];
+      }
+#pragma unroll
+      for (int u = 0; u < NB_U; ++u) {
+        float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
+        dx -= a.Lx * rintf(dx * a.ix);
+        dy -= a.Ly * rintf(dy * a.iy);
+        dz -= a.Lz * rintf(dz * a.iz);
+        const float r2raw = dx * dx + dy * dy + dz * dz;
+        bool in = r2raw <= a.rc2 && !(e[u] & 0x40000000u);
+        if (fabsf(r2raw - a.rc2) < a.guard2 && !(e[u] & 0x40000000u)) {
+          // near the cutoff: decide in float64 like the reference (rare)
+          if (oi < 0) oi = sorted_orig[s];
+          in = within_cutoff64(pos, oi, sorted_orig[e[u] & 0x3fffffffu], a.Lxd, a.Lyd, a.Lzd, a.rcd);
+        }
+        const float r2 = in ? r2raw : 1.0f;
+        const float rinv = rsqrtf(r2);
+        const float r2inv = rinv * rinv;
+        float fr = 0.f;
+        if (LJ) {
+          const float2 pp = (e[u] >> 31) ? pi14 : pi;
+          const float sig = pp.x + pj[u].x;
+          const float eps = pp.y * pj[u].y;
+          const float s2 = sig * sig * r2inv;
+          const float s6 = s2 * s2 * s2;
+          const float s12 = s6 * s6;
+          fr -= 24.f * eps * (2.f * s12 - s6) * r2inv;
+          if (ENERGY) elj += in ? (double)(4.f * eps * (s12 - s6)) : 0.0;
+        }
+        if (COUL) {
+          const float qq = xi.w * xj[u].w;
+          const float x = a.beta * (r2 * rinv);
+          const float ex = __expf(-x * x);
+          const float tt = __frcp_rn(1.f + ERFC_P * x);
+          const float poly =
+              ERFC_C1 + tt * (ERFC_C2 + tt * (ERFC_C3 + tt * (ERFC_C4 + tt * (ERFC_C5 + tt * (ERFC_C6 + tt * ERFC_C7)))));
+          const float erfcx = tt * poly * ex;
+          fr -= qq * (erfcx * rinv + a.twob_sqrtpi * ex) * r2inv;
+          if (ENERGY) ec += in ? (double)(qq * erfcx * rinv) : 0.0;
+        }
+        fr = in ? fr : 0.f;
+        fx += fr * dx;
+        fy += fr * dy;
+        fz += fr * dz;
+      }
+    }
+  }
+  for (int off = NB_G / 2; off > 0; off >>= 1) {
+    fx += __shfl_xor(fx, off, 64);
+    fy += __shfl_xor(fy, off, 64);
+    fz += __shfl_xor(fz, off, 64);
+  }
+  if (s < a.n && g == 0) {
+    const int o = sorted_orig[s];
+    atomicAdd((unsigned long long*)&ffix[o], (unsigned long long)to_fix_f(fx));
+    atomicAdd((unsigned long long*)&ffix[a.n + o], (unsigned long long)to_fix_f(fy));
+    atomicAdd((unsigned long long*)&ffix[2 * a.n + o], (unsigned long long)to_fix_f(fz));
+  }
+  if (ENERGY) {
+    __shared__ double sm[2][NB_BLOCK / 64];
+    elj = wave_sum(elj);
+    ec = wave_sum(ec);
+    if ((threadIdx.x & 63) == 0) {
+      sm[0][threadIdx.x >> 6] = elj;
+      sm[1][threadIdx.x >> 6] = ec;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double a0 = 0, a1 = 0;
+      for (int w = 0; w < NB_BLOCK / 64; ++w) {
+        a0 += sm[0][w];
+        a1 += sm[1][w];
+      }
+      // every pair was summed from both ends
+      if (LJ) atomicAdd((unsigned long long*)&efix[0], (unsigned long long)to_fix(0.5 * a0));
+      if (COUL) atomicAdd((unsigned long long*)&efix[1], (unsigned long long)to_fix(0.5 * a1));
+    }
+  }
+}
+
+int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy) {
+  NbArgs a;
+  a.n = c->n;
+  a.cap = c->nbr_cap;
+  a.Lx = (float)c->box[0];
+  a.Ly = (float)c->box[1];
+  a.Lz = (float)c->box[2];
+  a.ix = 1.0f / a.Lx;
+  a.iy = 1.0f / a.Ly;
+  a.iz = 1.0f / a.Lz;
+  a.rc2 = (float)(c->rc * c->rc);
+  // float32 r^2 carries ~2 * r * 4e-6 A^2 of error from the float positions: re-decide within
+  // ~20x that band in float64
+  a.guard2 = (float)(c->rc * 2e-4);
+  a.beta = (float)c->beta;
+  a.twob_sqrtpi = (float)(2.0 * c->beta / sqrt(M_PI));
+  a.Lxd = c->box[0];
+  a.Lyd = c->box[1];
+  a.Lzd = c->box[2];
+  a.rcd = c->rc;
+  dim3 grid(div_up((long long)c->n * NB_G, NB_BLOCK));
+#define NB_LAUNCH(L_, C_, E_)                                                                          \
+  k_nonbonded<L_, C_, E_><<<grid, NB_BLOCK, 0, c->stream>>>(a, c->xq.p, c->ljs.p, c->ljs14.p, c->nbr_count.p, \
+                                                           c->nbr_list.p, c->sorted_orig.p, c->pos, c->ffix.p, \
+                                                           c->efix.p, c->dstate.p)
+  if (lj && coul) {
+    if (energy) NB_LAUNCH(true, true, true); else NB_LAUNCH(true, true, false);
+  } else if (lj) {
+    if (energy) NB_LAUNCH(true, false, true); else NB_LAUNCH(true, false, false);
+  } else {
+    if (energy) NB_LAUNCH(false, true, true); else NB_LAUNCH(false, true, false);
+  }
+#undef NB_LAUNCH
+  MDX_LAUNCH_CHECK();
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Plain Coulomb (mdpy ElectrostaticConstraint): every i < j not excluded, minimum image, no
+// cutoff, E = q_i q_j / (4 pi eps0 r). float64, one lane per atom over all j (full double loop,
+// each pair from both ends), j tiles staged in LDS.
+#define PC_BLOCK 256
+__global__ __launch_bounds__(PC_BLOCK) void k_coulomb_plain(int n, const double* __restrict__ pos,
+                                                          const double* __restrict__ q, double ke, double Lx,
+                                                          double Ly, double Lz, const int* __restrict__ excl_ptr,
+                                                          const int* __restrict__ excl_idx, long long* __restrict__ ffix,
+                                                          long long* __restrict__ efix) {
+  __shared__ double4 tile[PC_BLOCK];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double xi = 0, yi = 0, zi = 0, qi = 0;
+  int e0 = 0, e1 = 0;
+  if (i < n) {
+    xi = pos[3 * i];
+    yi = pos[3 * i + 1];
+    zi = pos[3 * i + 2];
+    qi = q[i];
+    if (excl_ptr) {
+      e0 = excl_ptr[i];
+      e1 = excl_ptr[i + 1];
+    }
+  }
+  double fx = 0, fy = 0, fz = 0, en = 0;
+  const double ix = 1.0 / Lx, iy = 1.0 / Ly, iz = 1.0 / Lz;
+  for (int base = 0; base < n; base += PC_BLOCK) {
+    int j = base + threadIdx.x;
+    tile[threadIdx.x] = j < n ? make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], q[j]) : make_double4(0, 0, 0, 0);
+    __syncthreads();
+    int lim = min(PC_BLOCK, n - base);
+    if (i < n) {
+      for (int t = 0; t < lim; ++t) {
+        int jj = base + t;
+        double4 pj = tile[t];
+        double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+        double sx = dx * ix, sy = dy * iy, sz = dz * iz;
+        dx = (sx - rint(sx)) * Lx;
+        dy = (sy - rint(sy)) * Ly;
+        dz = (sz - rint(sz)) * Lz;
+        double r2 = dx * dx + dy * dy + dz * dz;
+        bool skip = (jj == i) || pj.w == 0.0 || qi == 0.0;
+        for (int q2 = e0; q2 < e1; ++q2) skip |= (excl_idx[q2] == jj);
+        if (skip) continue;
+        double rinv = 1.0 / sqrt(r2);
+        double qq = qi * pj.w * ke;
+        double fr = -qq * rinv * rinv * rinv;  // force on i = -qq/r^2 * d/r
+        fx += fr * dx;
+        fy += fr * dy;
+        fz += fr * dz;
+        en += qq * rinv;
+      }
+    }
+    __syncthreads();
+  }
+  if (i < n) {
+    atomicAdd((unsigned long long*)&ffix[i], (unsigned long long)to_fix(fx));
+    atomicAdd((unsigned long long*)&ffix[n + i], (unsigned long long)to_fix(fy));
+    atomicAdd((unsigned long long*)&ffix[2 * n + i], (unsigned long long)to_fix(fz));
+  }
+  __shared__ double sm[PC_BLOCK / 64];
+  en = wave_sum(en);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = en;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int w = 0; w < PC_BLOCK / 64; ++w) s += sm[w];
+    atomicAdd((unsigned long long*)&efix[7], (unsigned long long)to_fix(0.5 * s));
+  }
+}
+
+int mdx_enqueue_coulomb_plain(mdx_ctx* c) {
+  const double ke = 1.0 / (4.0 * M_PI * 0.5727652348434076);  // 1/(4 pi eps0), internal units
+  k_coulomb_plain<<<div_up(c->n, PC_BLOCK), PC_BLOCK, 0, c->stream>>>(
+      c->n, c->pos, c->charge64.p, ke, c->box[0], c->box[1], c->box[2], c->has_excl ? c->excl_ptr.p : nullptr,
+      c->excl_idx.p, c->ffix.p, c->efix.p);
+  MDX_LAUNCH_CHECK();
+  return 0;
+}

@@ -313,6 +313,7 @@ def solvated_chain(seed=0, n_chain=1200, n_waters=10000, L=68.0, cutoff=12.0, pm
     wm, wq, wlj, wtypes = _water_atom_arrays(n_waters)
     wb, wbp, wa, wap = _water_topology(n_waters, offset=n_chain)
     positions = np.concatenate([pos, wpos])
+    positions -= L * np.floor(positions / L)  # the walk can leave the box: store periodic images in [0, L)
     return System(
         name="chain31200", positions=positions, box=np.array([L, L, L]),
         masses=np.concatenate([np.full(n_chain, 12.011), wm]),

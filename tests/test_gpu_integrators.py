@@ -1,0 +1,153 @@
+"""GPU integrator parity: reference-compatible Verlet/Langevin against the float64 oracle
+(same seeded noise), hipGraph replay against direct launches, NVE conservation and the
+Langevin temperature."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def ref_force_fn(s):
+    """The reference force field: LJ (r <= rc) + plain all-pairs Coulomb + bond + angle."""
+    from mdpy_amd.unit import COULOMB_K
+    from oracle import mdref as R
+    ep, ei = s.exclusions()
+
+    def f(x):
+        pairs = R.exact_pairs(x, s.box, s.cutoff, ep, ei)
+        F = R.lj_pair_terms(x, s.lj, pairs, s.box)[0]
+        F += R.coulomb_allpairs(x, s.charges, ep, ei, s.box, COULOMB_K)[0]
+        F += R.bond_terms(x, s.bonds, s.bond_params, s.box)[0]
+        F += R.angle_terms(x, s.angles, s.angle_params, s.box)[0]
+        return F
+    return f
+
+
+REF_TERMS = 1 | 128 | 8 | 16   # LJ | plain Coulomb | bond | angle
+
+
+def thermal_v(s, T, seed):
+    from mdpy_amd.unit import KB
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((s.num_atoms, 3)) * np.sqrt(KB * T / s.masses)[:, None]
+
+
+def test_langevin_refcompat_matches_oracle():
+    from mdpy_amd import _lib as L, systems
+    from mdpy_amd.engine import Engine
+    from mdpy_amd.unit import KB
+    from oracle import mdref as R
+    say("case: Langevin refcompat vs oracle, water648, 4 steps")
+    s = systems.water_box_small()
+    nsteps = 4
+    v0 = thermal_v(s, 300.0, 3)
+    noise = np.random.RandomState(0).randn(nsteps, s.num_atoms, 3)
+    e = Engine(s, skin=1.0)
+    e.set_positions(s.positions)
+    e.set_velocities(v0)
+    e.configure_integrator(L.INT_LANGEVIN_REF, dt=1.0, temperature=300.0, friction=1e-3)
+    e.set_external_noise(noise)
+    e.integrate(REF_TERMS, nsteps, use_graph=False)
+    xg, vg = e.positions(), e.velocities()
+    pbc = s.pbc_matrix
+    x0 = R.wrap_positions(s.positions, pbc, np.linalg.inv(pbc))
+    f = ref_force_fn(s)
+    it = R.LangevinReference(1.0, KB * 300.0, 1e-3)
+    rows = iter(noise)
+    xo, vo, _ = it.integrate(x0, v0, f(x0), s.masses, f, lambda shape: next(rows), pbc, np.linalg.inv(pbc), nsteps)
+    dx = np.abs(xg - xo).max()
+    dv = np.abs(vg - vo).max() / np.abs(vo).max()
+    say("  max |dx| %.3e A, max rel dv %.3e" % (dx, dv))
+    assert dx < 1e-7 and dv < 1e-5
+    e.close()
+
+
+def test_verlet_refcompat_matches_oracle():
+    from mdpy_amd import _lib as L, systems
+    from mdpy_amd.engine import Engine
+    from oracle import mdref as R
+    say("case: Verlet refcompat vs oracle, water648, 4 steps")
+    s = systems.water_box_small()
+    v0 = thermal_v(s, 300.0, 4)
+    e = Engine(s, skin=1.0)
+    e.set_positions(s.positions)
+    e.set_velocities(v0)
+    e.configure_integrator(L.INT_VERLET_REF, dt=1.0)
+    e.integrate(REF_TERMS, 4, use_graph=False)
+    xg, vg = e.positions(), e.velocities()
+    pbc = s.pbc_matrix
+    x0 = R.wrap_positions(s.positions, pbc, np.linalg.inv(pbc))
+    it = R.VerletReference(1.0)
+    xo, vo = it.integrate(x0, v0, s.masses, ref_force_fn(s), pbc, np.linalg.inv(pbc), 4)
+    dx = np.abs(xg - xo).max()
+    dv = np.abs(vg - vo).max() / np.abs(vo).max()
+    say("  max |dx| %.3e A, max rel dv %.3e" % (dx, dv))
+    assert dx < 1e-7 and dv < 1e-5
+    e.close()
+
+
+def test_graph_replay_bitwise_equals_direct_launches():
+    from mdpy_amd import _lib as L, systems
+    from mdpy_amd.engine import Engine
+    say("case: hipGraph vs direct, water648, 25 G-JF steps")
+    s = systems.water_box_small()
+    out = []
+    for graph in (False, True):
+        e = Engine(s, skin=1.0)
+        e.set_positions(s.positions)
+        e.set_velocities(thermal_v(s, 300.0, 5))
+        e.configure_integrator(L.INT_LANGEVIN, dt=1.0, temperature=300.0, friction=1e-3, seed=7)
+        terms = L.TERM_LJ | L.TERM_PME | L.TERM_BOND | L.TERM_ANGLE
+        e.integrate(terms, 25, use_graph=graph)
+        out.append((e.positions(), e.velocities(), e.energy_history()))
+        e.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
+
+
+def test_nve_energy_conservation_pme_water():
+    from mdpy_amd import _lib as L, systems
+    from mdpy_amd.engine import Engine
+    say("case: NVE drift, water648, 400 VV steps at 0.5 fs")
+    s = systems.water_box_small()
+    e = Engine(s, skin=1.5)
+    e.set_positions(s.positions)
+    e.set_velocities(thermal_v(s, 300.0, 6))
+    e.configure_integrator(L.INT_VELOCITY_VERLET, dt=0.5)
+    terms = L.TERM_LJ | L.TERM_PME | L.TERM_BOND | L.TERM_ANGLE
+    e.integrate(terms, 400, use_graph=True)
+    h = e.energy_history()
+    assert h.shape[0] == 401
+    etot = h[:, :8].sum(1) + h[:, 8]
+    ke = h[:, 8].mean()
+    drift = np.polyfit(np.arange(len(etot)) * 0.5, etot, 1)[0]  # per fs
+    say("  E0 %.6e  std %.3e  KE %.3e  drift %.3e /fs" % (etot[0], etot.std(), ke, drift))
+    assert etot.std() < 2e-3 * ke
+    e.close()
+
+
+def test_gjf_langevin_thermalises():
+    from mdpy_amd import _lib as L, systems
+    from mdpy_amd.engine import Engine
+    from mdpy_amd.unit import KB
+    say("case: G-JF temperature, water648, 3000 steps, gamma 0.05/fs")
+    s = systems.water_box_small()
+    e = Engine(s, skin=1.5)
+    e.set_positions(s.positions)
+    e.configure_integrator(L.INT_LANGEVIN, dt=1.0, temperature=300.0, friction=0.05, seed=11)
+    terms = L.TERM_LJ | L.TERM_PME | L.TERM_BOND | L.TERM_ANGLE
+    e.integrate(terms, 3000, use_graph=True)
+    h = e.energy_history()
+    T = 2 * h[1000:, 8] / (3 * s.num_atoms * KB)
+    say("  <T> = %.1f K" % T.mean())
+    assert 270 < T.mean() < 330
+    e.close()

@@ -1,0 +1,249 @@
+"""GPU parity: the HIP path (through the C-ABI) against the float64 oracle on the same seeded
+inputs. Integer/index results must be bit-exact; float forces within the stated tolerances."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+FORCE_RTOL = 1e-5      # relative force RMS, float32 pair/PME arithmetic vs float64 oracle
+ENERGY_RTOL = 1e-5     # relative energy error per term (large terms)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_rms(F, Fref):
+    return math.sqrt(float(((F - Fref) ** 2).sum()) / float((Fref ** 2).sum()))
+
+
+@pytest.fixture(scope="module")
+def water648():
+    from mdpy_amd import systems
+    return systems.water_box_small()
+
+
+@pytest.fixture(scope="module")
+def water23328():
+    from mdpy_amd import systems
+    return systems.water_box_large()
+
+
+@pytest.fixture(scope="module")
+def chain():
+    from mdpy_amd import systems
+    return systems.solvated_chain()
+
+
+def wrapped(s):
+    from oracle import mdref as R
+    return R.wrap_positions(s.positions, s.pbc_matrix, np.linalg.inv(s.pbc_matrix))
+
+
+# ----------------------------------------------------------------------------- cell list
+# cell-list cutoff: the reference raises for floor(L/rc) < 2 (cell_list.py:56-69), so the 18.6 A box
+# uses its constraint cutoff (9 A; same n = 3 grid as 12 A would give)
+CELL_CUTOFF = {"water648": 9.0, "water23328": 12.0}
+
+
+@pytest.mark.parametrize("name", ["water648", "water23328"])
+def test_reference_celllist_bit_exact(name, request):
+    from mdpy_amd.engine import Engine
+    from oracle import mdref as R
+    s = request.getfixturevalue(name)
+    say("case: reference cell list", name)
+    e = Engine(s, cell_cutoff=CELL_CUTOFF[name])
+    e.set_positions(s.positions)  # wraps on the device
+    xw = wrapped(s)
+    assert np.array_equal(e.positions(), xw)
+    n, w, inv = R.cell_grid(s.box, CELL_CUTOFF[name])
+    cidx, table = R.cell_list_build(xw, inv, n)
+    gidx, gtable, gn = e.celllist_reference(CELL_CUTOFF[name])
+    assert np.array_equal(gn, n)
+    assert np.array_equal(gidx, cidx)
+    assert np.array_equal(gtable, table)
+    e.close()
+
+
+def test_reference_celllist_6po6_kat():
+    # mdpy/test/test_cell_list.py:41-58 on the 6PO6 coordinates (30 A box, 9 A)
+    import json
+    from mdpy_amd.engine import Engine
+    from mdpy_amd.systems import System
+    from oracle import mdref as R
+    say("case: 6PO6 cell list")
+    pos = np.array(json.load(open(os.path.join(GOLDEN, "kat.json")))["6PO6_positions"])
+    n = len(pos)
+    s = System(name="6po6", positions=pos, box=np.array([30.0] * 3), masses=np.ones(n), charges=np.zeros(n),
+               lj=np.zeros((n, 4)), types=["X"] * n, bonds=np.zeros((0, 2), np.int64), bond_params=np.zeros((0, 2)),
+               angles=np.zeros((0, 3), np.int64), angle_params=np.zeros((0, 4)), dihedrals=np.zeros((0, 4), np.int64),
+               dihedral_params=np.zeros((0, 3)), impropers=np.zeros((0, 4), np.int64), improper_params=np.zeros((0, 2)),
+               cutoff=9.0, pme_grid=(32, 32, 32))
+    e = Engine(s)
+    e.set_positions(pos, wrap=False)  # the reference test bins the raw PDB coordinates
+    idx, table, dims = e.celllist_reference(9.0)
+    assert list(dims) == [3, 3, 3]
+    atoms = [a for a in range(n) if (idx[a] == 0).all()]
+    for k, a in enumerate(atoms):
+        assert table[0, 0, 0, k] == a
+    assert 23 in R.cell_list_getitem(idx, table, (-1, -1, 0))
+    assert 38 in R.cell_list_getitem(idx, table, (-1, -1, -1))
+    e.close()
+
+
+# ----------------------------------------------------------------------------- neighbour sets
+@pytest.mark.parametrize("name", ["water648", "water23328"])
+def test_reference_neighbor_set_bit_exact(name, request):
+    from mdpy_amd.engine import Engine
+    from oracle import mdref as R
+    s = request.getfixturevalue(name)
+    say("case: reference neighbour set", name)
+    e = Engine(s, pair_mode="reference", cell_cutoff=CELL_CUTOFF[name])
+    e.set_positions(s.positions)
+    xw = wrapped(s)
+    ep, ei = s.exclusions()
+    n, w, inv = R.cell_grid(s.box, CELL_CUTOFF[name])
+    cidx, _ = R.cell_list_build(xw, inv, n)
+    ref = R.reference_pairs(xw, s.box, s.cutoff, ep, ei, cidx, n)
+    got = e.neighbor_pairs()
+    assert got.shape == ref.shape
+    assert np.array_equal(got, ref)
+    e.close()
+
+
+def test_exact_neighbor_set_chain(chain):
+    from mdpy_amd.engine import Engine
+    from oracle import mdref as R
+    say("case: exact neighbour set chain31200")
+    e = Engine(chain, skin=1.5)
+    e.set_positions(chain.positions)
+    xw = wrapped(chain)
+    ep, ei = chain.exclusions()
+    ref = R.exact_pairs(xw, chain.box, chain.cutoff, ep, ei)
+    got = e.neighbor_pairs()
+    assert np.array_equal(got, ref)
+    e.close()
+
+
+# ----------------------------------------------------------------------------- forces
+def oracle_terms(s, xw, pairs, beta):
+    from mdpy_amd.unit import ONE_OVER_4PI_EPS0
+    from oracle import ewald as EW, mdref as R
+    ep, ei = s.exclusions()
+    sp, si = s.scaling_pairs()
+    out = {}
+    out["lj"] = R.lj_pair_terms(xw, s.lj, pairs, s.box, sp, si)
+    ex = np.array(sorted({(a, int(b)) for a in range(s.num_atoms) for b in ei[ep[a]:ep[a + 1]] if b > a}),
+                  dtype=np.int64).reshape(-1, 2)
+    Fd, Ed = EW.direct_terms(xw, s.charges, pairs, s.box, beta, ONE_OVER_4PI_EPS0)
+    Fr, Er = EW.spme_reciprocal(xw, s.charges, s.box, s.pme_grid, 4, beta, ONE_OVER_4PI_EPS0)
+    Fx, Ex = EW.exclusion_terms(xw, s.charges, ex, s.box, beta, ONE_OVER_4PI_EPS0)
+    Es = EW.self_energy(s.charges, beta, ONE_OVER_4PI_EPS0)
+    Eb = EW.background_energy(s.charges, s.box, beta, ONE_OVER_4PI_EPS0)
+    out["coulomb_direct"] = (Fd, Ed)
+    out["coulomb_reciprocal"] = (Fr + Fx, Er + Ex + Es + Eb)
+    out["_scale"] = {"coulomb_reciprocal": abs(Er) + abs(Ex) + abs(Es) + abs(Eb)}
+    out["bond"] = R.bond_terms(xw, s.bonds, s.bond_params, s.box)
+    out["angle"] = R.angle_terms(xw, s.angles, s.angle_params, s.box)
+    out["dihedral"] = R.dihedral_terms(xw, s.dihedrals, s.dihedral_params, s.box)
+    out["improper"] = R.improper_terms(xw, s.impropers, s.improper_params, s.box)
+    return out
+
+
+TERM_BITS = {"lj": 1, "coulomb_direct": 2, "coulomb_reciprocal": 4, "bond": 8, "angle": 16, "dihedral": 32,
+             "improper": 64}
+
+
+@pytest.mark.parametrize("name", ["water648", "water23328", "chain"])
+def test_forces_match_oracle(name, request):
+    from mdpy_amd.engine import Engine
+    from oracle import mdref as R
+    s = request.getfixturevalue(name)
+    say("case: forces vs oracle", name)
+    e = Engine(s, skin=1.5)
+    e.set_positions(s.positions)
+    xw = wrapped(s)
+    ep, ei = s.exclusions()
+    pairs = R.exact_pairs(xw, s.box, s.cutoff, ep, ei)
+    ref = oracle_terms(s, xw, pairs, e.beta)
+    ref_scale = ref.pop("_scale")
+    Ftot_ref = np.zeros_like(xw)
+    Etot_ref = 0.0
+    for term, bit in TERM_BITS.items():
+        Fref, Eref = ref[term]
+        Ftot_ref += Fref
+        Etot_ref += Eref
+        if not np.any(Fref):
+            continue
+        F, en = e.compute(bit)
+        F = F.cpu().numpy()
+        rms_ref = math.sqrt(float((Fref ** 2).sum()) / len(Fref))
+        if rms_ref < 1e-9:  # e.g. water bonds/angles at their exact reference geometry
+            say("  %-20s |F| max %.3e (reference RMS %.1e)" % (term, np.abs(F).max(), rms_ref))
+            assert np.abs(F - Fref).max() < 1e-10
+            continue
+        err = rel_rms(F, Fref)
+        say("  %-20s rel force RMS %.3e  E %.10e ref %.10e" % (term, err, en[term], Eref))
+        # float64 bonded kernels: only the int64 fixed-point quantum (2^-40) separates them
+        tol = 1e-8 if term in ("bond", "angle", "dihedral", "improper") else 3 * FORCE_RTOL
+        assert err < tol, (term, err)
+        # energies: relative to the magnitude of the parts that cancel in the sum (the reciprocal
+        # term is grid energy + self + exclusion correction, each much larger than the total)
+        scale = ref_scale[term] if term in ref_scale else abs(Eref)
+        assert abs(en[term] - Eref) <= max(ENERGY_RTOL * scale, 1e-9), (term, en[term], Eref, scale)
+    F, en = e.compute(sum(TERM_BITS.values()))
+    err = rel_rms(F.cpu().numpy(), Ftot_ref)
+    say("  total rel force RMS %.3e" % err)
+    assert err < FORCE_RTOL
+    e.close()
+
+
+def test_forces_deterministic(water23328):
+    from mdpy_amd.engine import Engine
+    say("case: determinism water23328")
+    e = Engine(water23328, skin=1.5)
+    e.set_positions(water23328.positions)
+    F1, E1 = e.compute(127)
+    F1 = F1.cpu().numpy().copy()
+    F2, E2 = e.compute(127)
+    assert np.array_equal(F1, F2.cpu().numpy())
+    assert E1 == E2
+    e.close()
+
+
+def test_plain_coulomb_matches_reference_kat_and_oracle(water648):
+    # mdpy ElectrostaticConstraint semantics: all pairs, minimum image, no cutoff
+    from mdpy_amd.engine import Engine
+    from mdpy_amd.unit import COULOMB_K
+    from oracle import mdref as R
+    say("case: plain coulomb water648")
+    e = Engine(water648)
+    e.set_positions(water648.positions)
+    xw = wrapped(water648)
+    ep, ei = water648.exclusions()
+    Fref, Eref = R.coulomb_allpairs(xw, water648.charges, ep, ei, water648.box, COULOMB_K)
+    F, en = e.compute(128)
+    assert rel_rms(F.cpu().numpy(), Fref) < 1e-10
+    assert en["coulomb_plain"] == pytest.approx(Eref, rel=1e-10)
+    e.close()
+
+
+def test_dihedral_refcompat(chain):
+    from mdpy_amd.engine import Engine
+    from oracle import mdref as R
+    say("case: dihedral refcompat chain")
+    e = Engine(chain, dihedral_refcompat=True)
+    e.set_positions(chain.positions)
+    xw = wrapped(chain)
+    Fref, Eref = R.dihedral_terms(xw, chain.dihedrals, chain.dihedral_params, chain.box, refcompat=True)
+    F, en = e.compute(32)
+    assert rel_rms(F.cpu().numpy(), Fref) < 1e-9   # int64 fixed-point quantum
+    assert en["dihedral"] == pytest.approx(Eref, rel=1e-9)
+    e.close()
