@@ -97,10 +97,24 @@ __global__ void k_record(const long long* __restrict__ efix, int ne, double* __r
     for (int t = 0; t < ne; ++t) ehist[(k / every) * ne + t] = (double)efix[t] * kFixInv;
 }
 
-__global__ void k_forces_out(const long long* __restrict__ ffix, int n, double* __restrict__ out) {
+// total force on atom i along d: the fixed-point accumulator plus, when the PME plane kernels
+// ran, the four per-plane reciprocal contributions (each written once, summed in fixed order)
+__device__ __forceinline__ double total_force(const long long* __restrict__ ffix, const double* __restrict__ fpart,
+                                             int n, int i, int d) {
+  double f = (double)ffix[(long long)d * n + i] * kFixInv;
+  if (fpart) {
+    const long long n3 = 3LL * n;
+    const double* p = fpart + 3LL * i + d;
+    f += (p[0] + p[n3]) + (p[2 * n3] + p[3 * n3]);
+  }
+  return f;
+}
+
+__global__ void k_forces_out(const long long* __restrict__ ffix, const double* __restrict__ fpart, int n,
+                             double* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  for (int d = 0; d < 3; ++d) out[3 * i + d] = (double)ffix[(long long)d * n + i] * kFixInv;
+  for (int d = 0; d < 3; ++d) out[3 * i + d] = total_force(ffix, fpart, n, i, d);
 }
 
 __device__ void block_add_energy(double e, long long* slot) {
@@ -176,10 +190,11 @@ struct IntArgs {
   double dt, a, b, sigma;
   double L[3], inv[3];
   NoiseSrc ns;
+  const double* fpart;  // per-plane reciprocal forces of the last evaluation, or null
 };
 
-__device__ __forceinline__ double fix_force(const long long* ffix, int n, int i, int d) {
-  return (double)ffix[(long long)d * n + i] * kFixInv;
+__device__ __forceinline__ double fix_force(const long long* ffix, const IntArgs& A, int i, int d) {
+  return total_force(ffix, A.fpart, A.n, i, d);
 }
 
 // velocity Verlet, first half: v += dt/2 a; x += dt v
@@ -188,7 +203,7 @@ __global__ void k_vv_a(IntArgs A, double* __restrict__ x, double* __restrict__ v
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
   for (int d = 0; d < 3; ++d) {
-    double vv = v[3 * i + d] + 0.5 * A.dt * fix_force(ffix, A.n, i, d) * im[i];
+    double vv = v[3 * i + d] + 0.5 * A.dt * fix_force(ffix, A, i, d) * im[i];
     v[3 * i + d] = vv;
     x[3 * i + d] += A.dt * vv;
   }
@@ -200,7 +215,7 @@ __global__ void k_vv_b(IntArgs A, double* __restrict__ v, const double* __restri
   double e = 0;
   if (i < A.n) {
     for (int d = 0; d < 3; ++d) {
-      double vv = v[3 * i + d] + 0.5 * A.dt * fix_force(ffix, A.n, i, d) * im[i];
+      double vv = v[3 * i + d] + 0.5 * A.dt * fix_force(ffix, A, i, d) * im[i];
       v[3 * i + d] = vv;
       e += vv * vv;
     }
@@ -219,7 +234,7 @@ __global__ void k_gjf_a(IntArgs A, double* __restrict__ x, const double* __restr
   noise3(A.ns, st->step + 1, i, A.n, g);
   double sm = sqrt(im[i]);
   for (int d = 0; d < 3; ++d) {
-    double acc = fix_force(ffix, A.n, i, d) * im[i];
+    double acc = fix_force(ffix, A, i, d) * im[i];
     acc_prev[3 * i + d] = acc;
     x[3 * i + d] += A.b * A.dt * v[3 * i + d] + A.b * A.dt * A.dt * 0.5 * acc +
                     A.b * A.sigma * A.dt * sqrt(A.dt) * 0.5 * g[d] * sm;
@@ -236,7 +251,7 @@ __global__ void k_gjf_b(IntArgs A, double* __restrict__ v, const double* __restr
     noise3(A.ns, st->step, i, A.n, g);
     double sm = sqrt(im[i]);
     for (int d = 0; d < 3; ++d) {
-      double acc = fix_force(ffix, A.n, i, d) * im[i];
+      double acc = fix_force(ffix, A, i, d) * im[i];
       double vv = A.a * v[3 * i + d] + 0.5 * A.dt * (A.a * acc_prev[3 * i + d] + acc) + A.b * A.sigma * sqrt(A.dt) * g[d] * sm;
       v[3 * i + d] = vv;
       e += vv * vv;
@@ -266,7 +281,7 @@ __global__ void k_vref(IntArgs A, double* __restrict__ cur, double* __restrict__
   if (i >= A.n) return;
   double nx[3];
   for (int d = 0; d < 3; ++d) {
-    double acc = fix_force(ffix, A.n, i, d) * im[i];
+    double acc = fix_force(ffix, A, i, d) * im[i];
     nx[d] = 2 * cur[3 * i + d] - pre[3 * i + d] + acc * A.dt * A.dt;
     pre[3 * i + d] = cur[3 * i + d];
     cur[3 * i + d] = nx[d];
@@ -307,7 +322,7 @@ __global__ void k_lref_b(IntArgs A, double* __restrict__ cv, double* __restrict_
     noise3(A.ns, st->step, i, A.n, g);
     double sm = sqrt(im[i]);
     for (int d = 0; d < 3; ++d) {
-      double ca = fix_force(ffix, A.n, i, d) * im[i];
+      double ca = fix_force(ffix, A, i, d) * im[i];
       acc_cur[3 * i + d] = ca;
       double nv = A.a * pv[3 * i + d] - A.dt / 2 * (A.a * acc_pre[3 * i + d] + ca) + A.b * A.sigma * sqrt(A.dt) * g[d] * sm;
       pv[3 * i + d] = cv[3 * i + d];
@@ -326,7 +341,7 @@ __global__ void k_vref_prime(IntArgs A, const double* __restrict__ x, const doub
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n || cached) return;
   for (int d = 0; d < 3; ++d) {
-    double acc = fix_force(ffix, A.n, i, d) * im[i];
+    double acc = fix_force(ffix, A, i, d) * im[i];
     cur[3 * i + d] = x[3 * i + d];
     pre[3 * i + d] = x[3 * i + d] - v[3 * i + d] * A.dt + acc * A.dt * A.dt;  // verlet_integrator.py:31-34
   }
@@ -338,7 +353,7 @@ __global__ void k_lref_prime(IntArgs A, const double* __restrict__ x, const doub
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
   for (int d = 0; d < 3; ++d) {
-    acc_cur[3 * i + d] = fix_force(ffix, A.n, i, d) * im[i];  // langevin_integrator.py:42
+    acc_cur[3 * i + d] = fix_force(ffix, A, i, d) * im[i];  // langevin_integrator.py:42
     if (!cached) {
       cx[3 * i + d] = px[3 * i + d] = x[3 * i + d];
       cv[3 * i + d] = pv[3 * i + d] = v[3 * i + d];
@@ -617,6 +632,9 @@ extern "C" int mdx_nonbonded_config(mdx_ctx* c, double cutoff, double skin, int 
 }
 
 // ------------------------------------------------------------------ force evaluation
+// reciprocal forces held outside the fixed-point accumulator by the last evaluation
+static const double* live_fpart(const mdx_ctx* c) { return c->fpart_live ? c->fpart.p : nullptr; }
+
 int mdx_enqueue_zero_accumulators(mdx_ctx* c) {
   long long n3 = 3LL * c->n;
   k_zero_acc<<<div_up(n3, 256), 256, 0, c->stream>>>(c->ffix.p, n3, c->efix.p, MDX_NTERMS + 1, nullptr, 0,
@@ -638,7 +656,13 @@ int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy) {
   }
   if (terms & MDX_TERM_COUL_RECIP) {
     MDX_CHECK(c->has_charges && c->pme_configured, MDX_ERR_INVALID, "PME requested but not configured");
+    // the PME plane kernels walk the cell-sorted atom order of the list grid
+    const bool grid_here = c->nb_configured && !(lj || cd);
+    if (grid_here && (rc = mdx_nlist_update(c, false))) return rc;
     if ((rc = mdx_pme_enqueue(c, energy))) return rc;
+    if (grid_here && (rc = mdx_nlist_enqueue_clear(c))) return rc;
+  } else {
+    c->fpart_live = false;
   }
   if (terms & MDX_TERM_COUL_PLAIN) {
     MDX_CHECK(c->has_charges, MDX_ERR_INVALID, "Coulomb requested but no charges set");
@@ -658,7 +682,7 @@ extern "C" int mdx_compute_forces(mdx_ctx* c, unsigned terms, double* d_force_ou
     if (hipGetLastError() != hipSuccess) rc = MDX_ERR_HIP;
   }
   if (!rc && d_force_out) {
-    k_forces_out<<<div_up(c->n, 256), 256, 0, c->stream>>>(c->ffix.p, c->n, d_force_out);
+    k_forces_out<<<div_up(c->n, 256), 256, 0, c->stream>>>(c->ffix.p, live_fpart(c), c->n, d_force_out);
     if (hipGetLastError() != hipSuccess) rc = MDX_ERR_HIP;
   }
   c->evals++;
@@ -746,6 +770,7 @@ static IntArgs int_args(mdx_ctx* c) {
   A.ns.ext = c->ext_noise;
   A.ns.ext_base = c->ext_noise_base;
   A.ns.ext_steps = c->ext_noise_steps;
+  A.fpart = live_fpart(c);
   return A;
 }
 
@@ -768,6 +793,7 @@ int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy) {
       k_vv_a<<<g, b, 0, c->stream>>>(A, c->pos, c->vel, c->inv_mass.p, c->ffix.p);
       MDX_LAUNCH_CHECK();
       if ((rc = zero_and_record(c)) || (rc = mdx_enqueue_forces(c, terms, energy))) return rc;
+      A.fpart = live_fpart(c);
       k_vv_b<<<g, b, 0, c->stream>>>(A, c->vel, c->inv_mass.p, c->mass.p, c->ffix.p, ke);
       MDX_LAUNCH_CHECK();
       break;
@@ -775,6 +801,7 @@ int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy) {
       k_gjf_a<<<g, b, 0, c->stream>>>(A, c->pos, c->vel, c->inv_mass.p, c->ffix.p, c->acc_prev.p, c->dstate.p);
       MDX_LAUNCH_CHECK();
       if ((rc = zero_and_record(c)) || (rc = mdx_enqueue_forces(c, terms, energy))) return rc;
+      A.fpart = live_fpart(c);
       k_gjf_b<<<g, b, 0, c->stream>>>(A, c->vel, c->inv_mass.p, c->mass.p, c->ffix.p, c->acc_prev.p, c->dstate.p, ke);
       MDX_LAUNCH_CHECK();
       break;
@@ -782,6 +809,7 @@ int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy) {
       k_vref<<<g, b, 0, c->stream>>>(A, c->xa.p, c->xb.p, c->pos, c->inv_mass.p, c->ffix.p, c->dstate.p);
       MDX_LAUNCH_CHECK();
       if ((rc = zero_and_record(c)) || (rc = mdx_enqueue_forces(c, terms, energy))) return rc;
+      A.fpart = live_fpart(c);
       k_kinetic<<<g, b, 0, c->stream>>>(c->n, c->vel, c->mass.p, ke);
       MDX_LAUNCH_CHECK();
       break;
@@ -790,6 +818,7 @@ int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy) {
                                        c->acc_prev.p, c->dstate.p);
       MDX_LAUNCH_CHECK();
       if ((rc = zero_and_record(c)) || (rc = mdx_enqueue_forces(c, terms, energy))) return rc;
+      A.fpart = live_fpart(c);
       k_lref_b<<<g, b, 0, c->stream>>>(A, c->va.p, c->vb.p, c->inv_mass.p, c->mass.p, c->ffix.p, c->acc_cur.p,
                                        c->acc_prev.p, c->dstate.p, ke);
       MDX_LAUNCH_CHECK();
@@ -827,6 +856,7 @@ extern "C" int mdx_integrate(mdx_ctx* c, unsigned terms, int nsteps, int use_gra
   c->ehist_count = 0;
   // prime: forces at the current positions (every mdpy integrate call starts with ensemble.update)
   if ((rc = mdx_enqueue_zero_accumulators(c)) || (rc = mdx_enqueue_forces(c, terms, true))) LEAVE_RET(rc);
+  A.fpart = live_fpart(c);
   k_kinetic<<<g, b, 0, c->stream>>>(c->n, c->vel, c->mass.p, c->efix.p + MDX_E_KINETIC);
   if (c->int_kind == MDX_INT_VERLET_REF) {
     k_vref_prime<<<g, b, 0, c->stream>>>(A, c->pos, c->vel, c->inv_mass.p, c->ffix.p, c->xa.p, c->xb.p,

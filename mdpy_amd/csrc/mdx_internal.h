@@ -22,6 +22,7 @@ __device__ __forceinline__ long long to_fix_f(float v) { return __double2ll_rn((
 #define MDX_FLAG_OVERFLOW 1u
 #define MDX_FLAG_LOSS 2u
 #define MDX_FLAG_NONFINITE 4u
+#define MDX_FLAG_PME_COVERAGE 8u   // reciprocal-space gather missed an atom (internal invariant)
 
 // ------------------------------------------------------------------ error handling
 void mdx_set_error(const char* fmt, ...);
@@ -88,6 +89,8 @@ struct DevState {
   int maxpc;                // largest cell population (reference binning)
   int pad;
   long long hist_base;      // step counter value at the start of the current integrate call
+  int pme_slots;            // reciprocal force slots written in the current evaluation
+  unsigned pme_ticket;      // finished blocks of the PME gather kernel
 };
 
 struct mdx_ctx {
@@ -138,6 +141,7 @@ struct mdx_ctx {
   DevBuf<float2> ljs, ljs14;     // sorted: sigma/2, sqrt(eps)
   DevBuf<int> ref_cell_sorted;   // sorted: packed reference cell coords (REFERENCE mode)
   DevBuf<double> pos_at_build;   // original order, 3N
+  DevBuf<double> pos_sorted;     // sorted order, 3N, refreshed with xq every evaluation
   DevBuf<int> nbr_count;         // sorted
   DevBuf<unsigned> nbr_list;     // column major: [k * stride + s]
   int nbr_cap = 0;               // capacity per atom
@@ -157,6 +161,10 @@ struct mdx_ctx {
   DevBuf<float2> twiddle;        // per axis, K entries each (3 * maxK)
   DevBuf<int> excl_pairs;        // unique excluded pairs (i<j), 2 x n_excl_pairs
   int n_excl_pairs = 0;
+  DevBuf<double> fpart;          // plane path: reciprocal force per (kx, atom), 4 x 3N
+  bool pme_planes = false;       // plane kernels usable (list grid present, plane fits LDS)
+  int pme_wf = 0, pme_wg = 0;    // candidate windows of the two plane kernels
+  bool fpart_live = false;       // fpart belongs to the current evaluation
   // ---- integrator
   bool int_configured = false;
   int int_kind = MDX_INT_VELOCITY_VERLET;

@@ -378,14 +378,18 @@ __global__ void k_grid_sort_fill(const int* __restrict__ cell_start, const int* 
   }
 }
 
-// every evaluation: float32 positions (wrapped, centred) and scaled charges in sorted order
+// every evaluation: float32 positions (wrapped, centred) and scaled charges in sorted order,
+// plus the float64 positions as they are (sorted order, read by the PME plane kernels)
 __global__ void k_pack(const double* __restrict__ pos, const int* __restrict__ sorted_orig,
                        const float* __restrict__ qs, int n, double Lx, double Ly, double Lz,
-                       float4* __restrict__ xq) {
+                       float4* __restrict__ xq, double* __restrict__ pos_sorted) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   int o = sorted_orig[s];
   double x = pos[3 * o + 0], y = pos[3 * o + 1], z = pos[3 * o + 2];
+  pos_sorted[3 * s + 0] = x;
+  pos_sorted[3 * s + 1] = y;
+  pos_sorted[3 * s + 2] = z;
   x -= Lx * floor(x / Lx);
   y -= Ly * floor(y / Ly);
   z -= Lz * floor(z / Lz);
@@ -728,7 +732,8 @@ int mdx_nlist_prepare(mdx_ctx* c) {
       (rc = c->cell_cursor.ensure(c->ncell)) || (rc = c->atom_cell.ensure(c->n)) ||
       (rc = c->sorted_orig.ensure(c->n)) || (rc = c->orig_sorted.ensure(c->n)) || (rc = c->xq.ensure(c->n)) ||
       (rc = c->ljs.ensure(c->n)) || (rc = c->ljs14.ensure(c->n)) || (rc = c->ref_cell_sorted.ensure(c->n)) ||
-      (rc = c->pos_at_build.ensure(3 * (size_t)c->n)) || (rc = c->nbr_count.ensure(c->n)))
+      (rc = c->pos_at_build.ensure(3 * (size_t)c->n)) || (rc = c->nbr_count.ensure(c->n)) ||
+      (rc = c->pos_sorted.ensure(3 * (size_t)c->n)))
     return rc;
   // capacity: expected neighbours in a sphere of radius rl at the system density, with margin
   double vol = c->box[0] * c->box[1] * c->box[2];
@@ -750,7 +755,8 @@ int mdx_nlist_prepare(mdx_ctx* c) {
 int mdx_nlist_enqueue_pack(mdx_ctx* c) {
   k_pack<<<div_up(c->n, NL_BLOCK), NL_BLOCK, 0, c->stream>>>(c->pos, c->sorted_orig.p,
                                                             c->has_charges ? c->charge_s.p : nullptr, c->n,
-                                                            c->box[0], c->box[1], c->box[2], c->xq.p);
+                                                            c->box[0], c->box[1], c->box[2], c->xq.p,
+                                                            c->pos_sorted.p);
   MDX_LAUNCH_CHECK();
   return 0;
 }

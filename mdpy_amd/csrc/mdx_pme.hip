@@ -16,10 +16,12 @@
 #define FFT_THREADS 256
 #define MAX_RADICES 16
 
+// radix of pass p in bits [4p, 4p+4) of `code` (a register-friendly stand-in for an array that
+// a dynamically indexed kernel argument would otherwise put in scratch memory)
 struct FftPlan {
   int K;
   int nrad;
-  int radix[MAX_RADICES];
+  unsigned long long code;
 };
 
 struct FftLines {
@@ -74,35 +76,57 @@ __device__ __forceinline__ void dft5(float2* u, float s) {
   u[3] = csub(p2, q2);
 }
 
-// One transform of B lines of length K held in LDS (src), Stockham passes ping-ponging with
-// dst. Returns the buffer holding the result.
-template <int DIR>
-__device__ float2* lds_fft(float2* src, float2* dst, const FftPlan& pl, int B, const float2* __restrict__ tw) {
-  const int K = pl.K;
-  int p = 1;
+// one Stockham pass of radix R over B lines (element e of line l at l*ls + e*es); p is the
+// product of the radices of the passes already done
+template <int DIR, int R>
+__device__ __forceinline__ void stockham_pass(const float2* src, float2* dst, int K, int p, int B,
+                                              const float2* __restrict__ tw, int ls, int es) {
+  const int T = K / R;
+  const int tstep = K / (p * R);
   const float s = (float)DIR;
+  for (int b = threadIdx.x; b < B * T; b += blockDim.x) {
+    int l, i;
+    if (es == 1) { l = b / T; i = b - l * T; }
+    else { i = b / B; l = b - i * B; }
+    const int k = i % p;
+    const float2* X = src + l * ls;
+    float2* Y = dst + l * ls;
+    float2 u[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) u[r] = X[(i + r * T) * es];
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      float2 w = tw[(r * k * tstep) % K];
+      if (DIR > 0) w.y = -w.y;
+      u[r] = cmul(u[r], w);
+    }
+    if constexpr (R == 2) dft2(u);
+    else if constexpr (R == 3) dft3(u, s);
+    else if constexpr (R == 4) dft4(u, s);
+    else dft5(u, s);
+    const int j = (i - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) Y[(j + r * p) * es] = u[r];
+  }
+}
+
+// B lines of length K held in LDS (src); element e of line l sits at l*ls + e*es. Stockham
+// passes ping-pong with dst; returns the buffer holding the result. Consecutive threads take
+// consecutive elements of a line when es == 1 and consecutive lines otherwise, so LDS accesses
+// stay on distinct banks for both contiguous and strided lines.
+template <int DIR>
+__device__ float2* lds_fft(float2* src, float2* dst, const FftPlan& pl, int B, const float2* __restrict__ tw,
+                           int ls = -1, int es = 1) {
+  const int K = pl.K;
+  if (ls < 0) ls = K;
+  int p = 1;
   for (int pass = 0; pass < pl.nrad; ++pass) {
-    const int R = pl.radix[pass];
-    const int T = K / R;
-    const int tstep = K / (p * R);
-    for (int b = threadIdx.x; b < B * T; b += blockDim.x) {
-      const int l = b / T, i = b - l * T;
-      const int k = i % p;
-      const float2* X = src + l * K;
-      float2* Y = dst + l * K;
-      float2 u[5];
-      for (int r = 0; r < R; ++r) u[r] = X[i + r * T];
-      for (int r = 1; r < R; ++r) {
-        float2 w = tw[(r * k * tstep) % K];
-        if (DIR > 0) w.y = -w.y;
-        u[r] = cmul(u[r], w);
-      }
-      if (R == 2) dft2(u);
-      else if (R == 3) dft3(u, s);
-      else if (R == 4) dft4(u, s);
-      else dft5(u, s);
-      const int j = (i - k) * R + k;
-      for (int r = 0; r < R; ++r) Y[j + r * p] = u[r];
+    const int R = (int)((pl.code >> (4 * pass)) & 15ull);
+    switch (R) {
+      case 2: stockham_pass<DIR, 2>(src, dst, K, p, B, tw, ls, es); break;
+      case 3: stockham_pass<DIR, 3>(src, dst, K, p, B, tw, ls, es); break;
+      case 4: stockham_pass<DIR, 4>(src, dst, K, p, B, tw, ls, es); break;
+      default: stockham_pass<DIR, 5>(src, dst, K, p, B, tw, ls, es); break;
     }
     __syncthreads();
     float2* t = src;
@@ -111,6 +135,10 @@ __device__ float2* lds_fft(float2* src, float2* dst, const FftPlan& pl, int B, c
     p *= R;
   }
   return src;
+}
+
+__device__ __forceinline__ void load_twiddles(float2* dst, const float2* __restrict__ src, int K) {
+  for (int i = threadIdx.x; i < K; i += blockDim.x) dst[i] = src[i];
 }
 
 __device__ __forceinline__ long long line_base(const FftLines& ln, int l) {
@@ -131,6 +159,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(FftPlan pl, FftLines l
   const int nb = min(B, ln.nlines - l0);
   float2* A = lds;
   float2* Bf = lds + B * K;
+  float2* stw = Bf + B * K;
+  load_twiddles(stw, tw, K);
   const bool contiguous = ln.estride == 1;
   // load
   for (int idx = threadIdx.x; idx < nb * K; idx += blockDim.x) {
@@ -149,7 +179,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(FftPlan pl, FftLines l
     A[l * K + e] = v;
   }
   __syncthreads();
-  float2* res = lds_fft<DIR>(A, Bf, pl, nb, tw);
+  float2* res = lds_fft<DIR>(A, Bf, pl, nb, stw);
   if (MODE == 3) {
     float2* other = (res == A) ? Bf : A;
     double en = 0.0;
@@ -162,7 +192,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(FftPlan pl, FftLines l
       res[idx] = cscale(v, f);
     }
     __syncthreads();
-    res = lds_fft<-DIR>(res, other, pl, nb, tw);
+    res = lds_fft<-DIR>(res, other, pl, nb, stw);
     // block energy
     __shared__ double sm[FFT_THREADS / 64];
     for (int off = 32; off > 0; off >>= 1) en += __shfl_xor(en, off, 64);
@@ -333,16 +363,355 @@ __global__ void k_pme_excl(int npairs, int n, const int* __restrict__ pairs, con
   }
 }
 
+// ---------------------------------------------------------------- plane kernels
+// One workgroup per x-plane x0 of the grid. The atoms that touch the plane (floor(u_x) - kx ==
+// x0 for some kx in 0..3) are found in the cell-sorted atom order of the neighbour-list grid:
+// cells are x-major, so the x-slices of cells that can hold such atoms form at most two
+// contiguous ranges of sorted slots, and no atom is more than skin/2 away from its slice.
+// Phase 1 (one lane per candidate) compacts the touching atoms into LDS records; phase 2 gives
+// each record 16 lanes, one per (ky, kz) point of its 4x4 stencil in the plane.
+#define PLANE_THREADS 512
+
+struct PlaneArgs {
+  int n;
+  int Kx, Ky, Kz;
+  double Lx, Ly, Lz;
+  int gnx, gny, gnz;  // neighbour-list grid
+  double wx;          // its cell width along x
+  double margin;      // list skin/2 plus rounding slack
+  int W;              // candidates examined per round (bounded by the LDS record space)
+};
+
+struct SpreadRec {  // 16 B
+  int yz;           // by | bz << 12
+  float wy, wz, qx; // fractional offsets, q sqrt(ke) theta_kx(wx)
+};
+struct GatherRec {  // 24 B
+  int atom;
+  int yzk;          // by | bz << 12 | kx << 24
+  float wx, wy, wz, q;
+};
+
+// weight k of the order-4 cardinal B-spline at offset w (grid point base - k), and its
+// derivative; branch-free so lanes with different k do not diverge
+__device__ __forceinline__ float sel4(int k, float a0, float a1, float a2, float a3) {
+  return k == 0 ? a0 : (k == 1 ? a1 : (k == 2 ? a2 : a3));
+}
+__device__ __forceinline__ float bsp4(float w, int k) {
+  const float w2 = w * w, w3 = w2 * w, om = 1.f - w;
+  return sel4(k, w3, -3.f * w3 + 3.f * w2 + 3.f * w + 1.f, 3.f * w3 - 6.f * w2 + 4.f, om * om * om) * (1.f / 6.f);
+}
+__device__ __forceinline__ float dbsp4(float w, int k) {
+  const float w2 = w * w, om = 1.f - w;
+  return sel4(k, w2, -3.f * w2 + 2.f * w + 1.f, 3.f * w2 - 4.f * w, -om * om) * 0.5f;
+}
+
+// sorted-slot ranges [b0,e0) and [b1,e1) holding every atom that can touch plane x0
+__device__ void plane_ranges(const PlaneArgs& a, const int* __restrict__ cell_start, int x0, int* b0, int* e0,
+                             int* b1, int* e1) {
+  const double h = a.Lx / a.Kx;
+  const int lo = (int)floor((x0 * h - a.margin) / a.wx);
+  const int hi = (int)floor(((x0 + 4) * h + a.margin) / a.wx);
+  *b1 = *e1 = 0;
+  if (hi - lo + 1 >= a.gnx) {
+    *b0 = 0;
+    *e0 = a.n;
+    return;
+  }
+  const int first = ((lo % a.gnx) + a.gnx) % a.gnx;
+  const int last = ((hi % a.gnx) + a.gnx) % a.gnx;
+  const int per = a.gny * a.gnz;
+  const int s_first = cell_start[first * per];
+  const int e_last = last + 1 < a.gnx ? cell_start[(last + 1) * per] : a.n;
+  if (first <= last) {
+    *b0 = s_first;
+    *e0 = e_last;
+  } else {
+    *b0 = s_first;
+    *e0 = a.n;
+    *b1 = 0;
+    *e1 = e_last;
+  }
+}
+
+// wave-aggregated slot reservation in an LDS counter
+__device__ __forceinline__ int reserve_slot(bool keep, int* counter) {
+  const unsigned long long m = __ballot(keep);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  const int leader = m ? __ffsll((long long)m) - 1 : 0;
+  if (m && lane == leader) base = atomicAdd(counter, __popcll(m));
+  base = __shfl(base, leader, 64);
+  return base + __popcll(m & ((1ull << lane) - 1ull));
+}
+
+// candidates examined per lane per round of phase 1 (independent loads in flight)
+#define PLANE_U 4
+
+// spread q*sqrt(ke) of every atom touching plane x0 into an LDS int64 plane (integer sums, so the
+// result is independent of the order of the atomics), then forward FFT along z and y; the
+// plane goes to cg[x0][y][z].  LDS: twiddles (Kz + Ky), two complex planes; the records
+// overlay the second plane and what follows it.
+__global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_fwd(PlaneArgs a, FftPlan plz, FftPlan ply,
+                                                                 const float2* __restrict__ twz_g,
+                                                                 const float2* __restrict__ twy_g,
+                                                                 const int* __restrict__ cell_start,
+                                                                 const double* __restrict__ pos_sorted,
+                                                                 const float4* __restrict__ xq,
+                                                                 float2* __restrict__ cg) {
+  extern __shared__ float2 lds[];
+  __shared__ int nrec;
+  const int x0 = blockIdx.x;
+  const int P = a.Ky * a.Kz;
+  float2* twz = lds;
+  float2* twy = lds + a.Kz;
+  float2* A = twy + a.Ky;
+  float2* Bf = A + P;
+  long long* plane = (long long*)A;
+  SpreadRec* rec = (SpreadRec*)Bf;
+  load_twiddles(twz, twz_g, a.Kz);
+  load_twiddles(twy, twy_g, a.Ky);
+  for (int i = threadIdx.x; i < P; i += blockDim.x) plane[i] = 0;
+  int b0, e0, b1, e1;
+  plane_ranges(a, cell_start, x0, &b0, &e0, &b1, &e1);
+  const int len0 = e0 - b0, total = len0 + (e1 - b1);
+  for (int w0 = 0; w0 < total; w0 += a.W) {
+    if (threadIdx.x == 0) nrec = 0;
+    __syncthreads();
+    const int wend = min(total, w0 + a.W);
+    // uniform trip count so every lane reaches the ballots
+    for (int c0 = w0; c0 < wend; c0 += PLANE_U * blockDim.x) {
+      int sl[PLANE_U];
+      double px[PLANE_U];
+#pragma unroll
+      for (int u = 0; u < PLANE_U; ++u) {
+        const int ci = c0 + u * blockDim.x + threadIdx.x;
+        sl[u] = ci < wend ? (ci < len0 ? b0 + ci : b1 + (ci - len0)) : -1;
+        px[u] = sl[u] >= 0 ? pos_sorted[3 * sl[u]] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < PLANE_U; ++u) {
+        bool keep = false;
+        SpreadRec r;
+        if (sl[u] >= 0) {
+          int bx;
+          float wx;
+          frac_coord(px[u], a.Lx, a.Kx, &bx, &wx);
+          int kx = bx - x0;
+          kx += kx < 0 ? a.Kx : 0;
+          if (kx <= 3) {
+            const int s = sl[u];
+            const float q = xq[s].w;
+            int by, bz;
+            frac_coord(pos_sorted[3 * s + 1], a.Ly, a.Ky, &by, &r.wy);
+            frac_coord(pos_sorted[3 * s + 2], a.Lz, a.Kz, &bz, &r.wz);
+            r.yz = by | (bz << 12);
+            r.qx = q * bsp4(wx, kx);
+            keep = q != 0.f;
+          }
+        }
+        const int slot = reserve_slot(keep, &nrec);
+        if (keep) rec[slot] = r;
+      }
+    }
+    __syncthreads();
+    // four lanes per record, one per ky, each adding its 4 z points
+    const int nr = nrec;
+    for (int e = threadIdx.x; e < nr * 4; e += blockDim.x) {
+      const SpreadRec r = rec[e >> 2];
+      const int ky = e & 3;
+      int gy = (r.yz & 4095) - ky;
+      gy += gy < 0 ? a.Ky : 0;
+      const float qxy = r.qx * bsp4(r.wy, ky);
+      const float w = r.wz, w2 = w * w, w3 = w2 * w, om = 1.f - w;
+      const float tz[4] = {w3 * (1.f / 6.f), (-3.f * w3 + 3.f * w2 + 3.f * w + 1.f) * (1.f / 6.f),
+                           (3.f * w3 - 6.f * w2 + 4.f) * (1.f / 6.f), om * om * om * (1.f / 6.f)};
+      const int bz = r.yz >> 12;
+      long long* row = plane + gy * a.Kz;
+#pragma unroll
+      for (int kz = 0; kz < 4; ++kz) {
+        int gz = bz - kz;
+        gz += gz < 0 ? a.Kz : 0;
+        atomicAdd((unsigned long long*)&row[gz], (unsigned long long)to_fix_f(qxy * tz[kz]));
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  // in place: lane i reads int64 element i and writes complex element i (same 8 bytes)
+  for (int i = threadIdx.x; i < P; i += blockDim.x) A[i] = make_float2((float)((double)plane[i] * kFixInv), 0.f);
+  __syncthreads();
+  float2* r1 = lds_fft<-1>(A, Bf, plz, a.Ky, twz);                // z lines: contiguous
+  float2* o1 = (r1 == A) ? Bf : A;
+  float2* r2 = lds_fft<-1>(r1, o1, ply, a.Kz, twy, 1, a.Kz);      // y lines: stride Kz
+  float2* out = cg + (long long)x0 * P;
+  for (int i = threadIdx.x; i < P; i += blockDim.x) out[i] = r2[i];
+}
+
+// inverse FFT along y and z of plane x0, then for every atom touching the plane its share of
+// the reciprocal force, -q (dtheta_x theta_y theta_z, ...) . conv over this plane's 4x4 (y,z)
+// stencil, written to fpart[kx][3*atom + d] (each slot once per evaluation).
+// LDS: twiddles, real part of the convolved plane (floats), two complex planes that the
+// records overlay after the transforms.
+__global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_inv_gather(
+    PlaneArgs a, FftPlan plz, FftPlan ply, const float2* __restrict__ twz_g, const float2* __restrict__ twy_g,
+    const int* __restrict__ sorted_orig, const int* __restrict__ cell_start, const double* __restrict__ pos_sorted,
+    const float4* __restrict__ xq, const float2* __restrict__ cg, double* __restrict__ fpart, DevState* st) {
+  extern __shared__ float2 lds[];
+  __shared__ int nrec;
+  __shared__ int bcount;
+  const int x0 = blockIdx.x;
+  const int P = a.Ky * a.Kz;
+  float2* twz = lds;
+  float2* twy = lds + a.Kz;
+  float* convr = (float*)(twy + a.Ky);
+  float2* A = (float2*)(convr + P + (P & 1));
+  float2* Bf = A + P;
+  GatherRec* rec = (GatherRec*)A;
+  load_twiddles(twz, twz_g, a.Kz);
+  load_twiddles(twy, twy_g, a.Ky);
+  if (threadIdx.x == 0) bcount = 0;
+  const float2* in = cg + (long long)x0 * P;
+  for (int i = threadIdx.x; i < P; i += blockDim.x) A[i] = in[i];
+  __syncthreads();
+  float2* r1 = lds_fft<1>(A, Bf, ply, a.Kz, twy, 1, a.Kz);        // y lines first (reverse order)
+  float2* o1 = (r1 == A) ? Bf : A;
+  float2* r2 = lds_fft<1>(r1, o1, plz, a.Ky, twz);                // z lines
+  for (int i = threadIdx.x; i < P; i += blockDim.x) convr[i] = r2[i].x;
+  int b0, e0, b1, e1;
+  plane_ranges(a, cell_start, x0, &b0, &e0, &b1, &e1);
+  const int len0 = e0 - b0, total = len0 + (e1 - b1);
+  const long long n3 = 3LL * a.n;
+  for (int w0 = 0; w0 < total; w0 += a.W) {
+    if (threadIdx.x == 0) nrec = 0;
+    __syncthreads();  // also orders convr writes and the end of the transforms before the records
+    const int wend = min(total, w0 + a.W);
+    for (int c0 = w0; c0 < wend; c0 += PLANE_U * blockDim.x) {
+      int sl[PLANE_U];
+      double px[PLANE_U];
+#pragma unroll
+      for (int u = 0; u < PLANE_U; ++u) {
+        const int ci = c0 + u * blockDim.x + threadIdx.x;
+        sl[u] = ci < wend ? (ci < len0 ? b0 + ci : b1 + (ci - len0)) : -1;
+        px[u] = sl[u] >= 0 ? pos_sorted[3 * sl[u]] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < PLANE_U; ++u) {
+        bool keep = false;
+        GatherRec r;
+        if (sl[u] >= 0) {
+          int bx;
+          frac_coord(px[u], a.Lx, a.Kx, &bx, &r.wx);
+          int kx = bx - x0;
+          kx += kx < 0 ? a.Kx : 0;
+          if (kx <= 3) {
+            const int s = sl[u];
+            r.q = xq[s].w;
+            r.atom = sorted_orig[s];
+            if (r.q == 0.f) {
+              double* dst = fpart + kx * n3 + 3LL * r.atom;
+              dst[0] = dst[1] = dst[2] = 0.0;
+              atomicAdd(&bcount, 1);
+            } else {
+              int by, bz;
+              frac_coord(pos_sorted[3 * s + 1], a.Ly, a.Ky, &by, &r.wy);
+              frac_coord(pos_sorted[3 * s + 2], a.Lz, a.Kz, &bz, &r.wz);
+              r.yzk = by | (bz << 12) | (kx << 24);
+              keep = true;
+            }
+          }
+        }
+        const int slot = reserve_slot(keep, &nrec);
+        if (keep) rec[slot] = r;
+      }
+    }
+    __syncthreads();
+    const int nr = nrec;
+    // four lanes per record (one per ky, 4 z points each); nr * 4 and blockDim.x are multiples
+    // of 4, so the four lanes of a record stay together through the reduction
+    for (int e = threadIdx.x; e < nr * 4; e += blockDim.x) {
+      const GatherRec r = rec[e >> 2];
+      const int ky = e & 3;
+      int gy = (r.yzk & 4095) - ky;
+      gy += gy < 0 ? a.Ky : 0;
+      const float ty = bsp4(r.wy, ky), dy = dbsp4(r.wy, ky);
+      const float w = r.wz, w2 = w * w, w3 = w2 * w, om = 1.f - w;
+      const float tz[4] = {w3 * (1.f / 6.f), (-3.f * w3 + 3.f * w2 + 3.f * w + 1.f) * (1.f / 6.f),
+                           (3.f * w3 - 6.f * w2 + 4.f) * (1.f / 6.f), om * om * om * (1.f / 6.f)};
+      const float dz[4] = {0.5f * w2, 0.5f * (-3.f * w2 + 2.f * w + 1.f), 0.5f * (3.f * w2 - 4.f * w),
+                           -0.5f * om * om};
+      const int bz = (r.yzk >> 12) & 4095;
+      const float* row = convr + gy * a.Kz;
+      float c = 0.f, cz = 0.f;
+#pragma unroll
+      for (int kz = 0; kz < 4; ++kz) {
+        int gz = bz - kz;
+        gz += gz < 0 ? a.Kz : 0;
+        const float cv = row[gz];
+        c += tz[kz] * cv;
+        cz += dz[kz] * cv;
+      }
+      float sx = ty * c, sy = dy * c, sz = ty * cz;
+#pragma unroll
+      for (int off = 2; off > 0; off >>= 1) {
+        sx += __shfl_xor(sx, off, 64);
+        sy += __shfl_xor(sy, off, 64);
+        sz += __shfl_xor(sz, off, 64);
+      }
+      if (ky == 0) {
+        const int kx = r.yzk >> 24;
+        const float tx = bsp4(r.wx, kx), dx = dbsp4(r.wx, kx);
+        double* dst = fpart + kx * n3 + 3LL * r.atom;
+        dst[0] = -(double)r.q * (double)(dx * sx) * (a.Kx / a.Lx);
+        dst[1] = -(double)r.q * (double)(tx * sy) * (a.Ky / a.Ly);
+        dst[2] = -(double)r.q * (double)(tx * sz) * (a.Kz / a.Lz);
+      }
+    }
+    if (threadIdx.x == 0) bcount += nr;
+    __syncthreads();
+  }
+  // every atom owns exactly one slot per kx: the grid-wide count must be 4 n. The last block
+  // to finish checks it and re-arms the counters for the next evaluation.
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&st->pme_slots, bcount);
+    __threadfence();
+    const unsigned t = atomicAdd(&st->pme_ticket, 1u);
+    if (t == gridDim.x - 1) {
+      __threadfence();
+      const int tot = atomicAdd(&st->pme_slots, 0);
+      if (tot != 4 * a.n) atomicOr(&st->flags, MDX_FLAG_PME_COVERAGE);
+      atomicExch(&st->pme_slots, 0);
+      atomicExch(&st->pme_ticket, 0u);
+    }
+  }
+}
+
+// LDS bytes of the two plane kernels for a record window of W candidates
+static inline size_t plane_fwd_lds(int Ky, int Kz, int W) {
+  const size_t P = (size_t)Ky * Kz;
+  return (size_t)(Ky + Kz) * 8 + std::max(2 * P * 8, P * 8 + (size_t)W * sizeof(SpreadRec));
+}
+static inline size_t plane_gather_lds(int Ky, int Kz, int W) {
+  const size_t P = (size_t)Ky * Kz;
+  return (size_t)(Ky + Kz) * 8 + (P + (P & 1)) * 4 + std::max(2 * P * 8, (size_t)W * sizeof(GatherRec));
+}
+
 // ---------------------------------------------------------------- host
 static int factorize(int K, FftPlan* pl) {
   pl->K = K;
   pl->nrad = 0;
+  pl->code = 0;
   int k = K;
-  while (k % 4 == 0) { pl->radix[pl->nrad++] = 4; k /= 4; }
-  while (k % 2 == 0) { pl->radix[pl->nrad++] = 2; k /= 2; }
-  while (k % 3 == 0) { pl->radix[pl->nrad++] = 3; k /= 3; }
-  while (k % 5 == 0) { pl->radix[pl->nrad++] = 5; k /= 5; }
-  return k == 1 ? 0 : -1;
+  auto push = [&](unsigned long long r) {
+    if (pl->nrad < MAX_RADICES) pl->code |= r << (4 * pl->nrad);
+    pl->nrad++;
+  };
+  while (k % 4 == 0) { push(4); k /= 4; }
+  while (k % 2 == 0) { push(2); k /= 2; }
+  while (k % 3 == 0) { push(3); k /= 3; }
+  while (k % 5 == 0) { push(5); k /= 5; }
+  return (k == 1 && pl->nrad <= MAX_RADICES) ? 0 : -1;
 }
 
 // |b(m)|^-2 of Essmann eq. 4.4 for order 4: M4(1)=1/6, M4(2)=2/3, M4(3)=1/6
@@ -413,6 +782,22 @@ extern "C" int mdx_pme_config(mdx_ctx* c, const int* h_grid, int order, double b
       tw[(size_t)d * Kmax + k] = make_float2((float)cos(a), (float)sin(a));
     }
   MDX_HIP(hipMemcpy(c->twiddle.p, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+  // the plane kernels hold twiddles, two complex Ky x Kz planes and a window of atom records
+  // in LDS; the window is as large as the device allows (at most 8192 candidates)
+  int lds_max = 0;
+  MDX_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+  const size_t budget = lds_max > 1024 ? (size_t)lds_max - 1024 : 0;  // static LDS of the kernels
+  c->pme_wf = c->pme_wg = 0;
+  for (int W = 8192; W >= PLANE_THREADS; W -= PLANE_THREADS)
+    if (!c->pme_wf && plane_fwd_lds(c->K[1], c->K[2], W) <= budget) c->pme_wf = W;
+  for (int W = 8192; W >= PLANE_THREADS; W -= PLANE_THREADS)
+    if (!c->pme_wg && plane_gather_lds(c->K[1], c->K[2], W) <= budget) c->pme_wg = W;
+  c->pme_planes = c->pme_wf > 0 && c->pme_wg > 0 && c->K[1] < 4096 && c->K[2] < 4096;
+  if (c->pme_planes) {  // the whole budget, so contexts with other grids stay launchable
+    MDX_HIP(hipFuncSetAttribute((const void*)k_pme_plane_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)budget));
+    MDX_HIP(hipFuncSetAttribute((const void*)k_pme_plane_inv_gather, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)budget));
+  }
   c->pme_configured = true;
   return 0;
 }
@@ -448,7 +833,7 @@ static int fft_pass(mdx_ctx* c, int axis, int dir, int mode) {
     ln.B = std::min(16, Kz);
   }
   const float2* tw = c->twiddle.p + (size_t)axis * Kmax;
-  size_t lds = 2 * (size_t)ln.B * pl.K * sizeof(float2);
+  size_t lds = (2 * (size_t)ln.B + 1) * pl.K * sizeof(float2);
   dim3 grid(div_up(ln.nlines, ln.B));
 #define FFT_L(D, M)                                                                                    \
   k_fft_axis<D, M><<<grid, FFT_THREADS, lds, c->stream>>>(pl, ln, tw, c->cgrid.p, c->qfix.p, c->conv.p, c->bc.p, \
@@ -463,9 +848,55 @@ static int fft_pass(mdx_ctx* c, int axis, int dir, int mode) {
   return 0;
 }
 
+// plane path: spread + 2D forward FFT per x-plane, x-axis pass with the influence function,
+// 2D inverse FFT + gather per x-plane. Needs the cell-sorted atom order of the neighbour-list
+// grid (current for this evaluation) and two complex planes in LDS.
+static int pme_enqueue_planes(mdx_ctx* c) {
+  const int Kx = c->K[0], Ky = c->K[1], Kz = c->K[2];
+  PlaneArgs a;
+  a.n = c->n;
+  a.Kx = Kx;
+  a.Ky = Ky;
+  a.Kz = Kz;
+  a.Lx = c->box[0];
+  a.Ly = c->box[1];
+  a.Lz = c->box[2];
+  a.gnx = c->gn[0];
+  a.gny = c->gn[1];
+  a.gnz = c->gn[2];
+  a.wx = c->box[0] / c->gn[0];
+  a.margin = 0.5 * c->skin + 1e-6 * c->box[0] + 1e-9;
+  FftPlan plz, ply;
+  factorize(Kz, &plz);
+  factorize(Ky, &ply);
+  const int Kmax = std::max(Kx, std::max(Ky, Kz));
+  const float2* twy = c->twiddle.p + (size_t)Kmax;
+  const float2* twz = c->twiddle.p + 2 * (size_t)Kmax;
+  a.W = c->pme_wf;
+  k_pme_plane_fwd<<<Kx, PLANE_THREADS, plane_fwd_lds(Ky, Kz, a.W), c->stream>>>(
+      a, plz, ply, twz, twy, c->cell_start.p, c->pos_sorted.p, c->xq.p, c->cgrid.p);
+  MDX_LAUNCH_CHECK();
+  int rc;
+  if ((rc = fft_pass(c, 0, -1, 3))) return rc;  // x forward, * B C, energy, x inverse
+  a.W = c->pme_wg;
+  k_pme_plane_inv_gather<<<Kx, PLANE_THREADS, plane_gather_lds(Ky, Kz, a.W), c->stream>>>(
+      a, plz, ply, twz, twy, c->sorted_orig.p, c->cell_start.p, c->pos_sorted.p, c->xq.p, c->cgrid.p, c->fpart.p,
+      c->dstate.p);
+  MDX_LAUNCH_CHECK();
+  return 0;
+}
+
 int mdx_pme_enqueue(mdx_ctx* c, bool energy) {
   (void)energy;
   MDX_CHECK(c->pme_configured, MDX_ERR_INVALID, "PME not configured");
+  if (c->pme_planes && c->nb_configured) {
+    int rc;
+    if ((rc = c->fpart.ensure(12 * (size_t)c->n))) return rc;
+    if ((rc = pme_enqueue_planes(c))) return rc;
+    c->fpart_live = true;
+    return mdx_pme_enqueue_corrections(c);
+  }
+  c->fpart_live = false;
   PmeArgs a;
   a.n = c->n;
   a.Kx = c->K[0];

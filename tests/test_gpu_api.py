@@ -188,6 +188,9 @@ def test_water_simulation_through_api():
     assert sim.cur_step == 200 and len(log.records) == 4
     T = [r["temperature"] for r in log.records]
     say("  temperatures", ["%.0f" % x for x in T])
-    assert all(100 < x < 600 for x in T)
+    # the ideal lattice melts in the first ~50 fs and its potential energy heats it to ~600 K
+    # (+-20 K at 648 atoms); the thermostat (1/gamma = 100 fs) then pulls it back down
+    assert all(100 < x < 1000 for x in T)
+    assert T[-1] < T[0]
     x = ens.state.positions
     assert np.all(np.abs(x) <= 18.6 / 2 + 1e-6)  # wrapped to [-L/2, L/2]
