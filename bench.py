@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--energy-every", type=int, default=50,
                     help="potential energies every this many steps (forces every step)")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="after the run, one device-to-device copy of this many MiB (a kernel of known "
+                         "traffic for scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (replicas shard "
@@ -139,6 +142,13 @@ def main():
         except Exception as ex:  # reported, never silently replaced
             out["cpu_baseline"] = {"value": None, "error": repr(ex)}
     print(json.dumps(out), flush=True)
+    if args.calib_copy_mb > 0:
+        n = args.calib_copy_mb * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":

@@ -145,7 +145,8 @@ int mdx_set_energy_interval(mdx_ctx* ctx, int every);
  * h_out[k*(MDX_NTERMS+1) + t] for k < min(rows, h_cap); *h_count = rows. Synchronises. */
 int mdx_energy_history_fetch(mdx_ctx* ctx, double* h_out, int h_cap, int* h_count);
 /* Status flags raised on the device since the last call (bit 0 overflow, 1 particle loss,
- * 2 non-finite); clears them. Synchronises. */
+ * 2 non-finite, 3 PME coverage: an atom missed by the reciprocal-space gather); clears them.
+ * Synchronises. */
 int mdx_status_fetch(mdx_ctx* ctx, unsigned* h_flags);
 
 /* ---- instrumentation ------------------------------------------------------------------- */
