@@ -55,6 +55,69 @@ def cpu_baseline(system, steps=2):
                       "%.1f s wall; steady-state step = 3 evaluations" % (steps, evals, wall)}
 
 
+def replicas_leg(steps=200):
+    """64 uneven water boxes (300-3000 waters, r_c 9 A): aggregate atom-steps/s per batch mode."""
+    import torch
+    from mdpy_amd import systems
+    from mdpy_amd.replicas import ReplicaBatch
+    reps = systems.water_replicas(64, seed=0)
+    b = ReplicaBatch(reps, energy_every=50)
+    skins = b.autotune_skins(candidates=(1.0, 1.5, 2.0), steps=30, warmup=10)
+    rates = {}
+    for mode in ("sequential", "streams", "one_graph"):
+        b.integrate(20, mode)
+        rates[mode] = b.atom_steps_per_s(steps, mode)
+    torch.cuda.synchronize()
+    best = max(rates, key=rates.get)
+    out = {"metric": "aggregate atom-steps/s, 64 water replicas", "atoms": b.n_atoms, "steps": steps,
+           "rates": rates, "best_mode": best, "speedup_vs_sequential": rates[best] / rates["sequential"],
+           "skins": {str(k): skins.count(k) for k in sorted(set(skins))}, "device_flags": max(b.status())}
+    b.close()
+    return out
+
+
+def mlp_leg(s, e, terms, plain_ns_day, n_frames=300, every=10, epochs=4):
+    """Train the surrogate on frames of this box and measure its step rate and force error."""
+    import torch
+    from mdpy_amd import _lib as L
+    from mdpy_amd.mlp import Surrogate, elements_of, force_rmse_kj
+    xs, fs = [], []
+    for _ in range(n_frames):
+        e.integrate(terms, every, True, 50)
+        f, _ = e.compute(L.TERM_LJ | L.TERM_PME)
+        xs.append(e.pos.clone())
+        fs.append(f.clone())
+    x_keep = e.pos.clone()
+    sur = Surrogate(e, elements_of(s.types), rd=6.0, skin=1.0, seed=0)
+    sur.fit_norm(xs[:20])
+    ntr = int(0.9 * n_frames)
+    t0 = time.perf_counter()
+    hist = sur.train(xs[:ntr], fs[:ntr], epochs=epochs, lr=1e-3, lr_decay=0.7, log_every=25)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    rm = []
+    for x, f in zip(xs[ntr:], fs[ntr:]):
+        e.set_positions(x, wrap=False)
+        fp, _ = sur.eval()
+        rm.append(force_rmse_kj(fp.cpu().numpy(), f.cpu().numpy()))
+    zero = float(np.mean([force_rmse_kj(np.zeros((s.num_atoms, 3)), f.cpu().numpy()) for f in fs[ntr:]]))
+    e.set_positions(x_keep, wrap=False)
+    sur.attach(True)
+    mterms = L.TERM_MLP | L.TERM_BOND | L.TERM_ANGLE
+    e.integrate(mterms, 100, True, 50)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.integrate(mterms, 1000, True, 50)
+    torch.cuda.synchronize()
+    sps = (time.perf_counter() - t0) / 1000
+    sur.close()
+    return {"frames": n_frames, "train_frames": ntr, "epochs": epochs, "train_s": t_train,
+            "loss_by_epoch": hist, "force_rmse_kj_mol_nm": float(np.mean(rm)),
+            "force_rms_kj_mol_nm_no_surrogate": zero, "ns_per_day": 0.0864 / sps,
+            "speedup_vs_plain": (0.0864 / sps) / plain_ns_day,
+            "note": "surrogate of LJ + PME; fallback to the plain path when RMSE > 0.5 kJ/mol/nm"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +129,11 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--energy-every", type=int, default=50,
                     help="potential energies every this many steps (forces every step)")
+    ap.add_argument("--lanes", type=int, default=0, help="nonbonded lanes per atom (0 = autotune)")
+    ap.add_argument("--replicas", action="store_true",
+                    help="also measure the 64 batched water replicas (aggregate atom-steps/s per batch mode)")
+    ap.add_argument("--mlp", action="store_true",
+                    help="also train and measure the MLP surrogate on this box (adds a few minutes)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB (a kernel of known "
                          "traffic for scaling hardware byte counters)")
@@ -89,6 +157,10 @@ def main():
     use_graph = not args.no_graph
 
     e.integrate(terms, args.warmup, use_graph, args.energy_every)
+    # autotune the nonbonded kernel's lanes per atom on the current list, then re-warm the graph
+    lanes = e.tune_nonbonded(L.TERM_LJ | L.TERM_COUL_DIRECT, 0, reps=20) if args.lanes == 0 else \
+        e.tune_nonbonded(L.TERM_LJ | L.TERM_COUL_DIRECT, args.lanes)
+    e.integrate(terms, min(args.warmup, 50), use_graph, args.energy_every)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e.integrate(terms, args.steps, use_graph, args.energy_every)
@@ -123,7 +195,7 @@ def main():
         "data": "synthetic seeded water box (no files)",
         "config": {"workload": "water23328: 7,776 TIP3P-like waters, 61.9 A cube, r_c 12 A, PME 64^3 "
                                "order 4, G-JF Langevin 300 K, dt 1 fs, plain FP32 path",
-                   "atoms": s.num_atoms, "list_skin_A": args.skin, "hipgraph": use_graph,
+                   "atoms": s.num_atoms, "list_skin_A": args.skin, "hipgraph": use_graph, "nb_lanes": lanes,
                    "energy_every": args.energy_every,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "kernel": "k_nonbonded (LJ + erfc Coulomb)",
@@ -136,6 +208,10 @@ def main():
         "energies_finite": finite,
         "device_flags": status,
     }
+    if args.replicas:
+        out["replicas"] = replicas_leg()
+    if args.mlp:
+        out["mlp"] = mlp_leg(s, e, terms, ns_day)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(s, args.cpu_steps)

@@ -44,7 +44,8 @@ extern "C" {
 #define MDX_TERM_DIHEDRAL      (1u << 5)
 #define MDX_TERM_IMPROPER      (1u << 6)
 #define MDX_TERM_COUL_PLAIN    (1u << 7)  /* plain Coulomb, all pairs, minimum image, no cutoff  */
-#define MDX_NTERMS             8
+#define MDX_TERM_MLP           (1u << 8)  /* learned surrogate of LJ + PME (mdx_mlp_*)          */
+#define MDX_NTERMS             9
 /* energy slot of the kinetic energy in mdx_energies_fetch output (after the MDX_NTERMS terms) */
 #define MDX_E_KINETIC          MDX_NTERMS
 
@@ -59,6 +60,7 @@ extern "C" {
 #define MDX_INT_LANGEVIN_REF    3 /* langevin_integrator.py:37-72 as written                  */
 
 typedef struct mdx_ctx mdx_ctx;
+typedef struct mdx_mlp mdx_mlp;
 
 /* ---- library / context ------------------------------------------------------------------ */
 const char* mdx_last_error(void);
@@ -149,6 +151,53 @@ int mdx_energy_history_fetch(mdx_ctx* ctx, double* h_out, int h_cap, int* h_coun
  * Synchronises. */
 int mdx_status_fetch(mdx_ctx* ctx, unsigned* h_flags);
 
+/* ---- batched replicas (new; no reference counterpart) ---------------------------------- */
+/* Many independent systems, one context each (same device), advanced together: each context
+ * keeps its own box, lists, PME grid, integrator and graph. Created on the caller's stream
+ * `hip_stream` (NULL = default); the contexts are borrowed and must outlive the batch. */
+typedef struct mdx_batch mdx_batch;
+#define MDX_BATCH_ONE_GRAPH   0  /* one hipGraph with a branch per replica                      */
+#define MDX_BATCH_STREAMS     1  /* each replica replays its own graph on its own stream         */
+#define MDX_BATCH_SEQUENTIAL  2  /* replica after replica (the per-replica loop, for comparison) */
+int mdx_batch_create(mdx_ctx** ctxs, int n_ctx, void* hip_stream, mdx_batch** out);
+/* Advance every replica `nsteps` steps of `terms` (each with its own integrator settings; all
+ * replicas need the same energy interval). Energy histories are per context, as after
+ * mdx_integrate. */
+int mdx_batch_integrate(mdx_batch* batch, unsigned terms, int nsteps, int mode);
+int mdx_batch_destroy(mdx_batch* batch);
+
+/* ---- learned surrogate (new; no reference counterpart) ---------------------------------- */
+/* A per-atom MLP (descriptors within rd of each atom: 32 radial Gaussians + 16 angular moment
+ * invariants per neighbour element; per element 96 -> 256 -> 256 -> 256 -> 1, SiLU, BF16 on
+ * MFMA) whose energy gradient stands in for LJ + PME. h_elem: element per atom (0 = O, 1 = H).
+ * The surrogate keeps its own rd (+ skin) neighbour list on the host context's positions and
+ * runs on the host context's stream. Parameters per element: mdx_mlp_param_count() floats
+ * (W1 96x256 | b1 | W2 256x256 | b2 | W3 256x256 | b3 | w4 256 | b4), row-major. */
+int mdx_mlp_create(mdx_ctx* ctx, const int* h_elem, double rd, double skin, mdx_mlp** out);
+int mdx_mlp_destroy(mdx_mlp* mlp);
+int mdx_mlp_param_count(void);
+int mdx_mlp_set_params(mdx_mlp* mlp, int elem, const float* h_params);
+int mdx_mlp_get_params(mdx_mlp* mlp, int elem, float* h_params);
+/* feature normalisation x = (G - mean) / std (96 each) */
+int mdx_mlp_set_norm(mdx_mlp* mlp, const float* h_mean, const float* h_std);
+/* descriptors G (N x 96 f32, device, original atom order) at the host context's positions */
+int mdx_mlp_descriptors(mdx_mlp* mlp, float* d_out);
+/* surrogate forces (N x 3 f64, device; may be NULL) and total energy (host; may be NULL) */
+int mdx_mlp_eval(mdx_mlp* mlp, double* d_force_out, double* h_energy);
+/* attach to the host context: MDX_TERM_MLP in mdx_compute_forces / mdx_integrate then adds the
+ * surrogate's forces (energy slot MDX_TERM_MLP's bit index). NULL detaches. */
+int mdx_ctx_set_mlp(mdx_ctx* ctx, mdx_mlp* mlp);
+/* One force-matching step (loss = mean squared force error per component) on one frame
+ * (d_pos, d_fref: N x 3 f64 device). lr > 0 applies an Adam update; the loss goes to *h_loss
+ * when not NULL (synchronises). */
+int mdx_mlp_train_step(mdx_mlp* mlp, const double* d_pos, const double* d_fref, double lr, double* h_loss);
+int mdx_mlp_adam_config(mdx_mlp* mlp, double beta1, double beta2, double eps);
+/* gradients of the last mdx_mlp_train_step (same layout as the parameters) */
+int mdx_mlp_get_grads(mdx_mlp* mlp, int elem, float* h_grad);
+/* C (M x N f32) = A (M x K bf16) B (K x N bf16), all row-major device arrays: the MFMA GEMM
+ * the dense layers use, exposed for testing */
+int mdx_mlp_gemm_check(mdx_mlp* mlp, int M, int N, int K, const void* d_A, const void* d_B, float* d_C);
+
 /* ---- instrumentation ------------------------------------------------------------------- */
 /* Number of neighbour-list rebuilds and evaluations since the context was created. */
 int mdx_stats_fetch(mdx_ctx* ctx, int64_t* h_rebuilds, int64_t* h_evals);
@@ -156,6 +205,10 @@ int mdx_stats_fetch(mdx_ctx* ctx, int64_t* h_rebuilds, int64_t* h_evals);
  * stream: which 0 = nonbonded kernel (current list, forces only), 1 = force evaluation of
  * `terms`, 2 = PME reciprocal chain, 3 = one integrator step of `terms`. Synchronises. */
 int mdx_time_kernel(mdx_ctx* ctx, int which, unsigned terms, int reps, double* h_avg_ms);
+/* Lanes per atom of the nonbonded kernel: 16, 32 or 64 set directly, or 0 = time each variant
+ * on the current neighbour list (`reps` forces-only launches of `terms`' LJ/direct parts) and
+ * keep the fastest. The choice goes to *h_lanes (may be NULL). Synchronises. */
+int mdx_nonbonded_tune(mdx_ctx* ctx, unsigned terms, int lanes, int reps, int* h_lanes);
 /* Candidate (list) pair slots and in-cutoff unique pairs of the last nonbonded evaluation
  * (counts the list on the device; for roofline accounting; synchronises). */
 int mdx_pair_counts(mdx_ctx* ctx, int64_t* h_list_slots, int64_t* h_unique_in_cutoff);

@@ -27,12 +27,13 @@ TERM_ANGLE = 1 << 4
 TERM_DIHEDRAL = 1 << 5
 TERM_IMPROPER = 1 << 6
 TERM_COUL_PLAIN = 1 << 7
-NTERMS = 8
+TERM_MLP = 1 << 8
+NTERMS = 9
 E_KINETIC = NTERMS
 TERM_PME = TERM_COUL_DIRECT | TERM_COUL_RECIP
 TERM_BONDED = TERM_BOND | TERM_ANGLE | TERM_DIHEDRAL | TERM_IMPROPER
 TERM_NAMES = ["lj", "coulomb_direct", "coulomb_reciprocal", "bond", "angle", "dihedral", "improper",
-              "coulomb_plain", "kinetic"]
+              "coulomb_plain", "mlp", "kinetic"]
 
 PAIRS_EXACT = 0
 PAIRS_REFERENCE = 1
@@ -41,6 +42,11 @@ INT_VELOCITY_VERLET = 0
 INT_LANGEVIN = 1
 INT_VERLET_REF = 2
 INT_LANGEVIN_REF = 3
+
+BATCH_ONE_GRAPH = 0
+BATCH_STREAMS = 1
+BATCH_SEQUENTIAL = 2
+BATCH_MODES = {"one_graph": BATCH_ONE_GRAPH, "streams": BATCH_STREAMS, "sequential": BATCH_SEQUENTIAL}
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_dbl_p = ctypes.POINTER(ctypes.c_double)
@@ -85,6 +91,23 @@ SIGNATURES = {
     "mdx_stats_fetch": (ctypes.c_int, [_vp, _c_i64_p, _c_i64_p]),
     "mdx_pair_counts": (ctypes.c_int, [_vp, _c_i64_p, _c_i64_p]),
     "mdx_time_kernel": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint, ctypes.c_int, _c_dbl_p]),
+    "mdx_nonbonded_tune": (ctypes.c_int, [_vp, ctypes.c_uint, ctypes.c_int, ctypes.c_int, _c_int_p]),
+    "mdx_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _vp, ctypes.POINTER(_vp)]),
+    "mdx_batch_integrate": (ctypes.c_int, [_vp, ctypes.c_uint, ctypes.c_int, ctypes.c_int]),
+    "mdx_batch_destroy": (ctypes.c_int, [_vp]),
+    "mdx_mlp_create": (ctypes.c_int, [_vp, _c_int_p, ctypes.c_double, ctypes.c_double, ctypes.POINTER(_vp)]),
+    "mdx_mlp_destroy": (ctypes.c_int, [_vp]),
+    "mdx_mlp_param_count": (ctypes.c_int, []),
+    "mdx_mlp_set_params": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+    "mdx_mlp_get_params": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+    "mdx_mlp_set_norm": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "mdx_mlp_descriptors": (ctypes.c_int, [_vp, _vp]),
+    "mdx_mlp_eval": (ctypes.c_int, [_vp, _vp, _c_dbl_p]),
+    "mdx_ctx_set_mlp": (ctypes.c_int, [_vp, _vp]),
+    "mdx_mlp_train_step": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _c_dbl_p]),
+    "mdx_mlp_adam_config": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "mdx_mlp_get_grads": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+    "mdx_mlp_gemm_check": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -240,6 +240,117 @@ __global__ void k_gjf_a(IntArgs A, double* __restrict__ x, const double* __restr
                     A.b * A.sigma * A.dt * sqrt(A.dt) * 0.5 * g[d] * sm;
   }
 }
+// Fused step prologue (G-JF position half or velocity-Verlet first half) for contexts with a
+// neighbour-list grid: besides the integrator update each lane zeroes its atom's force
+// accumulator, measures the displacement since the list build and writes the atom's packed
+// sorted-order copy (valid unless the list is rebuilt, which repacks). The last workgroup to
+// finish records the previous step's energies, advances the step counter, clears the energy
+// accumulators and, when some atom moved more than skin/2, zeroes the cell counts for the
+// rebuild kernels that follow.
+struct StepPro {
+  long long* ffix;
+  const double* pos_build;
+  const int* orig_sorted;
+  const float* qs;
+  float4* xq;
+  double* pos_sorted;
+  long long* efix;
+  int ne;
+  double* ehist;
+  int ehist_cap, every, record;
+  int* cell_count;
+  int ncell;
+  double thr2;
+};
+
+__device__ void prologue_atom(const IntArgs& A, const StepPro& P, DevState* st, int i, const double* xn, double* d2,
+                              bool* bad) {
+  for (int d = 0; d < 3; ++d) P.ffix[(long long)d * A.n + i] = 0;
+  for (int d = 0; d < 3; ++d) {
+    if (!isfinite(xn[d])) *bad = true;
+    double dd = xn[d] - P.pos_build[3 * i + d];
+    dd -= A.L[d] * rint(dd * A.inv[d]);
+    *d2 += dd * dd;
+  }
+  const int s = P.orig_sorted[i];
+  pack_atom(xn[0], xn[1], xn[2], P.qs ? P.qs[i] : 0.f, A.L[0], A.L[1], A.L[2], &P.xq[s], &P.pos_sorted[3 * s]);
+  (void)st;
+}
+
+__device__ void prologue_block_end(const StepPro& P, DevState* st, double d2, bool bad) {
+  __shared__ double sm[4];
+  __shared__ int last;
+  if (bad) atomicOr(&st->flags, MDX_FLAG_NONFINITE);
+  for (int off = 32; off > 0; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off, 64));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = d2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, sm[w]);
+    atomicMax(&st->max_disp2_bits, enc_double_order(v));
+    __threadfence();
+    last = atomicAdd(&st->step_ticket, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const bool rebuild = atomicAdd(&st->rebuild, 0) != 0 ||
+                       dec_double_order(atomicAdd(&st->max_disp2_bits, 0ull)) > P.thr2;
+  if (rebuild)
+    for (int k = threadIdx.x; k < P.ncell; k += blockDim.x) P.cell_count[k] = 0;
+  if (threadIdx.x == 0) {
+    if (P.record) {
+      const long long k = st->step - st->hist_base;
+      if (k >= 0 && k % P.every == 0 && k / P.every < P.ehist_cap)
+        for (int t = 0; t < P.ne; ++t) P.ehist[(k / P.every) * P.ne + t] = (double)P.efix[t] * kFixInv;
+      atomicAdd((unsigned long long*)&st->step, 1ull);
+    }
+    for (int t = 0; t < P.ne; ++t) atomicExch((unsigned long long*)&P.efix[t], 0ull);
+    if (rebuild) atomicAdd((unsigned long long*)&st->nrebuild, 1ull);
+    atomicExch(&st->step_ticket, 0u);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gjf_a_pro(IntArgs A, StepPro P, double* __restrict__ x,
+                                                   const double* __restrict__ v, const double* __restrict__ im,
+                                                   double* __restrict__ acc_prev, DevState* st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  bool bad = false;
+  if (i < A.n) {
+    double g[3], xn[3];
+    noise3(A.ns, st->step + 1, i, A.n, g);
+    const double sm = sqrt(im[i]);
+    for (int d = 0; d < 3; ++d) {
+      const double acc = fix_force(P.ffix, A, i, d) * im[i];
+      acc_prev[3 * i + d] = acc;
+      xn[d] = x[3 * i + d] + A.b * A.dt * v[3 * i + d] + A.b * A.dt * A.dt * 0.5 * acc +
+              A.b * A.sigma * A.dt * sqrt(A.dt) * 0.5 * g[d] * sm;
+      x[3 * i + d] = xn[d];
+    }
+    prologue_atom(A, P, st, i, xn, &d2, &bad);
+  }
+  prologue_block_end(P, st, d2, bad);
+}
+
+__global__ __launch_bounds__(256) void k_vv_a_pro(IntArgs A, StepPro P, double* __restrict__ x, double* __restrict__ v,
+                                                  const double* __restrict__ im, DevState* st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  bool bad = false;
+  if (i < A.n) {
+    double xn[3];
+    for (int d = 0; d < 3; ++d) {
+      const double vv = v[3 * i + d] + 0.5 * A.dt * fix_force(P.ffix, A, i, d) * im[i];
+      v[3 * i + d] = vv;
+      xn[d] = x[3 * i + d] + A.dt * vv;
+      x[3 * i + d] = xn[d];
+    }
+    prologue_atom(A, P, st, i, xn, &d2, &bad);
+  }
+  prologue_block_end(P, st, d2, bad);
+}
+
 // velocity half: v_{n+1} = a v_n + dt/2 (a a_n + a_{n+1}) + b sigma sqrt(dt) xi_{n+1}/sqrt(m)
 __global__ void k_gjf_b(IntArgs A, double* __restrict__ v, const double* __restrict__ im, const double* __restrict__ m,
                         const long long* __restrict__ ffix, const double* __restrict__ acc_prev, const DevState* st,
@@ -443,12 +554,14 @@ extern "C" int mdx_ctx_destroy(mdx_ctx* c) {
   c->d_stencil.release(); c->qfix.release(); c->cgrid.release(); c->conv.release(); c->bc.release();
   c->twiddle.release(); c->excl_pairs.release(); c->xa.release(); c->xb.release(); c->va.release();
   c->vb.release(); c->acc_prev.release(); c->acc_cur.release(); c->ehist.release(); c->tmp64.release();
-  hipStreamDestroy(c->stream);
+  c->pos_sorted.release(); c->fpart.release();
+  if (c->owns_stream) hipStreamDestroy(c->stream);
   delete c;
   return 0;
 }
 
-static void invalidate_graph(mdx_ctx* c) {
+void mdx_invalidate_graph(mdx_ctx* c) {
+  c->graph_gen++;
   if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
   c->graph_exec = nullptr;
   c->graph_kind = -1;
@@ -458,7 +571,7 @@ extern "C" int mdx_system_set(mdx_ctx* c, int n, double* d_pos, double* d_vel, c
   MDX_CHECK(c && n > 0 && d_pos && h_box, MDX_ERR_INVALID, "mdx_system_set: bad arguments");
   MDX_CHECK(n < (1 << 30), MDX_ERR_INVALID, "too many atoms");
   MDX_HIP(hipSetDevice(c->device));
-  invalidate_graph(c);
+  mdx_invalidate_graph(c);
   bool resized = c->n != n;
   c->n = n;
   c->pos = d_pos;
@@ -486,7 +599,7 @@ extern "C" int mdx_set_box(mdx_ctx* c, const double* h_box) {
     MDX_CHECK(h_box[d] > 0, MDX_ERR_INVALID, "box edges must be positive");
     c->box[d] = h_box[d];
   }
-  invalidate_graph(c);
+  mdx_invalidate_graph(c);
   int rc;
   if (c->nb_configured && (rc = mdx_nlist_prepare(c))) return rc;
   if (c->pme_configured) {
@@ -506,6 +619,7 @@ static int upload(DevBuf<T>& b, const T* h, size_t count) {
 
 extern "C" int mdx_set_masses(mdx_ctx* c, const double* h_mass) {
   MDX_CHECK(c && c->n > 0 && h_mass, MDX_ERR_INVALID, "mdx_set_masses: bad arguments");
+  mdx_invalidate_graph(c);  // parameter buffers may move
   std::vector<double> im(c->n);
   for (int i = 0; i < c->n; ++i) {
     MDX_CHECK(h_mass[i] > 0, MDX_ERR_INVALID, "mass of atom %d is not positive", i);
@@ -519,6 +633,7 @@ extern "C" int mdx_set_masses(mdx_ctx* c, const double* h_mass) {
 
 extern "C" int mdx_set_charges(mdx_ctx* c, const double* h_q) {
   MDX_CHECK(c && c->n > 0 && h_q, MDX_ERR_INVALID, "mdx_set_charges: bad arguments");
+  mdx_invalidate_graph(c);  // parameter buffers may move
   const double sk = sqrt(mdx_coulomb_constant());
   std::vector<float> qs(c->n);
   c->qsum = 0;
@@ -536,6 +651,7 @@ extern "C" int mdx_set_charges(mdx_ctx* c, const double* h_q) {
 
 extern "C" int mdx_set_lj(mdx_ctx* c, const double* h_lj) {
   MDX_CHECK(c && c->n > 0 && h_lj, MDX_ERR_INVALID, "mdx_set_lj: bad arguments");
+  mdx_invalidate_graph(c);  // parameter buffers may move
   for (int i = 0; i < 4 * c->n; ++i)
     MDX_CHECK(!((i % 2 == 0) && h_lj[i] < 0), MDX_ERR_INVALID, "LJ epsilon must be >= 0 (atom %d)", i / 4);
   int rc = upload(c->lj64, h_lj, 4 * (size_t)c->n);
@@ -562,6 +678,7 @@ static int set_csr(mdx_ctx* c, const int* ptr, const int* idx, DevBuf<int>& dp, 
 
 extern "C" int mdx_set_exclusions(mdx_ctx* c, const int* h_ptr, const int* h_idx) {
   MDX_CHECK(c && c->n > 0, MDX_ERR_INVALID, "mdx_set_exclusions: no system");
+  mdx_invalidate_graph(c);  // parameter buffers may move
   int rc = set_csr(c, h_ptr, h_idx, c->excl_ptr, c->excl_idx, &c->max_excl);
   if (rc) return rc;
   c->has_excl = h_ptr[c->n] > 0;
@@ -583,6 +700,7 @@ extern "C" int mdx_set_exclusions(mdx_ctx* c, const int* h_ptr, const int* h_idx
 
 extern "C" int mdx_set_scaling(mdx_ctx* c, const int* h_ptr, const int* h_idx) {
   MDX_CHECK(c && c->n > 0, MDX_ERR_INVALID, "mdx_set_scaling: no system");
+  mdx_invalidate_graph(c);  // parameter buffers may move
   int rc = set_csr(c, h_ptr, h_idx, c->s14_ptr, c->s14_idx, &c->max_s14);
   if (rc) return rc;
   c->has_s14 = h_ptr[c->n] > 0;
@@ -594,6 +712,7 @@ extern "C" int mdx_set_scaling(mdx_ctx* c, const int* h_ptr, const int* h_idx) {
 static int set_terms(mdx_ctx* c, int count, int width_i, int width_p, const int* idx, const double* par,
                      DevBuf<int>& di, DevBuf<double>& dp, int* out_count) {
   MDX_CHECK(c && c->n > 0 && count >= 0, MDX_ERR_INVALID, "bad bonded-term arguments");
+  mdx_invalidate_graph(c);  // parameter buffers may move
   for (long long k = 0; k < (long long)count * width_i; ++k)
     MDX_CHECK(idx[k] >= 0 && idx[k] < c->n, MDX_ERR_INVALID, "bonded-term atom index out of range");
   int rc;
@@ -624,7 +743,7 @@ extern "C" int mdx_nonbonded_config(mdx_ctx* c, double cutoff, double skin, int 
   c->skin = pair_mode == MDX_PAIRS_REFERENCE ? 0.0 : skin;
   c->pair_mode = pair_mode;
   c->cell_cutoff = cell_cutoff > 0 ? cell_cutoff : 12.0;
-  invalidate_graph(c);
+  mdx_invalidate_graph(c);
   int rc = mdx_nlist_prepare(c);
   if (rc) return rc;
   c->nb_configured = true;
@@ -667,6 +786,9 @@ int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy) {
   if (terms & MDX_TERM_COUL_PLAIN) {
     MDX_CHECK(c->has_charges, MDX_ERR_INVALID, "Coulomb requested but no charges set");
     if ((rc = mdx_enqueue_coulomb_plain(c))) return rc;
+  }
+  if (terms & MDX_TERM_MLP) {
+    if ((rc = mdx_mlp_enqueue(c, energy))) return rc;
   }
   if ((rc = mdx_bonded_enqueue(c, terms))) return rc;
   return 0;
@@ -728,7 +850,7 @@ extern "C" int mdx_integrator_config(mdx_ctx* c, int kind, double dt, double gam
   c->kT = kT;
   c->seed = seed;
   c->int_configured = true;
-  invalidate_graph(c);
+  mdx_invalidate_graph(c);
   return 0;
 }
 
@@ -751,7 +873,7 @@ extern "C" int mdx_set_external_noise(mdx_ctx* c, const double* d_noise, int nst
   c->ext_noise = d_noise;
   c->ext_noise_steps = d_noise ? nsteps : 0;
   c->ext_noise_base = step + 1;  // first noise row is used by the next step
-  invalidate_graph(c);
+  mdx_invalidate_graph(c);
   return 0;
 }
 
@@ -783,11 +905,56 @@ static int zero_and_record(mdx_ctx* c) {
 }
 
 // One MD step; the launch sequence depends only on (kind, terms), so it can be captured.
+static StepPro step_pro(mdx_ctx* c) {
+  StepPro P;
+  P.ffix = c->ffix.p;
+  P.pos_build = c->pos_at_build.p;
+  P.orig_sorted = c->orig_sorted.p;
+  P.qs = c->has_charges ? c->charge_s.p : nullptr;
+  P.xq = c->xq.p;
+  P.pos_sorted = c->pos_sorted.p;
+  P.efix = c->efix.p;
+  P.ne = MDX_NTERMS + 1;
+  P.ehist = c->ehist.p;
+  P.ehist_cap = c->ehist_cap;
+  P.every = c->energy_every;
+  P.record = 1;
+  P.cell_count = c->cell_count.p;
+  P.ncell = c->ncell;
+  const double half = 0.5 * c->skin;
+  P.thr2 = c->skin > 0 ? half * half : -1.0;
+  return P;
+}
+
 int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy) {
   IntArgs A = int_args(c);
   dim3 g(div_up(c->n, 256)), b(256);
   int rc;
   long long* ke = c->efix.p + MDX_E_KINETIC;
+  // the fused prologue applies when this step's forces walk the list grid (pair terms or the
+  // PME plane kernels) of an exact-pair-set context
+  const bool grid_used = c->nb_configured && c->pair_mode == MDX_PAIRS_EXACT &&
+                         (terms & (MDX_TERM_LJ | MDX_TERM_COUL_DIRECT | MDX_TERM_COUL_RECIP)) != 0 &&
+                         ((terms & (MDX_TERM_LJ | MDX_TERM_COUL_DIRECT)) || c->pme_planes);
+  if (grid_used && (c->int_kind == MDX_INT_LANGEVIN || c->int_kind == MDX_INT_VELOCITY_VERLET)) {
+    StepPro P = step_pro(c);
+    if (c->int_kind == MDX_INT_LANGEVIN)
+      k_gjf_a_pro<<<g, b, 0, c->stream>>>(A, P, c->pos, c->vel, c->inv_mass.p, c->acc_prev.p, c->dstate.p);
+    else
+      k_vv_a_pro<<<g, b, 0, c->stream>>>(A, P, c->pos, c->vel, c->inv_mass.p, c->dstate.p);
+    MDX_LAUNCH_CHECK();
+    c->prologue_done = true;
+    rc = mdx_enqueue_forces(c, terms, energy);
+    c->prologue_done = false;
+    if (rc) return rc;
+    A.fpart = live_fpart(c);
+    if (c->int_kind == MDX_INT_LANGEVIN)
+      k_gjf_b<<<g, b, 0, c->stream>>>(A, c->vel, c->inv_mass.p, c->mass.p, c->ffix.p, c->acc_prev.p, c->dstate.p, ke);
+    else
+      k_vv_b<<<g, b, 0, c->stream>>>(A, c->vel, c->inv_mass.p, c->mass.p, c->ffix.p, ke);
+    MDX_LAUNCH_CHECK();
+    return 0;
+  }
   switch (c->int_kind) {
     case MDX_INT_VELOCITY_VERLET:
       k_vv_a<<<g, b, 0, c->stream>>>(A, c->pos, c->vel, c->inv_mass.p, c->ffix.p);
@@ -827,25 +994,24 @@ int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy) {
   return 0;
 }
 
-extern "C" int mdx_integrate(mdx_ctx* c, unsigned terms, int nsteps, int use_graph) {
+// ---- integrate = prologue (prime forces, energy-history base) + steps + epilogue
+static int integ_begin(mdx_ctx* c, unsigned terms, int nsteps) {
   MDX_CHECK(c && c->n > 0 && c->vel, MDX_ERR_INVALID, "mdx_integrate: system with velocities required");
   MDX_CHECK(c->int_configured && c->has_mass, MDX_ERR_INVALID, "integrator or masses not configured");
   MDX_CHECK(nsteps >= 0, MDX_ERR_INVALID, "nsteps must be >= 0");
-  ENTER();
   int rc = 0;
   size_t n3 = 3 * (size_t)c->n;
-  if ((rc = c->acc_prev.ensure(n3)) || (rc = c->acc_cur.ensure(n3))) LEAVE_RET(rc);
+  if ((rc = c->acc_prev.ensure(n3)) || (rc = c->acc_cur.ensure(n3))) return rc;
   if (c->int_kind >= MDX_INT_VERLET_REF &&
       ((rc = c->xa.ensure(n3)) || (rc = c->xb.ensure(n3)) || (rc = c->va.ensure(n3)) || (rc = c->vb.ensure(n3))))
-    LEAVE_RET(rc);
+    return rc;
   const int ne = MDX_NTERMS + 1;
-  const int every = c->energy_every;
-  const int nrows = nsteps / every + 1;
+  const int nrows = nsteps / c->energy_every + 1;
   if (nrows > c->ehist_cap) {
-    if ((rc = c->ehist.ensure((size_t)nrows * ne))) LEAVE_RET(rc);
+    if ((rc = c->ehist.ensure((size_t)nrows * ne))) return rc;
     c->ehist_cap = nrows;
+    mdx_invalidate_graph(c);  // captured steps hold the old history pointer
   }
-  IntArgs A = int_args(c);
   dim3 g(div_up(c->n, 256)), b(256);
   // energy-history rows are (device step counter - hist_base); set the base for this call
   long long s0 = 0;
@@ -855,8 +1021,8 @@ extern "C" int mdx_integrate(mdx_ctx* c, unsigned terms, int nsteps, int use_gra
   MDX_HIP(hipStreamSynchronize(c->stream));
   c->ehist_count = 0;
   // prime: forces at the current positions (every mdpy integrate call starts with ensemble.update)
-  if ((rc = mdx_enqueue_zero_accumulators(c)) || (rc = mdx_enqueue_forces(c, terms, true))) LEAVE_RET(rc);
-  A.fpart = live_fpart(c);
+  if ((rc = mdx_enqueue_zero_accumulators(c)) || (rc = mdx_enqueue_forces(c, terms, true))) return rc;
+  IntArgs A = int_args(c);
   k_kinetic<<<g, b, 0, c->stream>>>(c->n, c->vel, c->mass.p, c->efix.p + MDX_E_KINETIC);
   if (c->int_kind == MDX_INT_VERLET_REF) {
     k_vref_prime<<<g, b, 0, c->stream>>>(A, c->pos, c->vel, c->inv_mass.p, c->ffix.p, c->xa.p, c->xb.p,
@@ -867,44 +1033,16 @@ extern "C" int mdx_integrate(mdx_ctx* c, unsigned terms, int nsteps, int use_gra
   }
   MDX_LAUNCH_CHECK();
   c->int_primed = true;
-  // potential energies are evaluated on steps that are multiples of `every` (the prime counts
-  // as step 0); the graph holds a whole number of energy intervals
-  int local = 0;
-  if (use_graph && nsteps > 0) {
-    const int gsteps = std::min(nsteps, every >= 10 ? every : every * ((10 + every - 1) / every));
-    if (!c->graph_exec || c->graph_terms != terms || c->graph_steps != gsteps || c->graph_kind != c->int_kind ||
-        c->graph_every != every) {
-      invalidate_graph(c);
-      hipGraph_t graph;
-      MDX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      for (int k = 0; k < gsteps && !rc; ++k) rc = mdx_integrator_enqueue_step(c, terms, (k + 1) % every == 0);
-      hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
-      if (rc) LEAVE_RET(rc);
-      if (e2 != hipSuccess) { mdx_set_error("graph capture failed: %s", hipGetErrorString(e2)); LEAVE_RET(MDX_ERR_HIP); }
-      hipError_t e3 = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
-      hipGraphDestroy(graph);
-      if (e3 != hipSuccess) { mdx_set_error("graph instantiate failed: %s", hipGetErrorString(e3)); LEAVE_RET(MDX_ERR_HIP); }
-      c->graph_terms = terms;
-      c->graph_steps = gsteps;
-      c->graph_every = every;
-      c->graph_kind = c->int_kind;
-    }
-    int full = nsteps / gsteps, rem = nsteps % gsteps;
-    for (int k = 0; k < full; ++k) {
-      hipError_t e = hipGraphLaunch(c->graph_exec, c->stream);
-      if (e != hipSuccess) { mdx_set_error("hipGraphLaunch: %s", hipGetErrorString(e)); LEAVE_RET(MDX_ERR_HIP); }
-    }
-    local = full * gsteps;
-    for (int k = 0; k < rem && !rc; ++k, ++local) rc = mdx_integrator_enqueue_step(c, terms, (local + 1) % every == 0);
-  } else {
-    for (int k = 0; k < nsteps && !rc; ++k, ++local) rc = mdx_integrator_enqueue_step(c, terms, (local + 1) % every == 0);
-  }
-  if (!rc) {
-    k_record<<<1, 1, 0, c->stream>>>(c->efix.p, ne, c->ehist.p, c->ehist_cap, c->dstate.p, every);
-    if (hipGetLastError() != hipSuccess) rc = MDX_ERR_HIP;
-  }
-  c->ehist_count = nrows;
-  if (rc) LEAVE_RET(rc);
+  return 0;
+}
+
+static int integ_end(mdx_ctx* c, int nsteps) {
+  const int ne = MDX_NTERMS + 1;
+  dim3 g(div_up(c->n, 256)), b(256);
+  IntArgs A = int_args(c);
+  k_record<<<1, 1, 0, c->stream>>>(c->efix.p, ne, c->ehist.p, c->ehist_cap, c->dstate.p, c->energy_every);
+  MDX_LAUNCH_CHECK();
+  c->ehist_count = nsteps / c->energy_every + 1;
   // finish: state velocities / wrapped positions as the reference leaves them
   if (c->int_kind == MDX_INT_VERLET_REF) {
     k_vref_finish<<<g, b, 0, c->stream>>>(A, c->xa.p, c->xb.p, c->vel);
@@ -916,7 +1054,224 @@ extern "C" int mdx_integrate(mdx_ctx* c, unsigned terms, int nsteps, int use_gra
   MDX_LAUNCH_CHECK();
   c->evals += nsteps + 1;
   // positions were wrapped: the neighbour list must not trust its displacement reference
-  LEAVE_RET(mdx_enqueue_request_rebuild(c));
+  return mdx_enqueue_request_rebuild(c);
+}
+
+// steps per graph: a whole number of energy intervals, at least 10 steps when possible
+static int graph_steps_for(const mdx_ctx* c, int nsteps) {
+  const int every = c->energy_every;
+  return std::min(nsteps, every >= 10 ? every : every * ((10 + every - 1) / every));
+}
+
+// direct (uncaptured) steps local0 .. local0+count-1 of the current call
+static int integ_direct(mdx_ctx* c, unsigned terms, int local0, int count) {
+  int rc = 0;
+  for (int k = 0; k < count && !rc; ++k) rc = mdx_integrator_enqueue_step(c, terms, (local0 + k + 1) % c->energy_every == 0);
+  return rc;
+}
+
+// the context's own graph of `gsteps` steps (captured on its stream; cached)
+static int ensure_graph(mdx_ctx* c, unsigned terms, int gsteps) {
+  const int every = c->energy_every;
+  if (c->graph_exec && c->graph_terms == terms && c->graph_steps == gsteps && c->graph_kind == c->int_kind &&
+      c->graph_every == every)
+    return 0;
+  mdx_invalidate_graph(c);
+  hipGraph_t graph;
+  MDX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = integ_direct(c, terms, 0, gsteps);
+  hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+  if (rc) return rc;
+  if (e2 != hipSuccess) { mdx_set_error("graph capture failed: %s", hipGetErrorString(e2)); return MDX_ERR_HIP; }
+  hipError_t e3 = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e3 != hipSuccess) { mdx_set_error("graph instantiate failed: %s", hipGetErrorString(e3)); return MDX_ERR_HIP; }
+  c->graph_terms = terms;
+  c->graph_steps = gsteps;
+  c->graph_every = every;
+  c->graph_kind = c->int_kind;
+  return 0;
+}
+
+extern "C" int mdx_integrate(mdx_ctx* c, unsigned terms, int nsteps, int use_graph) {
+  MDX_CHECK(c, MDX_ERR_INVALID, "null context");
+  ENTER();
+  int rc = integ_begin(c, terms, nsteps);
+  if (rc) LEAVE_RET(rc);
+  // potential energies are evaluated on steps that are multiples of `every` (the prime counts
+  // as step 0); the graph holds a whole number of energy intervals
+  if (use_graph && nsteps > 0) {
+    const int gsteps = graph_steps_for(c, nsteps);
+    if ((rc = ensure_graph(c, terms, gsteps))) LEAVE_RET(rc);
+    const int full = nsteps / gsteps, rem = nsteps % gsteps;
+    for (int k = 0; k < full; ++k) {
+      hipError_t e = hipGraphLaunch(c->graph_exec, c->stream);
+      if (e != hipSuccess) { mdx_set_error("hipGraphLaunch: %s", hipGetErrorString(e)); LEAVE_RET(MDX_ERR_HIP); }
+    }
+    rc = integ_direct(c, terms, full * gsteps, rem);
+  } else {
+    rc = integ_direct(c, terms, 0, nsteps);
+  }
+  if (rc) LEAVE_RET(rc);
+  LEAVE_RET(integ_end(c, nsteps));
+}
+
+// ------------------------------------------------------------------ batched replicas
+// Independent systems (one context each, same device) advanced together. The contexts keep
+// their own streams, lists, grids and graphs; the batch joins the caller's stream to all of
+// them and back.
+struct mdx_batch {
+  std::vector<mdx_ctx*> ctx;
+  hipStream_t ext = nullptr;     // caller's stream
+  hipStream_t lead = nullptr;    // capture / launch stream of the one-graph mode
+  hipEvent_t ev_in = nullptr, ev_fork = nullptr;
+  std::vector<hipEvent_t> ev_out;
+  hipGraphExec_t exec = nullptr;
+  unsigned exec_terms = 0;
+  int exec_steps = 0;
+  std::vector<unsigned long long> exec_gen;  // graph generations of the contexts it captured
+};
+
+static void batch_drop_graph(mdx_batch* b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  b->exec = nullptr;
+}
+
+extern "C" int mdx_batch_create(mdx_ctx** ctxs, int n, void* hip_stream, mdx_batch** out) {
+  MDX_CHECK(ctxs && n > 0 && out, MDX_ERR_INVALID, "mdx_batch_create: bad arguments");
+  for (int i = 0; i < n; ++i) {
+    MDX_CHECK(ctxs[i], MDX_ERR_INVALID, "mdx_batch_create: context %d is NULL", i);
+    MDX_CHECK(ctxs[i]->device == ctxs[0]->device, MDX_ERR_INVALID, "all contexts of a batch share one device");
+    for (int j = 0; j < i; ++j) MDX_CHECK(ctxs[j] != ctxs[i], MDX_ERR_INVALID, "context %d repeated", i);
+  }
+  MDX_HIP(hipSetDevice(ctxs[0]->device));
+  mdx_batch* b = new mdx_batch();
+  b->ctx.assign(ctxs, ctxs + n);
+  b->ext = (hipStream_t)hip_stream;
+  b->ev_out.resize(n, nullptr);
+  bool ok = hipStreamCreateWithFlags(&b->lead, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; i < n && ok; ++i) ok = hipEventCreateWithFlags(&b->ev_out[i], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    mdx_batch_destroy(b);
+    mdx_set_error("mdx_batch_create: stream/event creation failed");
+    return MDX_ERR_HIP;
+  }
+  *out = b;
+  return 0;
+}
+
+extern "C" int mdx_batch_destroy(mdx_batch* b) {
+  if (!b) return 0;
+  batch_drop_graph(b);
+  for (hipEvent_t e : b->ev_out)
+    if (e) hipEventDestroy(e);
+  if (b->ev_in) hipEventDestroy(b->ev_in);
+  if (b->ev_fork) hipEventDestroy(b->ev_fork);
+  if (b->lead) hipStreamDestroy(b->lead);
+  delete b;
+  return 0;
+}
+
+// stream `s` waits for all work enqueued so far on `from`
+static int stream_after(hipStream_t s, hipStream_t from, hipEvent_t ev) {
+  MDX_HIP(hipEventRecord(ev, from));
+  MDX_HIP(hipStreamWaitEvent(s, ev, 0));
+  return 0;
+}
+
+// one hipGraph holding `gsteps` steps of every replica, one branch per replica
+static int batch_build_graph(mdx_batch* b, unsigned terms, int gsteps) {
+  bool fresh = b->exec && b->exec_terms == terms && b->exec_steps == gsteps && b->exec_gen.size() == b->ctx.size();
+  for (size_t i = 0; fresh && i < b->ctx.size(); ++i) fresh = b->exec_gen[i] == b->ctx[i]->graph_gen;
+  if (fresh) return 0;
+  batch_drop_graph(b);
+  hipGraph_t graph;
+  MDX_HIP(hipStreamBeginCapture(b->lead, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  hipError_t e = hipEventRecord(b->ev_fork, b->lead);
+  for (size_t i = 0; i < b->ctx.size() && !rc && e == hipSuccess; ++i) {
+    mdx_ctx* c = b->ctx[i];
+    e = hipStreamWaitEvent(c->stream, b->ev_fork, 0);
+    if (e == hipSuccess) rc = integ_direct(c, terms, 0, gsteps);
+    if (!rc && e == hipSuccess) e = hipEventRecord(b->ev_out[i], c->stream);
+    if (!rc && e == hipSuccess) e = hipStreamWaitEvent(b->lead, b->ev_out[i], 0);
+  }
+  hipError_t e2 = hipStreamEndCapture(b->lead, &graph);
+  if (rc) return rc;
+  if (e != hipSuccess || e2 != hipSuccess) {
+    mdx_set_error("batch graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    return MDX_ERR_HIP;
+  }
+  hipError_t e3 = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e3 != hipSuccess) { mdx_set_error("batch graph instantiate failed: %s", hipGetErrorString(e3)); return MDX_ERR_HIP; }
+  b->exec_terms = terms;
+  b->exec_steps = gsteps;
+  b->exec_gen.resize(b->ctx.size());
+  for (size_t i = 0; i < b->ctx.size(); ++i) b->exec_gen[i] = b->ctx[i]->graph_gen;
+  return 0;
+}
+
+extern "C" int mdx_batch_integrate(mdx_batch* b, unsigned terms, int nsteps, int mode) {
+  MDX_CHECK(b && !b->ctx.empty(), MDX_ERR_INVALID, "mdx_batch_integrate: empty batch");
+  MDX_CHECK(mode >= MDX_BATCH_ONE_GRAPH && mode <= MDX_BATCH_SEQUENTIAL, MDX_ERR_INVALID, "unknown batch mode %d", mode);
+  MDX_CHECK(nsteps >= 0, MDX_ERR_INVALID, "nsteps must be >= 0");
+  const int every = b->ctx[0]->energy_every;
+  for (mdx_ctx* c : b->ctx)
+    MDX_CHECK(c->energy_every == every, MDX_ERR_INVALID, "all replicas of a batch need the same energy interval");
+  MDX_HIP(hipSetDevice(b->ctx[0]->device));
+  // join: every replica stream (and the lead stream) waits for the caller's stream
+  MDX_HIP(hipEventRecord(b->ev_in, b->ext));
+  MDX_HIP(hipStreamWaitEvent(b->lead, b->ev_in, 0));
+  for (mdx_ctx* c : b->ctx) MDX_HIP(hipStreamWaitEvent(c->stream, b->ev_in, 0));
+  int rc = 0;
+  for (mdx_ctx* c : b->ctx)
+    if ((rc = integ_begin(c, terms, nsteps))) return rc;
+  const int gsteps = graph_steps_for(b->ctx[0], nsteps);
+  const int full = gsteps > 0 ? nsteps / gsteps : 0, rem = gsteps > 0 ? nsteps % gsteps : 0;
+  if (mode == MDX_BATCH_ONE_GRAPH && full > 0) {
+    // the lead stream must also follow the primes enqueued on the replica streams
+    for (size_t i = 0; i < b->ctx.size(); ++i)
+      if ((rc = stream_after(b->lead, b->ctx[i]->stream, b->ev_out[i]))) return rc;
+    if ((rc = batch_build_graph(b, terms, gsteps))) return rc;
+    for (int k = 0; k < full; ++k) {
+      hipError_t e = hipGraphLaunch(b->exec, b->lead);
+      if (e != hipSuccess) { mdx_set_error("hipGraphLaunch: %s", hipGetErrorString(e)); return MDX_ERR_HIP; }
+    }
+    for (size_t i = 0; i < b->ctx.size(); ++i)
+      if ((rc = stream_after(b->ctx[i]->stream, b->lead, b->ev_fork))) return rc;
+  } else if (full > 0) {
+    for (mdx_ctx* c : b->ctx)
+      if ((rc = ensure_graph(c, terms, gsteps))) return rc;
+    if (mode == MDX_BATCH_STREAMS) {  // interleaved launches: the replicas' graphs run side by side
+      for (int k = 0; k < full; ++k)
+        for (mdx_ctx* c : b->ctx) {
+          hipError_t e = hipGraphLaunch(c->graph_exec, c->stream);
+          if (e != hipSuccess) { mdx_set_error("hipGraphLaunch: %s", hipGetErrorString(e)); return MDX_ERR_HIP; }
+        }
+    } else {  // sequential: one replica after another, on one stream order
+      for (size_t i = 0; i < b->ctx.size(); ++i) {
+        mdx_ctx* c = b->ctx[i];
+        if (i > 0 && (rc = stream_after(c->stream, b->ctx[i - 1]->stream, b->ev_out[i - 1]))) return rc;
+        for (int k = 0; k < full; ++k) {
+          hipError_t e = hipGraphLaunch(c->graph_exec, c->stream);
+          if (e != hipSuccess) { mdx_set_error("hipGraphLaunch: %s", hipGetErrorString(e)); return MDX_ERR_HIP; }
+        }
+      }
+    }
+  }
+  for (mdx_ctx* c : b->ctx) {
+    if ((rc = integ_direct(c, terms, full * gsteps, rem))) return rc;
+    if ((rc = integ_end(c, nsteps))) return rc;
+  }
+  // leave: the caller's stream waits for every replica
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
+    MDX_HIP(hipEventRecord(b->ev_out[i], b->ctx[i]->stream));
+    MDX_HIP(hipStreamWaitEvent(b->ext, b->ev_out[i], 0));
+  }
+  return 0;
 }
 
 extern "C" int mdx_energy_history_fetch(mdx_ctx* c, double* h_out, int h_cap, int* h_count) {
@@ -980,6 +1335,30 @@ extern "C" int mdx_pair_counts(mdx_ctx* c, int64_t* h_list_slots, int64_t* h_uni
 // own stream (the stream the kernel runs on). which: 0 nonbonded (LJ + erfc Coulomb, forces
 // only, on the current list), 1 full force evaluation of `terms`, 2 PME reciprocal chain,
 // 3 one integrator step of `terms`.
+extern "C" int mdx_nonbonded_tune(mdx_ctx* c, unsigned terms, int lanes, int reps, int* h_lanes) {
+  MDX_CHECK(c && c->nb_configured, MDX_ERR_INVALID, "mdx_nonbonded_tune: nonbonded not configured");
+  MDX_CHECK(lanes == 0 || lanes == 16 || lanes == 32 || lanes == 64, MDX_ERR_INVALID, "lanes must be 0, 16, 32 or 64");
+  int best = lanes;
+  if (lanes == 0) {
+    double best_ms = 1e300;
+    const int cand[3] = {16, 32, 64};
+    for (int k = 0; k < 3; ++k) {
+      c->nb_lanes = cand[k];
+      double ms = 0;
+      int rc = mdx_time_kernel(c, 0, terms, reps, &ms);
+      if (rc) return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = cand[k];
+      }
+    }
+  }
+  c->nb_lanes = best;
+  mdx_invalidate_graph(c);
+  if (h_lanes) *h_lanes = best;
+  return 0;
+}
+
 extern "C" int mdx_time_kernel(mdx_ctx* c, int which, unsigned terms, int reps, double* h_avg_ms) {
   MDX_CHECK(c && h_avg_ms && reps > 0, MDX_ERR_INVALID, "mdx_time_kernel: bad arguments");
   MDX_ENTER();

@@ -58,68 +58,58 @@ __device__ void block_energy(double e, long long* __restrict__ slot) {
   }
 }
 
-__global__ void k_bonds(int nb, int n, const int* __restrict__ idx, const double* __restrict__ par, Box3 b,
-                        const double* __restrict__ pos, long long* __restrict__ ffix, long long* __restrict__ eslot,
-                        int energy) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  double e = 0;
-  if (t < nb) {
-    int i = idx[2 * t], j = idx[2 * t + 1];
-    double k = par[2 * t], r0 = par[2 * t + 1];
-    double d[3];
-    sub_mi(b, pos, j, i, d);
-    double r = norm3(d);
-    double fv = 2 * k * (r - r0);
-    double f[3] = {fv * d[0] / r, fv * d[1] / r, fv * d[2] / r};
-    add_force(ffix, n, i, 1.0, f);
-    add_force(ffix, n, j, -1.0, f);
-    e = k * (r - r0) * (r - r0);
-  }
-  if (energy) block_energy(e, eslot);
+// ---------------------------------------------------------------- per-term device functions
+__device__ double bond_term(int t, int n, const int* __restrict__ idx, const double* __restrict__ par, const Box3& b,
+                            const double* __restrict__ pos, long long* __restrict__ ffix) {
+  int i = idx[2 * t], j = idx[2 * t + 1];
+  double k = par[2 * t], r0 = par[2 * t + 1];
+  double d[3];
+  sub_mi(b, pos, j, i, d);
+  double r = norm3(d);
+  double fv = 2 * k * (r - r0);
+  double f[3] = {fv * d[0] / r, fv * d[1] / r, fv * d[2] / r};
+  add_force(ffix, n, i, 1.0, f);
+  add_force(ffix, n, j, -1.0, f);
+  return k * (r - r0) * (r - r0);
 }
 
-__global__ void k_angles(int na, int n, const int* __restrict__ idx, const double* __restrict__ par, Box3 b,
-                         const double* __restrict__ pos, long long* __restrict__ ffix, long long* __restrict__ eslot,
-                         int energy) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  double e = 0;
-  if (t < na) {
-    int i1 = idx[3 * t], i2 = idx[3 * t + 1], i3 = idx[3 * t + 2];
-    double k = par[4 * t], th0 = par[4 * t + 1], ku = par[4 * t + 2], u0 = par[4 * t + 3];
-    double r21[3], r23[3];
-    sub_mi(b, pos, i1, i2, r21);
-    sub_mi(b, pos, i3, i2, r23);
-    double l21 = norm3(r21), l23 = norm3(r23);
-    double th = acos(dot3(r21, r23) / (l21 * l23));
-    double fv = -2 * k * (th - th0);
-    double nv[3], v1[3], v3[3], mr23[3] = {-r23[0], -r23[1], -r23[2]};
-    cross3(r21, r23, nv);
-    cross3(r21, nv, v1);
-    unit3(v1);
-    cross3(mr23, nv, v3);
-    unit3(v3);
-    double f1[3], f3[3], f2[3];
-    for (int q = 0; q < 3; ++q) {
-      f1[q] = fv * v1[q] / l21;
-      f3[q] = fv * v3[q] / l23;
-      f2[q] = -(f1[q] + f3[q]);
-    }
-    e = k * (th - th0) * (th - th0);
-    // Urey-Bradley
-    double r13[3];
-    sub_mi(b, pos, i3, i1, r13);
-    double l13 = norm3(r13);
-    double fu = 2 * ku * (l13 - u0);
-    for (int q = 0; q < 3; ++q) {
-      f1[q] += fu * r13[q] / l13;
-      f3[q] -= fu * r13[q] / l13;
-    }
-    e += ku * (l13 - u0) * (l13 - u0);
-    add_force(ffix, n, i1, 1.0, f1);
-    add_force(ffix, n, i2, 1.0, f2);
-    add_force(ffix, n, i3, 1.0, f3);
+__device__ double angle_term(int t, int n, const int* __restrict__ idx, const double* __restrict__ par, const Box3& b,
+                             const double* __restrict__ pos, long long* __restrict__ ffix) {
+  int i1 = idx[3 * t], i2 = idx[3 * t + 1], i3 = idx[3 * t + 2];
+  double k = par[4 * t], th0 = par[4 * t + 1], ku = par[4 * t + 2], u0 = par[4 * t + 3];
+  double r21[3], r23[3];
+  sub_mi(b, pos, i1, i2, r21);
+  sub_mi(b, pos, i3, i2, r23);
+  double l21 = norm3(r21), l23 = norm3(r23);
+  double th = acos(dot3(r21, r23) / (l21 * l23));
+  double fv = -2 * k * (th - th0);
+  double nv[3], v1[3], v3[3], mr23[3] = {-r23[0], -r23[1], -r23[2]};
+  cross3(r21, r23, nv);
+  cross3(r21, nv, v1);
+  unit3(v1);
+  cross3(mr23, nv, v3);
+  unit3(v3);
+  double f1[3], f3[3], f2[3];
+  for (int q = 0; q < 3; ++q) {
+    f1[q] = fv * v1[q] / l21;
+    f3[q] = fv * v3[q] / l23;
+    f2[q] = -(f1[q] + f3[q]);
   }
-  if (energy) block_energy(e, eslot);
+  double e = k * (th - th0) * (th - th0);
+  // Urey-Bradley
+  double r13[3];
+  sub_mi(b, pos, i3, i1, r13);
+  double l13 = norm3(r13);
+  double fu = 2 * ku * (l13 - u0);
+  for (int q = 0; q < 3; ++q) {
+    f1[q] += fu * r13[q] / l13;
+    f3[q] -= fu * r13[q] / l13;
+  }
+  e += ku * (l13 - u0) * (l13 - u0);
+  add_force(ffix, n, i1, 1.0, f1);
+  add_force(ffix, n, i2, 1.0, f2);
+  add_force(ffix, n, i3, 1.0, f3);
+  return e;
 }
 
 // dihedral angle (geometry.py:78-90) and the four-body force distribution shared by
@@ -174,65 +164,124 @@ __device__ void four_body_forces(const Box3& b, const double* __restrict__ pos, 
   add_force(ffix, n, d, 1.0, fd);
 }
 
-__global__ void k_dihedrals(int nd, int n, const int* __restrict__ idx, const double* __restrict__ par, Box3 b,
-                            const double* __restrict__ pos, long long* __restrict__ ffix, long long* __restrict__ eslot,
-                            int energy, int refcompat) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  double e = 0;
-  if (t < nd) {
-    int a = idx[4 * t], bb = idx[4 * t + 1], c = idx[4 * t + 2], d = idx[4 * t + 3];
-    double k = par[3 * t], nn = par[3 * t + 1], delta = par[3 * t + 2];
-    double phi = four_body_angle(b, pos, a, bb, c, d);
-    double fv = refcompat ? -k * (1 - nn * sin(nn * phi - delta))  // charmm_dihedral_constraint.py:81
-                          : k * nn * sin(nn * phi - delta);         // -dE/dphi
-    four_body_forces(b, pos, a, bb, c, d, fv, ffix, n);
-    e = k * (1 + cos(nn * phi - delta));
-  }
-  if (energy) block_energy(e, eslot);
+__device__ double dihedral_term(int t, int n, const int* __restrict__ idx, const double* __restrict__ par,
+                                const Box3& b, const double* __restrict__ pos, long long* __restrict__ ffix,
+                                int refcompat) {
+  int a = idx[4 * t], bb = idx[4 * t + 1], c = idx[4 * t + 2], d = idx[4 * t + 3];
+  double k = par[3 * t], nn = par[3 * t + 1], delta = par[3 * t + 2];
+  double phi = four_body_angle(b, pos, a, bb, c, d);
+  double fv = refcompat ? -k * (1 - nn * sin(nn * phi - delta))  // charmm_dihedral_constraint.py:81
+                        : k * nn * sin(nn * phi - delta);         // -dE/dphi
+  four_body_forces(b, pos, a, bb, c, d, fv, ffix, n);
+  return k * (1 + cos(nn * phi - delta));
 }
 
-__global__ void k_impropers(int ni, int n, const int* __restrict__ idx, const double* __restrict__ par, Box3 b,
-                            const double* __restrict__ pos, long long* __restrict__ ffix, long long* __restrict__ eslot,
-                            int energy) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  double e = 0;
-  if (t < ni) {
-    int a = idx[4 * t], bb = idx[4 * t + 1], c = idx[4 * t + 2], d = idx[4 * t + 3];
-    double k = par[2 * t], psi0 = par[2 * t + 1];
-    double psi = four_body_angle(b, pos, a, bb, c, d);
-    four_body_forces(b, pos, a, bb, c, d, -2 * k * (psi - psi0), ffix, n);
-    e = k * (psi - psi0) * (psi - psi0);
-  }
-  if (energy) block_energy(e, eslot);
+__device__ double improper_term(int t, int n, const int* __restrict__ idx, const double* __restrict__ par,
+                                const Box3& b, const double* __restrict__ pos, long long* __restrict__ ffix) {
+  int a = idx[4 * t], bb = idx[4 * t + 1], c = idx[4 * t + 2], d = idx[4 * t + 3];
+  double k = par[2 * t], psi0 = par[2 * t + 1];
+  double psi = four_body_angle(b, pos, a, bb, c, d);
+  four_body_forces(b, pos, a, bb, c, d, -2 * k * (psi - psi0), ffix, n);
+  return k * (psi - psi0) * (psi - psi0);
 }
 
-int mdx_bonded_enqueue(mdx_ctx* c, unsigned terms) {
+// PME correction of one excluded pair: -ke q_i q_j erf(beta r)/r (float64)
+__device__ double pme_excl_term(int t, int n, const int* __restrict__ pairs, const double* __restrict__ pos,
+                                const double* __restrict__ q, double ke, double beta, const Box3& b,
+                                long long* __restrict__ ffix) {
+  int i = pairs[2 * t], j = pairs[2 * t + 1];
+  double d[3];
+  sub_mi(b, pos, j, i, d);
+  double r = norm3(d);
+  double qq = ke * q[i] * q[j];
+  double er = erf(beta * r);
+  double dEdr = qq * (er / (r * r) - 2.0 * beta / sqrt(M_PI) * exp(-beta * beta * r * r) / r);
+  double g = dEdr / r;  // force on i = +g d, on j = -g d
+  double f[3] = {g * d[0], g * d[1], g * d[2]};
+  add_force(ffix, n, i, 1.0, f);
+  add_force(ffix, n, j, -1.0, f);
+  return -qq * er / r;
+}
+
+// ---------------------------------------------------------------- one launch for all terms
+// Consecutive block ranges handle bonds, angles, dihedrals, impropers and the PME exclusion
+// correction (whose first lane also adds the self and background energies); every block
+// belongs to one term, so its energy goes to that term's slot.
+struct BondedArgs {
+  int n;
   Box3 b;
+  const double* pos;
+  long long* ffix;
+  long long* efix;
+  int cnt[5], blocks[5];
+  const int* idx[5];
+  const double* par[5];
+  int refcompat;
+  const double* q;
+  double ke, beta, e_const;
+};
+
+__global__ __launch_bounds__(BD_BLOCK) void k_bonded(BondedArgs a) {
+  int seg = 0, blk = blockIdx.x;
+  while (seg < 4 && blk >= a.blocks[seg]) blk -= a.blocks[seg++];
+  const int t = blk * BD_BLOCK + threadIdx.x;
+  double e = 0.0;
+  if (t < a.cnt[seg]) {
+    switch (seg) {
+      case 0: e = bond_term(t, a.n, a.idx[0], a.par[0], a.b, a.pos, a.ffix); break;
+      case 1: e = angle_term(t, a.n, a.idx[1], a.par[1], a.b, a.pos, a.ffix); break;
+      case 2: e = dihedral_term(t, a.n, a.idx[2], a.par[2], a.b, a.pos, a.ffix, a.refcompat); break;
+      case 3: e = improper_term(t, a.n, a.idx[3], a.par[3], a.b, a.pos, a.ffix); break;
+      default: e = pme_excl_term(t, a.n, a.idx[4], a.pos, a.q, a.ke, a.beta, a.b, a.ffix); break;
+    }
+  }
+  if (seg == 4 && t == 0) e += a.e_const;
+  block_energy(e, a.efix + (seg < 4 ? 3 + seg : 2));
+}
+
+// bonded terms of `terms` plus, with MDX_TERM_COUL_RECIP, the PME exclusion/self/background part
+int mdx_bonded_enqueue(mdx_ctx* c, unsigned terms) {
+  BondedArgs a;
+  a.n = c->n;
   for (int q = 0; q < 3; ++q) {
-    b.L[q] = c->box[q];
-    b.inv[q] = 1.0 / c->box[q];
+    a.b.L[q] = c->box[q];
+    a.b.inv[q] = 1.0 / c->box[q];
   }
-  int en = 1;
-  if ((terms & MDX_TERM_BOND) && c->nb > 0) {
-    k_bonds<<<div_up(c->nb, BD_BLOCK), BD_BLOCK, 0, c->stream>>>(c->nb, c->n, c->bond_idx.p, c->bond_par.p, b, c->pos,
-                                                                c->ffix.p, c->efix.p + 3, en);
-    MDX_LAUNCH_CHECK();
+  a.pos = c->pos;
+  a.ffix = c->ffix.p;
+  a.efix = c->efix.p;
+  const bool on[4] = {(terms & MDX_TERM_BOND) != 0, (terms & MDX_TERM_ANGLE) != 0, (terms & MDX_TERM_DIHEDRAL) != 0,
+                      (terms & MDX_TERM_IMPROPER) != 0};
+  const int counts[4] = {c->nb, c->na, c->nd, c->ni};
+  const int* idx[4] = {c->bond_idx.p, c->angle_idx.p, c->dih_idx.p, c->imp_idx.p};
+  const double* par[4] = {c->bond_par.p, c->angle_par.p, c->dih_par.p, c->imp_par.p};
+  int total = 0;
+  for (int k = 0; k < 4; ++k) {
+    a.cnt[k] = on[k] ? counts[k] : 0;
+    a.blocks[k] = div_up(a.cnt[k], BD_BLOCK);
+    a.idx[k] = idx[k];
+    a.par[k] = par[k];
+    total += a.blocks[k];
   }
-  if ((terms & MDX_TERM_ANGLE) && c->na > 0) {
-    k_angles<<<div_up(c->na, BD_BLOCK), BD_BLOCK, 0, c->stream>>>(c->na, c->n, c->angle_idx.p, c->angle_par.p, b,
-                                                                 c->pos, c->ffix.p, c->efix.p + 4, en);
-    MDX_LAUNCH_CHECK();
+  a.refcompat = c->dih_refcompat;
+  const bool recip = (terms & MDX_TERM_COUL_RECIP) != 0;
+  a.cnt[4] = recip ? c->n_excl_pairs : 0;
+  a.blocks[4] = recip ? std::max(1, div_up(c->n_excl_pairs, BD_BLOCK)) : 0;
+  a.idx[4] = c->excl_pairs.p;
+  a.par[4] = nullptr;
+  total += a.blocks[4];
+  a.q = c->charge64.p;
+  a.ke = 1.0 / (4.0 * M_PI * 0.5727652348434076);
+  a.beta = c->beta;
+  a.e_const = 0.0;
+  if (recip) {
+    const double V = c->box[0] * c->box[1] * c->box[2];
+    const double e_self = -a.ke * c->beta / sqrt(M_PI) * c->q2sum;
+    const double e_bg = -a.ke * M_PI * c->qsum * c->qsum / (2.0 * c->beta * c->beta * V);
+    a.e_const = e_self + e_bg;
   }
-  if ((terms & MDX_TERM_DIHEDRAL) && c->nd > 0) {
-    k_dihedrals<<<div_up(c->nd, BD_BLOCK), BD_BLOCK, 0, c->stream>>>(c->nd, c->n, c->dih_idx.p, c->dih_par.p, b,
-                                                                    c->pos, c->ffix.p, c->efix.p + 5, en,
-                                                                    c->dih_refcompat);
-    MDX_LAUNCH_CHECK();
-  }
-  if ((terms & MDX_TERM_IMPROPER) && c->ni > 0) {
-    k_impropers<<<div_up(c->ni, BD_BLOCK), BD_BLOCK, 0, c->stream>>>(c->ni, c->n, c->imp_idx.p, c->imp_par.p, b,
-                                                                    c->pos, c->ffix.p, c->efix.p + 6, en);
-    MDX_LAUNCH_CHECK();
-  }
+  if (total == 0) return 0;
+  k_bonded<<<total, BD_BLOCK, 0, c->stream>>>(a);
+  MDX_LAUNCH_CHECK();
   return 0;
 }

@@ -91,11 +91,26 @@ struct DevState {
   long long hist_base;      // step counter value at the start of the current integrate call
   int pme_slots;            // reciprocal force slots written in the current evaluation
   unsigned pme_ticket;      // finished blocks of the PME gather kernel
+  unsigned step_ticket;     // finished blocks of the fused step prologue
 };
+
+// float32 sorted-order copy of one atom: wrapped, centred position and q*sqrt(ke) (the layout
+// the pair kernels read), plus its float64 position
+__device__ __forceinline__ void pack_atom(double x, double y, double z, float q, double Lx, double Ly, double Lz,
+                                          float4* __restrict__ xq_s, double* __restrict__ ps_s) {
+  ps_s[0] = x;
+  ps_s[1] = y;
+  ps_s[2] = z;
+  x -= Lx * floor(x / Lx);
+  y -= Ly * floor(y / Ly);
+  z -= Lz * floor(z / Lz);
+  *xq_s = make_float4((float)(x - 0.5 * Lx), (float)(y - 0.5 * Ly), (float)(z - 0.5 * Lz), q);
+}
 
 struct mdx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  bool owns_stream = true;       // false for internal contexts that run on another context's stream
   // ---- system (borrowed)
   int n = 0;
   double* pos = nullptr;
@@ -145,8 +160,10 @@ struct mdx_ctx {
   DevBuf<int> nbr_count;         // sorted
   DevBuf<unsigned> nbr_list;     // column major: [k * stride + s]
   int nbr_cap = 0;               // capacity per atom
+  int nb_lanes = 32;             // lanes per atom in the nonbonded kernel (16, 32 or 64)
   int list_stride = 0;
   bool list_valid = false;
+  bool prologue_done = false;    // the integrator's fused kernel already checked/zeroed/packed this step
   std::vector<int> stencil;      // packed stencil offsets (dx,dy,dz) of the list grid
   DevBuf<int> d_stencil;
   // ---- PME
@@ -186,8 +203,11 @@ struct mdx_ctx {
   int graph_kind = -1;
   int graph_every = 0;
   int energy_every = 1;          // potential energies every this many steps during integrate
+  unsigned long long graph_gen = 0;  // bumped whenever captured graphs of this context go stale
   // scratch
   DevBuf<long long> tmp64;
+  // learned surrogate (optional)
+  mdx_mlp* mlp = nullptr;
 };
 
 // ------------------------------------------------------------------ shared helpers
@@ -231,6 +251,8 @@ int mdx_leave(mdx_ctx* c);
     int _r = mdx_leave(c);            \
     return _v ? _v : _r;              \
   } while (0)
+// drop the context's captured step graph (and mark graphs of batches holding it stale)
+void mdx_invalidate_graph(mdx_ctx* c);
 // celllist / neighbour list
 int mdx_nlist_prepare(mdx_ctx* c);                    // allocate grid/list buffers for current config
 int mdx_nlist_update(mdx_ctx* c, bool force_rebuild);  // enqueue displacement check + conditional rebuild + pack
@@ -241,11 +263,12 @@ int mdx_enqueue_request_rebuild(mdx_ctx* c);
 int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy);
 int mdx_enqueue_coulomb_plain(mdx_ctx* c);
 int mdx_pme_enqueue(mdx_ctx* c, bool energy);
-int mdx_pme_enqueue_corrections(mdx_ctx* c);
 int mdx_bonded_enqueue(mdx_ctx* c, unsigned terms);
 int mdx_enqueue_zero_accumulators(mdx_ctx* c);
 int mdx_enqueue_forces_out(mdx_ctx* c, double* d_out);
 int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy);
+#define MDX_E_MLP 8  // energy slot of MDX_TERM_MLP (its bit index)
+int mdx_mlp_enqueue(mdx_ctx* c, bool energy);
 // integrator
 int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy);
 int mdx_integrator_prime(mdx_ctx* c, unsigned terms);

@@ -333,7 +333,8 @@ __global__ void k_grid_sort_fill(const int* __restrict__ cell_start, const int* 
                                  const double* __restrict__ lj64, float2* __restrict__ ljs,
                                  float2* __restrict__ ljs14, const int* __restrict__ ref_lin,
                                  int* __restrict__ ref_cell_sorted, int rny, int rnz, const DevState* st,
-                                 double thr2) {
+                                 double thr2, const float* __restrict__ qs, float4* __restrict__ xq,
+                                 double* __restrict__ pos_sorted, double Lx, double Ly, double Lz) {
   if (!need_rebuild(st, thr2)) return;
   __shared__ int buf[SORT_MAX];
   int cell = blockIdx.x;
@@ -365,6 +366,7 @@ __global__ void k_grid_sort_fill(const int* __restrict__ cell_start, const int* 
     sorted_orig[s] = o;
     orig_sorted[o] = s;
     for (int d = 0; d < 3; ++d) pos_build[3 * o + d] = pos[3 * o + d];
+    pack_atom(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], qs ? qs[o] : 0.f, Lx, Ly, Lz, &xq[s], &pos_sorted[3 * s]);
     if (lj64) {
       double e = lj64[4 * o + 0], sg = lj64[4 * o + 1], e14 = lj64[4 * o + 2], s14 = lj64[4 * o + 3];
       ljs[s] = make_float2((float)(0.5 * sg), (float)sqrt(e));
@@ -386,14 +388,7 @@ __global__ void k_pack(const double* __restrict__ pos, const int* __restrict__ s
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   int o = sorted_orig[s];
-  double x = pos[3 * o + 0], y = pos[3 * o + 1], z = pos[3 * o + 2];
-  pos_sorted[3 * s + 0] = x;
-  pos_sorted[3 * s + 1] = y;
-  pos_sorted[3 * s + 2] = z;
-  x -= Lx * floor(x / Lx);
-  y -= Ly * floor(y / Ly);
-  z -= Lz * floor(z / Lz);
-  xq[s] = make_float4((float)(x - 0.5 * Lx), (float)(y - 0.5 * Ly), (float)(z - 0.5 * Lz), qs ? qs[o] : 0.f);
+  pack_atom(pos[3 * o + 0], pos[3 * o + 1], pos[3 * o + 2], qs ? qs[o] : 0.f, Lx, Ly, Lz, &xq[s], &pos_sorted[3 * s]);
 }
 
 struct ListArgs {
@@ -517,8 +512,26 @@ __global__ __launch_bounds__(NL_BLOCK) void k_build_list(
 // the home cell) are staged in LDS in chunks, then each NB_G-lane group scans them for one home
 // atom at a time and compacts the accepted ones with a wave ballot. Same candidate order, hence
 // the same lists, as a per-atom walk of the stencil; global memory is read once per cell.
-#define BL_CH 1536
+// The scan loop reads LDS only (the home atom's exclusions and 1-4 partners are staged there
+// too) and evaluates its tests without short-circuit branches, so an iteration waits for LDS
+// once and never for global memory; the float64 decision of skin-0 lists is a separate
+// instantiation.
+#define BL_CH 1024
 #define BL_MAXST 512
+#define BL_MAXEX 32  // exclusions (and 1-4 partners) per atom
+struct CandTest {
+  float4 xi;
+  float Lx, Ly, Lz, ix, iy, iz, rl2;
+};
+__device__ __forceinline__ bool cand_in_range(const CandTest& T, float4 xj) {
+  float dx = xj.x - T.xi.x, dy = xj.y - T.xi.y, dz = xj.z - T.xi.z;
+  dx -= T.Lx * rintf(dx * T.ix);
+  dy -= T.Ly * rintf(dy * T.iy);
+  dz -= T.Lz * rintf(dz * T.iz);
+  return dx * dx + dy * dy + dz * dz < T.rl2;
+}
+
+template <bool EXACT64>
 __global__ __launch_bounds__(NL_BLOCK) void k_build_list_cell(
     ListArgs a, const float4* __restrict__ xq, const int* __restrict__ sorted_orig,
     const int* __restrict__ cell_start, const int* __restrict__ cell_count, const int* __restrict__ stencil,
@@ -527,9 +540,11 @@ __global__ __launch_bounds__(NL_BLOCK) void k_build_list_cell(
     int* __restrict__ nbr_count, unsigned* __restrict__ nbr_list, DevState* st, double thr2) {
   if (!need_rebuild(st, thr2)) return;
   __shared__ float4 cxq[BL_CH];
-  __shared__ int ct[BL_CH], co[BL_CH], cr[BL_CH];
+  __shared__ int2 cto[BL_CH];      // (sorted slot, original atom id)
+  __shared__ int cr[BL_CH];
   __shared__ int sb[BL_MAXST], sp[BL_MAXST + 1];
   __shared__ int acnt[SORT_MAX];
+  __shared__ int exl[NL_BLOCK / NB_G][BL_MAXEX], s4l[NL_BLOCK / NB_G][BL_MAXEX];
   const GridArgs& gr = a.g;
   const int cell = blockIdx.x;
   const int hb = cell_start[cell], hn = cell_count[cell];
@@ -559,8 +574,16 @@ __global__ __launch_bounds__(NL_BLOCK) void k_build_list_cell(
   const int ngroups = blockDim.x / NB_G;
   const unsigned long long gmask = (NB_G == 64 ? ~0ull : ((1ull << NB_G) - 1ull)) << (lane & ~(NB_G - 1));
   const unsigned long long below = (1ull << lane) - 1ull;
-  const float Lx = (float)gr.Lx, Ly = (float)gr.Ly, Lz = (float)gr.Lz;
-  const float ix = 1.0f / Lx, iy = 1.0f / Ly, iz = 1.0f / Lz;
+  CandTest T;
+  T.Lx = (float)gr.Lx;
+  T.Ly = (float)gr.Ly;
+  T.Lz = (float)gr.Lz;
+  T.ix = 1.0f / T.Lx;
+  T.iy = 1.0f / T.Ly;
+  T.iz = 1.0f / T.Lz;
+  T.rl2 = a.rl2;
+  int* my_ex = exl[q];
+  int* my_s4 = s4l[q];
   for (int base = 0; base < total; base += BL_CH) {
     const int nch = min(BL_CH, total - base);
     for (int c = tid; c < nch; c += blockDim.x) {
@@ -573,78 +596,80 @@ __global__ __launch_bounds__(NL_BLOCK) void k_build_list_cell(
       }
       const int t = sb[lo] + (idx - sp[lo]);
       cxq[c] = xq[t];
-      ct[c] = t;
-      co[c] = sorted_orig[t];
+      cto[c] = make_int2(t, sorted_orig[t]);
       if (a.ref_mode) cr[c] = ref_cell_sorted[t];
     }
     __syncthreads();
-    for (int h = q; h < hn; h += ngroups) {
+    for (int h = q; h < hn; h += ngroups) {  // uniform per group
       const int s = hb + h;
       const int o = sorted_orig[s];
-      const float4 xi = xq[s];
+      T.xi = xq[s];
       const int e0 = excl_ptr ? excl_ptr[o] : 0, e1 = excl_ptr ? excl_ptr[o + 1] : 0;
       const int t0s = s14_ptr ? s14_ptr[o] : 0, t1s = s14_ptr ? s14_ptr[o + 1] : 0;
+      const int nex = min(e1 - e0, BL_MAXEX), ns4 = min(t1s - t0s, BL_MAXEX);
+      if (g < nex) my_ex[g] = excl_idx[e0 + g];
+      if (g < ns4) my_s4[g] = s14_idx[t0s + g];
+      if (g == 0 && (e1 - e0 > BL_MAXEX || t1s - t0s > BL_MAXEX)) atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
       const int rci = a.ref_mode ? ref_cell_sorted[s] : 0;
-      constexpr int NREG = 6;
-      int exr[NREG], s4r[NREG];
-#pragma unroll
-      for (int r = 0; r < NREG; ++r) {
-        exr[r] = e0 + r < e1 ? excl_idx[e0 + r] : -1;
-        s4r[r] = t0s + r < t1s ? s14_idx[t0s + r] : -1;
+      double pi[3] = {0.0, 0.0, 0.0};
+      if (EXACT64) {
+        pi[0] = pos[3 * o];
+        pi[1] = pos[3 * o + 1];
+        pi[2] = pos[3 * o + 2];
       }
+      // the group's lanes are in one wave: LDS writes above are visible after this wave-level fence
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       unsigned* row = nbr_list + (long long)s * a.cap;
       int cnt = acnt[h];
-      for (int cb = 0; cb < nch; cb += NB_G) {
-        const int c = cb + g;
-        bool acc = false, s14 = false;
-        int t = 0;
-        if (c < nch) {
-          t = ct[c];
-          if (t != s) {
-            const float4 xj = cxq[c];
-            const int jo = co[c];
-            float ddx = xj.x - xi.x, ddy = xj.y - xi.y, ddz = xj.z - xi.z;
-            ddx -= Lx * rintf(ddx * ix);
-            ddy -= Ly * rintf(ddy * iy);
-            ddz -= Lz * rintf(ddz * iz);
-            const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            if (r2 < a.rl2) {
-              acc = true;
-              if (a.exact64) {  // float64 decision with the reference's sequence (pbc.py:40-44)
-                double d[3];
-                const double L[3] = {gr.Lx, gr.Ly, gr.Lz};
-                for (int r = 0; r < 3; ++r) {
-                  double v = pos[3 * jo + r] - pos[3 * o + r];
-                  double sv = v * (1.0 / L[r]);
-                  sv -= rint(sv);
-                  d[r] = sv * L[r];
-                }
-                acc = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) <= a.rc;
-              }
+      for (int cb = 0; cb < nch; cb += 2 * NB_G) {
+        const int cc[2] = {cb + g, cb + NB_G + g};
+        float4 xj[2];
+        int2 tj[2];
+        int rcj[2] = {0, 0};
 #pragma unroll
-              for (int r = 0; r < NREG; ++r) acc = acc && (exr[r] != jo);
-              for (int r = e0 + NREG; r < e1; ++r) acc = acc && (excl_idx[r] != jo);
-              if (acc && a.ref_mode) {
-                const int rcj = cr[c];
-                const int ax = cyc_dist(rci >> 20, rcj >> 20, a.rnx);
-                const int ay = cyc_dist((rci >> 10) & 1023, (rcj >> 10) & 1023, a.rny);
-                const int az = cyc_dist(rci & 1023, rcj & 1023, a.rnz);
-                acc = ax <= 1 && ay <= 1 && az <= 1;
-              }
-              if (acc) {
-#pragma unroll
-                for (int r = 0; r < NREG; ++r) s14 |= (s4r[r] == jo);
-                for (int r = t0s + NREG; r < t1s; ++r) s14 |= (s14_idx[r] == jo);
-              }
-            }
-          }
+        for (int u = 0; u < 2; ++u) {
+          const int ci = min(cc[u], nch - 1);
+          xj[u] = cxq[ci];
+          tj[u] = cto[ci];
+          if (a.ref_mode) rcj[u] = cr[ci];
         }
-        const unsigned long long m = __ballot(acc) & gmask;
-        const int slot = cnt + __popcll(m & below);
-        if (acc && slot < a.cap) row[slot] = (unsigned)t | (s14 ? 0x80000000u : 0u);
-        cnt += __popcll(m);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int jo = tj[u].y;
+          bool acc = (cc[u] < nch) & (tj[u].x != s) & cand_in_range(T, xj[u]);
+          bool s14 = false;
+          for (int r = 0; r < nex; ++r) acc &= (my_ex[r] != jo);
+          for (int r = 0; r < ns4; ++r) s14 |= (my_s4[r] == jo);
+          if (a.ref_mode) {
+            const int ax = cyc_dist(rci >> 20, rcj[u] >> 20, a.rnx);
+            const int ay = cyc_dist((rci >> 10) & 1023, (rcj[u] >> 10) & 1023, a.rny);
+            const int az = cyc_dist(rci & 1023, rcj[u] & 1023, a.rnz);
+            acc &= (ax <= 1) & (ay <= 1) & (az <= 1);
+          }
+          if (EXACT64 && acc) {  // float64 decision with the reference's sequence (pbc.py:40-44)
+            double d[3];
+            const double L[3] = {gr.Lx, gr.Ly, gr.Lz};
+            for (int r = 0; r < 3; ++r) {
+              double v = pos[3 * jo + r] - pi[r];
+              double sv = v * (1.0 / L[r]);
+              sv -= rint(sv);
+              d[r] = sv * L[r];
+            }
+            acc = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) <= a.rc;
+          }
+          const unsigned long long m = __ballot(acc) & gmask;
+          const int slot = cnt + __popcll(m & below);
+          if (acc && slot < a.cap) row[slot] = (unsigned)tj[u].x | (s14 ? 0x80000000u : 0u);
+          cnt += __popcll(m);
+        }
       }
       if (g == 0) acnt[h] = cnt;
+      // the next home atom of this group overwrites my_ex / my_s4
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     __syncthreads();
   }
@@ -776,10 +801,13 @@ int mdx_nlist_update(mdx_ctx* c, bool force_rebuild) {
   make_grid(c, &g);
   int nb = div_up(c->n, NL_BLOCK);
   DevState* st = c->dstate.p;
-  k_check_disp<<<nb, NL_BLOCK, 0, c->stream>>>(c->pos, c->pos_at_build.p, c->n, c->box[0], c->box[1], c->box[2], st);
-  MDX_LAUNCH_CHECK();
-  k_prep_rebuild<<<div_up(c->ncell, NL_BLOCK), NL_BLOCK, 0, c->stream>>>(st, thr2, c->cell_count.p, c->ncell);
-  MDX_LAUNCH_CHECK();
+  const bool pro = c->prologue_done;  // displacement, rebuild decision and packing already done
+  if (!pro) {
+    k_check_disp<<<nb, NL_BLOCK, 0, c->stream>>>(c->pos, c->pos_at_build.p, c->n, c->box[0], c->box[1], c->box[2], st);
+    MDX_LAUNCH_CHECK();
+    k_prep_rebuild<<<div_up(c->ncell, NL_BLOCK), NL_BLOCK, 0, c->stream>>>(st, thr2, c->cell_count.p, c->ncell);
+    MDX_LAUNCH_CHECK();
+  }
   // reference cells are needed for the REFERENCE stencil filter; they depend on the minimum
   // position, so REFERENCE mode runs with skin 0 (rebuild every evaluation) and re-bins here.
   if (c->pair_mode == MDX_PAIRS_REFERENCE) {
@@ -797,9 +825,10 @@ int mdx_nlist_update(mdx_ctx* c, bool force_rebuild) {
   k_grid_sort_fill<<<c->ncell, 256, 0, c->stream>>>(
       c->cell_start.p, c->cell_count.p, c->sorted_orig.p, c->orig_sorted.p, c->pos, c->pos_at_build.p,
       c->has_lj ? c->lj64.p : nullptr, c->ljs.p, c->ljs14.p, ref ? c->ref_cell_lin.p : nullptr,
-      c->ref_cell_sorted.p, c->ref_n[1], c->ref_n[2], st, thr2);
+      c->ref_cell_sorted.p, c->ref_n[1], c->ref_n[2], st, thr2, c->has_charges ? c->charge_s.p : nullptr, c->xq.p,
+      c->pos_sorted.p, c->box[0], c->box[1], c->box[2]);
   MDX_LAUNCH_CHECK();
-  int rc = mdx_nlist_enqueue_pack(c);
+  int rc = pro ? 0 : mdx_nlist_enqueue_pack(c);
   if (rc) return rc;
   ListArgs a;
   a.g = g;
@@ -815,7 +844,8 @@ int mdx_nlist_update(mdx_ctx* c, bool force_rebuild) {
   a.rnz = c->ref_n[2];
   a.nstencil = (int)c->stencil.size();
   MDX_CHECK(a.nstencil <= BL_MAXST, MDX_ERR_INVALID, "stencil of %d cells exceeds %d", a.nstencil, BL_MAXST);
-  k_build_list_cell<<<c->ncell, NL_BLOCK, 0, c->stream>>>(
+  auto build = a.exact64 ? k_build_list_cell<true> : k_build_list_cell<false>;
+  build<<<c->ncell, NL_BLOCK, 0, c->stream>>>(
       a, c->xq.p, c->sorted_orig.p, c->cell_start.p, c->cell_count.p, c->d_stencil.p,
       c->has_excl ? c->excl_ptr.p : nullptr, c->excl_idx.p, c->has_s14 ? c->s14_ptr.p : nullptr, c->s14_idx.p,
       c->ref_cell_sorted.p, c->pos, c->nbr_count.p, c->nbr_list.p, st, thr2);

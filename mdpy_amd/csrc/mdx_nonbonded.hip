@@ -8,7 +8,8 @@
 // The Coulomb part is the PME direct space q_i q_j erfc(beta r)/r over the same pair set.
 // Plain Coulomb follows mdpy/constraint/electrostatic_constraint.py:52-79.
 //
-// Layout: a group of NB_G lanes per atom in cell-sorted order walks the atom's contiguous
+// Layout: a group of G lanes per atom (16, 32 or 64, chosen by mdx_nonbonded_tune) in
+// cell-sorted order walks the atom's contiguous
 // neighbour list (coalesced index loads), gathers each neighbour's float4 (x, y, z, q*sqrt(ke))
 // and float2 (sigma/2, sqrt(eps)), and keeps partial forces in registers. Every pair is
 // evaluated from both ends (full list), so there are no atomics in the pair loop; the group
@@ -59,10 +60,10 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// NB_G lanes share one atom: lane g walks list entries g, g + NB_G, ... of the atom's
+// G lanes share one atom: lane g walks list entries g, g + G, ... of the atom's
 // contiguous (row-major) list, so a group's index loads and its neighbours' position loads
 // are contiguous; the group reduces its partial forces with xor shuffles at the end.
-template <bool LJ, bool COUL, bool ENERGY>
+template <bool LJ, bool COUL, bool ENERGY, int G>
 __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, const float4* __restrict__ xq,
                                                       const float2* __restrict__ ljs,
                                                       const float2* __restrict__ ljs14,
@@ -72,8 +73,8 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, const float4* 
                                                       const double* __restrict__ pos, long long* __restrict__ ffix,
                                                       long long* __restrict__ efix, DevState* st) {
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = gt / NB_G;
-  const int g = gt & (NB_G - 1);
+  const int s = gt / G;
+  const int g = gt & (G - 1);
   if (gt == 0) {  // every rebuild kernel has finished reading these
     st->rebuild = 0;
     st->max_disp2_bits = 0;
@@ -92,11 +93,11 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, const float4* 
     int oi = -1;
     // NB_U list entries per lane per pass: all index loads, then all neighbour loads, are
     // issued before any of them is consumed, so each pass waits for memory once, not 3*NB_U times
-    for (int kb = g; kb < cnt; kb += NB_G * NB_U) {
+    for (int kb = g; kb < cnt; kb += G * NB_U) {
       unsigned e[NB_U];
 #pragma unroll
       for (int u = 0; u < NB_U; ++u) {
-        const int k = kb + u * NB_G;
+        const int k = kb + u * G;
         e[u] = k < cnt ? lp[k] : (unsigned)s | 0x40000000u;  // bit 30: padding slot
       }
       float4 xj[NB_U];
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, const float4* 
       }
     }
   }
-  for (int off = NB_G / 2; off > 0; off >>= 1) {
+  for (int off = G / 2; off > 0; off >>= 1) {
     fx += __shfl_xor(fx, off, 64);
     fy += __shfl_xor(fy, off, 64);
     fz += __shfl_xor(fz, off, 64);
@@ -205,11 +206,19 @@ int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy) {
   a.Lyd = c->box[1];
   a.Lzd = c->box[2];
   a.rcd = c->rc;
-  dim3 grid(div_up((long long)c->n * NB_G, NB_BLOCK));
+  const int lanes = c->nb_lanes;
+  dim3 grid(div_up((long long)c->n * lanes, NB_BLOCK));
 #define NB_LAUNCH(L_, C_, E_)                                                                          \
-  k_nonbonded<L_, C_, E_><<<grid, NB_BLOCK, 0, c->stream>>>(a, c->xq.p, c->ljs.p, c->ljs14.p, c->nbr_count.p, \
-                                                           c->nbr_list.p, c->sorted_orig.p, c->pos, c->ffix.p, \
-                                                           c->efix.p, c->dstate.p)
+  do {                                                                                                 \
+    if (lanes == 16)                                                                                   \
+      k_nonbonded<L_, C_, E_, 16><<<grid, NB_BLOCK, 0, c->stream>>>(NB_ARGS);                          \
+    else if (lanes == 64)                                                                              \
+      k_nonbonded<L_, C_, E_, 64><<<grid, NB_BLOCK, 0, c->stream>>>(NB_ARGS);                          \
+    else                                                                                               \
+      k_nonbonded<L_, C_, E_, 32><<<grid, NB_BLOCK, 0, c->stream>>>(NB_ARGS);                          \
+  } while (0)
+#define NB_ARGS a, c->xq.p, c->ljs.p, c->ljs14.p, c->nbr_count.p, c->nbr_list.p, c->sorted_orig.p, c->pos, \
+                c->ffix.p, c->efix.p, c->dstate.p
   if (lj && coul) {
     if (energy) NB_LAUNCH(true, true, true); else NB_LAUNCH(true, true, false);
   } else if (lj) {
@@ -218,6 +227,7 @@ int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy) {
     if (energy) NB_LAUNCH(false, true, true); else NB_LAUNCH(false, true, false);
   }
 #undef NB_LAUNCH
+#undef NB_ARGS
   MDX_LAUNCH_CHECK();
   return 0;
 }

@@ -8,7 +8,8 @@
 //            the x-axis kernel also applies the influence function B*C, sums the reciprocal
 //            energy and runs the inverse x transform on the same LDS tile
 //   gather   one lane per atom, 64 grid values, force = -q grad(theta) . conv
-//   corrections  float64: excluded-pair erf correction, self energy, neutralising background
+//   corrections  float64 excluded-pair erf correction, self energy and neutralising background,
+//                launched with the bonded terms (mdx_bonded.hip)
 #include "mdx_internal.h"
 #include <cmath>
 
@@ -321,46 +322,6 @@ __global__ __launch_bounds__(PME_BLOCK) void k_gather(PmeArgs a, const double* _
   atomicAdd((unsigned long long*)&ffix[i], (unsigned long long)to_fix(fx));
   atomicAdd((unsigned long long*)&ffix[a.n + i], (unsigned long long)to_fix(fy));
   atomicAdd((unsigned long long*)&ffix[2 * a.n + i], (unsigned long long)to_fix(fz));
-}
-
-// excluded pairs: -ke q_i q_j erf(beta r)/r (float64); block 0 also adds self + background
-__global__ void k_pme_excl(int npairs, int n, const int* __restrict__ pairs, const double* __restrict__ pos,
-                           const double* __restrict__ q, double ke, double beta, double Lx, double Ly, double Lz,
-                           double e_const, long long* __restrict__ ffix, long long* __restrict__ eslot) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  double e = 0.0;
-  if (t < npairs) {
-    int i = pairs[2 * t], j = pairs[2 * t + 1];
-    double L[3] = {Lx, Ly, Lz}, d[3];
-    for (int k = 0; k < 3; ++k) {
-      double v = pos[3 * j + k] - pos[3 * i + k];
-      double s = v / L[k];
-      s -= rint(s);
-      d[k] = s * L[k];
-    }
-    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    double qq = ke * q[i] * q[j];
-    double er = erf(beta * r);
-    e = -qq * er / r;
-    double dEdr = qq * (er / (r * r) - 2.0 * beta / sqrt(M_PI) * exp(-beta * beta * r * r) / r);
-    double g = dEdr / r;  // force on i = +g d, on j = -g d
-    atomicAdd((unsigned long long*)&ffix[i], (unsigned long long)to_fix(g * d[0]));
-    atomicAdd((unsigned long long*)&ffix[n + i], (unsigned long long)to_fix(g * d[1]));
-    atomicAdd((unsigned long long*)&ffix[2 * n + i], (unsigned long long)to_fix(g * d[2]));
-    atomicAdd((unsigned long long*)&ffix[j], (unsigned long long)to_fix(-g * d[0]));
-    atomicAdd((unsigned long long*)&ffix[n + j], (unsigned long long)to_fix(-g * d[1]));
-    atomicAdd((unsigned long long*)&ffix[2 * n + j], (unsigned long long)to_fix(-g * d[2]));
-  }
-  if (t == 0) e += e_const;
-  __shared__ double sm[PME_BLOCK / 64];
-  for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
-  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = e;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0;
-    for (int w = 0; w < PME_BLOCK / 64; ++w) s += sm[w];
-    atomicAdd((unsigned long long*)eslot, (unsigned long long)to_fix(s));
-  }
 }
 
 // ---------------------------------------------------------------- plane kernels
@@ -745,9 +706,11 @@ extern "C" int mdx_pme_config(mdx_ctx* c, const int* h_grid, int order, double b
   }
   c->order = order;
   c->beta = beta;
+  mdx_invalidate_graph(c);
   size_t G = (size_t)c->K[0] * c->K[1] * c->K[2];
   int rc;
-  if ((rc = c->qfix.ensure(G)) || (rc = c->cgrid.ensure(G)) || (rc = c->conv.ensure(G)) || (rc = c->bc.ensure(G)))
+  if ((rc = c->qfix.ensure(G)) || (rc = c->cgrid.ensure(G)) || (rc = c->conv.ensure(G)) || (rc = c->bc.ensure(G)) ||
+      (rc = c->fpart.ensure(12 * (size_t)c->n)))
     return rc;
   MDX_HIP(hipMemset(c->qfix.p, 0, G * sizeof(long long)));
   // influence function B(m) C(m), C(0) = 0
@@ -890,11 +853,11 @@ int mdx_pme_enqueue(mdx_ctx* c, bool energy) {
   (void)energy;
   MDX_CHECK(c->pme_configured, MDX_ERR_INVALID, "PME not configured");
   if (c->pme_planes && c->nb_configured) {
+    MDX_CHECK(c->fpart.p && c->fpart.n >= 12 * (size_t)c->n, MDX_ERR_INVALID, "PME configured for another system size");
     int rc;
-    if ((rc = c->fpart.ensure(12 * (size_t)c->n))) return rc;
     if ((rc = pme_enqueue_planes(c))) return rc;
     c->fpart_live = true;
-    return mdx_pme_enqueue_corrections(c);
+    return 0;
   }
   c->fpart_live = false;
   PmeArgs a;
@@ -915,18 +878,6 @@ int mdx_pme_enqueue(mdx_ctx* c, bool energy) {
   if ((rc = fft_pass(c, 2, 1, 2))) return rc;   // z inverse, real part out
   k_gather<<<div_up(c->n, PME_BLOCK), PME_BLOCK, 0, c->stream>>>(a, c->pos, c->charge_s.p, c->conv.p, c->ffix.p);
   MDX_LAUNCH_CHECK();
-  return mdx_pme_enqueue_corrections(c);
-}
-
-int mdx_pme_enqueue_corrections(mdx_ctx* c) {
-  const double ke = 1.0 / (4.0 * M_PI * 0.5727652348434076);
-  double V = c->box[0] * c->box[1] * c->box[2];
-  double e_self = -ke * c->beta / sqrt(M_PI) * c->q2sum;
-  double e_bg = -ke * M_PI * c->qsum * c->qsum / (2.0 * c->beta * c->beta * V);
-  int np = std::max(1, c->n_excl_pairs);
-  k_pme_excl<<<div_up(np, PME_BLOCK), PME_BLOCK, 0, c->stream>>>(
-      c->n_excl_pairs, c->n, c->excl_pairs.p, c->pos, c->charge64.p, ke, c->beta, c->box[0], c->box[1], c->box[2],
-      e_self + e_bg, c->ffix.p, c->efix.p + 2);
-  MDX_LAUNCH_CHECK();
   return 0;
 }
+

@@ -176,7 +176,10 @@ class Engine:
 
     # ------------------------------------------------------------------ state
     def set_positions(self, x, wrap=True):
-        self.pos.copy_(self.torch.as_tensor(np.asarray(x, dtype=np.float64), device=self.device))
+        if isinstance(x, self.torch.Tensor):
+            self.pos.copy_(x.to(device=self.device, dtype=self.torch.float64))
+        else:
+            self.pos.copy_(self.torch.as_tensor(np.asarray(x, dtype=np.float64), device=self.device))
         if wrap:
             L.call("mdx_wrap_positions", self.ctx)
 
@@ -240,6 +243,12 @@ class Engine:
         a, b = ctypes.c_int64(), ctypes.c_int64()
         L.call("mdx_pair_counts", self.ctx, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def tune_nonbonded(self, terms, lanes=0, reps=20):
+        """Lanes per atom of the nonbonded kernel (0 = time 16/32/64 on the current list)."""
+        out = ctypes.c_int()
+        L.call("mdx_nonbonded_tune", self.ctx, int(terms), int(lanes), int(reps), ctypes.byref(out))
+        return out.value
 
     def time_kernel(self, which, terms, reps=20):
         """Average ms per launch (HIP events on the library stream): which 0 nonbonded,

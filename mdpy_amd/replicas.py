@@ -1,0 +1,119 @@
+"""Batched replicas: many independent boxes of uneven size advanced together on one GPU.
+
+Each replica is a full system (its own box, neighbour-list grid, PME grid, integrator and
+Philox stream) held by its own libmdx context; a batch (``mdx_batch_*``) advances all of them
+in one of three ways:
+
+``one_graph``   one hipGraph with a branch per replica, replayed on one stream;
+``streams``     every replica replays its own graph on its own stream, launches interleaved;
+``sequential``  replica after replica (the per-replica loop the batch is measured against).
+
+The autotuner picks, per replica, the neighbour-list skin that minimises its step time, and
+then the batch mode with the highest aggregate atom-steps/s.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+
+from . import _lib as L
+from .engine import Engine
+from .unit import KB
+
+TERMS_WATER = L.TERM_LJ | L.TERM_PME | L.TERM_BOND | L.TERM_ANGLE
+
+
+def maxwell_velocities(masses, temperature, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal((len(masses), 3)) * np.sqrt(KB * temperature / np.asarray(masses))[:, None]
+    v -= (np.asarray(masses)[:, None] * v).sum(0) / np.sum(masses)
+    return v
+
+
+class ReplicaBatch:
+    """Independent replicas on one device, advanced by one batch call."""
+
+    def __init__(self, systems, device=0, skins=None, temperature=300.0, friction=1e-3, dt=1.0, seed=0,
+                 terms=TERMS_WATER, energy_every=50):
+        import torch
+        self.torch = torch
+        self.systems = list(systems)
+        self.terms = int(terms)
+        self.energy_every = int(energy_every)
+        self.temperature, self.friction, self.dt, self.seed = temperature, friction, dt, seed
+        skins = list(skins) if skins is not None else [1.5] * len(self.systems)
+        self.engines = []
+        for r, s in enumerate(self.systems):
+            e = Engine(s, device=device, skin=skins[r])
+            e.set_positions(s.positions)
+            e.set_velocities(maxwell_velocities(s.masses, temperature, seed * 7919 + r))
+            e.configure_integrator(L.INT_LANGEVIN, dt=dt, temperature=temperature, friction=friction,
+                                   seed=seed * 7919 + r)
+            L.call("mdx_set_energy_interval", e.ctx, self.energy_every)
+            self.engines.append(e)
+        self.skins = skins
+        self.n_atoms = int(sum(s.num_atoms for s in self.systems))
+        self._batch = None
+        self._make_batch()
+
+    def _make_batch(self):
+        if self._batch is not None:
+            L.call("mdx_batch_destroy", self._batch)
+        arr = (ctypes.c_void_p * len(self.engines))(*[e.ctx.value for e in self.engines])
+        self._batch = ctypes.c_void_p()
+        stream = self.torch.cuda.current_stream().cuda_stream
+        L.call("mdx_batch_create", arr, len(self.engines), ctypes.c_void_p(stream), ctypes.byref(self._batch))
+
+    def integrate(self, nsteps, mode="one_graph"):
+        L.call("mdx_batch_integrate", self._batch, self.terms, int(nsteps), L.BATCH_MODES[mode])
+
+    def time(self, nsteps, mode="one_graph"):
+        """Wall seconds for nsteps of every replica (synchronised on both sides)."""
+        self.torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        self.integrate(nsteps, mode)
+        self.torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    def atom_steps_per_s(self, nsteps, mode="one_graph"):
+        return self.n_atoms * nsteps / self.time(nsteps, mode)
+
+    def status(self):
+        return [e.status() for e in self.engines]
+
+    # ------------------------------------------------------------------ autotuning
+    def autotune_skins(self, candidates=(0.5, 1.0, 1.5, 2.0), steps=60, warmup=20):
+        """Per replica, the list skin with the shortest step time (each replica timed alone)."""
+        best = []
+        for r, e in enumerate(self.engines):
+            times = []
+            for skin in candidates:
+                e.configure_nonbonded(e.cutoff, skin=skin)
+                e.integrate(self.terms, warmup, True, self.energy_every)
+                e.synchronize()
+                t0 = time.perf_counter()
+                e.integrate(self.terms, steps, True, self.energy_every)
+                e.synchronize()
+                times.append((time.perf_counter() - t0) / steps)
+            k = int(np.argmin(times))
+            e.configure_nonbonded(e.cutoff, skin=candidates[k])
+            best.append(candidates[k])
+        self.skins = best
+        return best
+
+    def autotune_mode(self, steps=100, warmup=20):
+        """The batch mode with the highest aggregate atom-steps/s."""
+        rates = {}
+        for mode in ("one_graph", "streams", "sequential"):
+            self.integrate(warmup, mode)
+            rates[mode] = self.atom_steps_per_s(steps, mode)
+        return max(rates, key=rates.get), rates
+
+    def close(self):
+        if self._batch is not None:
+            L.call("mdx_batch_destroy", self._batch)
+            self._batch = None
+        for e in self.engines:
+            e.close()

@@ -127,8 +127,8 @@ def test_nve_energy_conservation_pme_water():
     e.integrate(terms, 400, use_graph=True)
     h = e.energy_history()
     assert h.shape[0] == 401
-    etot = h[:, :8].sum(1) + h[:, 8]
-    ke = h[:, 8].mean()
+    etot = h[:, :L.NTERMS].sum(1) + h[:, L.E_KINETIC]
+    ke = h[:, L.E_KINETIC].mean()
     drift = np.polyfit(np.arange(len(etot)) * 0.5, etot, 1)[0]  # per fs
     say("  E0 %.6e  std %.3e  KE %.3e  drift %.3e /fs" % (etot[0], etot.std(), ke, drift))
     assert etot.std() < 2e-3 * ke
