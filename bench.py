@@ -55,6 +55,50 @@ def cpu_baseline(system, steps=2):
                       "%.1f s wall; steady-state step = 3 evaluations" % (steps, evals, wall)}
 
 
+def nve_leg(e, terms, steps=1000, dt=1.0, every=10):
+    """Energy drift of velocity Verlet from the thermalised state: slope of E_total(t) from a
+    linear fit, per atom, in kJ/mol/ns."""
+    from mdpy_amd import _lib as L
+    from mdpy_amd.unit import KJ_PER_MOL
+    e.configure_integrator(L.INT_VELOCITY_VERLET, dt=dt)
+    e.integrate(terms, steps, True, every)
+    h = e.energy_history()
+    etot = h[:, :L.NTERMS].sum(1) + h[:, L.E_KINETIC]
+    t_ns = np.arange(len(etot)) * every * dt * 1e-6
+    slope = np.polyfit(t_ns, etot, 1)[0] / KJ_PER_MOL / e.n
+    return {"steps": steps, "dt_fs": dt, "drift_kj_mol_ns_atom": float(slope),
+            "abs_drift_kj_mol_ns_atom": float(abs(slope)),
+            "etot_std_kj_mol_atom": float(etot.std() / KJ_PER_MOL / e.n), "device_flags": e.status()}
+
+
+def chain_leg(steps=1000, warmup=100):
+    """ns/day of the solvated chain (1,200-atom chain in 10,000 waters, 68 A, r_c 12 A, PME 72^3)."""
+    import torch
+    from mdpy_amd import _lib as L, systems
+    from mdpy_amd.engine import Engine
+    s = systems.solvated_chain()
+    e = Engine(s, skin=1.5)
+    e.set_positions(s.positions)
+    e.set_velocities(maxwell_velocities(s.masses, 300.0, 0))
+    e.configure_integrator(L.INT_LANGEVIN, dt=1.0, temperature=300.0, friction=1e-3, seed=0)
+    terms = L.TERM_LJ | L.TERM_PME | L.TERM_BOND | L.TERM_ANGLE | L.TERM_DIHEDRAL | L.TERM_IMPROPER
+    e.integrate(terms, warmup, True, 50)
+    lanes = e.tune_nonbonded(L.TERM_LJ | L.TERM_COUL_DIRECT, 0, reps=10)
+    e.integrate(terms, 50, True, 50)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.integrate(terms, steps, True, 50)
+    torch.cuda.synchronize()
+    sps = (time.perf_counter() - t0) / steps
+    slots, pairs = e.pair_counts()
+    nb_ms = e.time_kernel(0, L.TERM_LJ | L.TERM_COUL_DIRECT, reps=20)
+    out = {"atoms": s.num_atoms, "ns_per_day": 0.0864 / sps, "ms_per_step": sps * 1e3, "nb_lanes": lanes,
+           "nonbonded_ms": nb_ms, "unique_pairs": pairs, "pairs_per_s": pairs / (nb_ms * 1e-3),
+           "device_flags": e.status()}
+    e.close()
+    return out
+
+
 def replicas_leg(steps=200):
     """64 uneven water boxes (300-3000 waters, r_c 9 A): aggregate atom-steps/s per batch mode."""
     import torch
@@ -64,7 +108,7 @@ def replicas_leg(steps=200):
     b = ReplicaBatch(reps, energy_every=50)
     skins = b.autotune_skins(candidates=(1.0, 1.5, 2.0), steps=30, warmup=10)
     rates = {}
-    for mode in ("sequential", "streams", "one_graph"):
+    for mode in ("sequential", "streams"):
         b.integrate(20, mode)
         rates[mode] = b.atom_steps_per_s(steps, mode)
     torch.cuda.synchronize()
@@ -134,6 +178,8 @@ def main():
                     help="also measure the 64 batched water replicas (aggregate atom-steps/s per batch mode)")
     ap.add_argument("--mlp", action="store_true",
                     help="also train and measure the MLP surrogate on this box (adds a few minutes)")
+    ap.add_argument("--nve", action="store_true", help="also run 1,000 velocity-Verlet steps and report the drift")
+    ap.add_argument("--chain", action="store_true", help="also measure the solvated 31k-atom chain (72^3 PME)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB (a kernel of known "
                          "traffic for scaling hardware byte counters)")
@@ -208,6 +254,10 @@ def main():
         "energies_finite": finite,
         "device_flags": status,
     }
+    if args.nve:
+        out["nve"] = nve_leg(e, terms)
+    if args.chain:
+        out["chain"] = chain_leg()
     if args.replicas:
         out["replicas"] = replicas_leg()
     if args.mlp:

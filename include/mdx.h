@@ -156,7 +156,8 @@ int mdx_status_fetch(mdx_ctx* ctx, unsigned* h_flags);
  * keeps its own box, lists, PME grid, integrator and graph. Created on the caller's stream
  * `hip_stream` (NULL = default); the contexts are borrowed and must outlive the batch. */
 typedef struct mdx_batch mdx_batch;
-#define MDX_BATCH_ONE_GRAPH   0  /* one hipGraph with a branch per replica                      */
+#define MDX_BATCH_ONE_GRAPH   0  /* one hipGraph with a branch per replica (disabled: returns
+                                    MDX_ERR_INVALID until a host crash is understood)          */
 #define MDX_BATCH_STREAMS     1  /* each replica replays its own graph on its own stream         */
 #define MDX_BATCH_SEQUENTIAL  2  /* replica after replica (the per-replica loop, for comparison) */
 int mdx_batch_create(mdx_ctx** ctxs, int n_ctx, void* hip_stream, mdx_batch** out);

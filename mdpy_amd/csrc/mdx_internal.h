@@ -111,6 +111,8 @@ struct mdx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool owns_stream = true;       // false for internal contexts that run on another context's stream
+  hipStream_t side_stream = nullptr;  // concurrent branch (PME) of a force evaluation
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // ---- system (borrowed)
   int n = 0;
   double* pos = nullptr;

@@ -147,7 +147,7 @@ def test_gjf_langevin_thermalises():
     terms = L.TERM_LJ | L.TERM_PME | L.TERM_BOND | L.TERM_ANGLE
     e.integrate(terms, 3000, use_graph=True)
     h = e.energy_history()
-    T = 2 * h[1000:, 8] / (3 * s.num_atoms * KB)
+    T = 2 * h[1000:, L.E_KINETIC] / (3 * s.num_atoms * KB)
     say("  <T> = %.1f K" % T.mean())
     assert 270 < T.mean() < 330
     e.close()
