@@ -125,6 +125,7 @@ def mlp_leg(s, e, terms, plain_ns_day, n_frames=300, every=10, epochs=4):
     import torch
     from mdpy_amd import _lib as L
     from mdpy_amd.mlp import Surrogate, elements_of, force_rmse_kj
+    e.configure_integrator(L.INT_LANGEVIN, dt=1.0, temperature=300.0, friction=1e-3, seed=1)
     xs, fs = [], []
     for _ in range(n_frames):
         e.integrate(terms, every, True, 50)
@@ -225,6 +226,13 @@ def main():
     eval_ms = e.time_kernel(1, terms, reps=20)
     pme_ms = e.time_kernel(2, terms, reps=20)
     stats = e.stats()
+    # HBM bytes per nonbonded launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
+    # workload, scaled by the same counters on a 256 MiB copy (profiles/)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_nonbonded_water23328_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("traffic_bytes_per_launch")
 
     out = {
         "metric": "ns/day @23,328-atom PME water, plain FP32 vs MLP surrogate at force RMSE<=0.5",
@@ -246,7 +254,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "kernel": "k_nonbonded (LJ + erfc Coulomb)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel_ms": nb_ms, "unique_pairs": pairs, "list_slots": slots,
                      "pairs_per_s": pairs / (nb_ms * 1e-3), "flops_per_pair": FLOPS_PER_PAIR},
         "breakdown_ms": {"force_eval": eval_ms, "pme_reciprocal": pme_ms, "nonbonded": nb_ms},
@@ -254,13 +262,21 @@ def main():
         "energies_finite": finite,
         "device_flags": status,
     }
+    def progress(msg):
+        print("bench: " + msg, file=sys.stderr, flush=True)
+
     if args.nve:
+        progress("NVE drift leg")
         out["nve"] = nve_leg(e, terms)
     if args.chain:
+        progress("solvated chain leg")
         out["chain"] = chain_leg()
     if args.replicas:
-        out["replicas"] = replicas_leg()
+        # the 64-replica leg ended in a host-side segmentation fault whose cause is not yet
+        # identified; it stays off until it is
+        out["replicas"] = {"error": "disabled: the 64-replica run crashed on the host (cause under investigation)"}
     if args.mlp:
+        progress("MLP surrogate leg")
         out["mlp"] = mlp_leg(s, e, terms, ns_day)
     if not args.no_cpu_baseline:
         try:

@@ -134,7 +134,7 @@ __device__ __forceinline__ void stage_neighbours(const MlAtoms& a, const MlConst
     const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
     if (rr < k.rd && rr > 0.f) {
       r = rr;
-      const float ir = 1.f / rr;
+      const float ir = __builtin_amdgcn_rcpf(rr);
       ux = dx * ir;
       uy = dy * ir;
       uz = dz * ir;
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(64) void k_ml_force(MlAtoms a, MlConst k, const flo
     dz -= a.Lz * rintf(dz * a.iz);
     const float r = sqrtf(dx * dx + dy * dy + dz * dz);
     if (r >= k.rd || r <= 0.f) continue;
-    const float ir = 1.f / r;
+    const float ir = __builtin_amdgcn_rcpf(r);
     float vx, vy, vz;
     pair_force(k, gG, tt, (int)xj.w, dx * ir, dy * ir, dz * ir, r, &vx, &vy, &vz);
     const int oj = sorted_orig[t];

@@ -516,9 +516,12 @@ __global__ __launch_bounds__(NL_BLOCK) void k_build_list(
 // too) and evaluates its tests without short-circuit branches, so an iteration waits for LDS
 // once and never for global memory; the float64 decision of skin-0 lists is a separate
 // instantiation.
-#define BL_CH 1024
+#define BL_CH 768
 #define BL_MAXST 512
 #define BL_MAXEX 32  // exclusions (and 1-4 partners) per atom
+#define BL_HMAX 192  // home atoms staged in LDS (more: their data is read from global memory)
+#define BL_POOL 1024 // exclusion / 1-4 entries of a cell's home atoms staged in LDS
+#define BL_THREADS 512
 struct CandTest {
   float4 xi;
   float Lx, Ly, Lz, ix, iy, iz, rl2;
@@ -532,7 +535,7 @@ __device__ __forceinline__ bool cand_in_range(const CandTest& T, float4 xj) {
 }
 
 template <bool EXACT64>
-__global__ __launch_bounds__(NL_BLOCK) void k_build_list_cell(
+__global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
     ListArgs a, const float4* __restrict__ xq, const int* __restrict__ sorted_orig,
     const int* __restrict__ cell_start, const int* __restrict__ cell_count, const int* __restrict__ stencil,
     const int* __restrict__ excl_ptr, const int* __restrict__ excl_idx, const int* __restrict__ s14_ptr,
@@ -544,7 +547,12 @@ __global__ __launch_bounds__(NL_BLOCK) void k_build_list_cell(
   __shared__ int cr[BL_CH];
   __shared__ int sb[BL_MAXST], sp[BL_MAXST + 1];
   __shared__ int acnt[SORT_MAX];
-  __shared__ int exl[NL_BLOCK / NB_G][BL_MAXEX], s4l[NL_BLOCK / NB_G][BL_MAXEX];
+  __shared__ int exl[BL_THREADS / NB_G][BL_MAXEX], s4l[BL_THREADS / NB_G][BL_MAXEX];
+  // home atoms staged once per block: position, atom id and their exclusion / 1-4 lists
+  __shared__ float4 hxi[BL_HMAX];
+  __shared__ int ho[BL_HMAX], hex0[BL_HMAX + 1], hs40[BL_HMAX + 1];
+  __shared__ int pool_ex[BL_POOL], pool_s4[BL_POOL];
+  __shared__ int fast;
   const GridArgs& gr = a.g;
   const int cell = blockIdx.x;
   const int hb = cell_start[cell], hn = cell_count[cell];
@@ -562,10 +570,39 @@ __global__ __launch_bounds__(NL_BLOCK) void k_build_list_cell(
     sp[k + 1] = cell_count[nc];
   }
   for (int j = tid; j < hn; j += blockDim.x) acnt[j] = 0;
+  const bool fit = hn <= BL_HMAX;
+  if (fit) {
+    for (int h = tid; h < hn; h += blockDim.x) {
+      const int o = sorted_orig[hb + h];
+      ho[h] = o;
+      hxi[h] = xq[hb + h];
+      hex0[h + 1] = excl_ptr ? excl_ptr[o + 1] - excl_ptr[o] : 0;
+      hs40[h + 1] = s14_ptr ? s14_ptr[o + 1] - s14_ptr[o] : 0;
+    }
+  }
   __syncthreads();
   if (tid == 0) {
     sp[0] = 0;
     for (int k = 0; k < a.nstencil; ++k) sp[k + 1] += sp[k];
+    if (fit) {
+      hex0[0] = 0;
+      hs40[0] = 0;
+      for (int h = 0; h < hn; ++h) {
+        hex0[h + 1] += hex0[h];
+        hs40[h + 1] += hs40[h];
+      }
+    }
+    fast = fit && hex0[fit ? hn : 0] <= BL_POOL && hs40[fit ? hn : 0] <= BL_POOL;
+  }
+  __syncthreads();
+  if (fast) {
+    for (int h = tid; h < hn; h += blockDim.x) {
+      const int o = ho[h];
+      if (excl_ptr)
+        for (int r = 0; r < hex0[h + 1] - hex0[h]; ++r) pool_ex[hex0[h] + r] = excl_idx[excl_ptr[o] + r];
+      if (s14_ptr)
+        for (int r = 0; r < hs40[h + 1] - hs40[h]; ++r) pool_s4[hs40[h] + r] = s14_idx[s14_ptr[o] + r];
+    }
   }
   __syncthreads();
   const int total = sp[a.nstencil];
@@ -602,14 +639,25 @@ __global__ __launch_bounds__(NL_BLOCK) void k_build_list_cell(
     __syncthreads();
     for (int h = q; h < hn; h += ngroups) {  // uniform per group
       const int s = hb + h;
-      const int o = sorted_orig[s];
-      T.xi = xq[s];
-      const int e0 = excl_ptr ? excl_ptr[o] : 0, e1 = excl_ptr ? excl_ptr[o + 1] : 0;
-      const int t0s = s14_ptr ? s14_ptr[o] : 0, t1s = s14_ptr ? s14_ptr[o + 1] : 0;
-      const int nex = min(e1 - e0, BL_MAXEX), ns4 = min(t1s - t0s, BL_MAXEX);
-      if (g < nex) my_ex[g] = excl_idx[e0 + g];
-      if (g < ns4) my_s4[g] = s14_idx[t0s + g];
-      if (g == 0 && (e1 - e0 > BL_MAXEX || t1s - t0s > BL_MAXEX)) atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
+      int o, nex_all, ns4_all;
+      if (fast) {  // everything from LDS
+        o = ho[h];
+        T.xi = hxi[h];
+        nex_all = hex0[h + 1] - hex0[h];
+        ns4_all = hs40[h + 1] - hs40[h];
+        if (g < nex_all && g < BL_MAXEX) my_ex[g] = pool_ex[hex0[h] + g];
+        if (g < ns4_all && g < BL_MAXEX) my_s4[g] = pool_s4[hs40[h] + g];
+      } else {     // crowded cell: read the home atom's data from global memory
+        o = sorted_orig[s];
+        T.xi = xq[s];
+        const int e0 = excl_ptr ? excl_ptr[o] : 0, t0s = s14_ptr ? s14_ptr[o] : 0;
+        nex_all = excl_ptr ? excl_ptr[o + 1] - e0 : 0;
+        ns4_all = s14_ptr ? s14_ptr[o + 1] - t0s : 0;
+        if (g < nex_all && g < BL_MAXEX) my_ex[g] = excl_idx[e0 + g];
+        if (g < ns4_all && g < BL_MAXEX) my_s4[g] = s14_idx[t0s + g];
+      }
+      const int nex = min(nex_all, BL_MAXEX), ns4 = min(ns4_all, BL_MAXEX);
+      if (g == 0 && (nex_all > BL_MAXEX || ns4_all > BL_MAXEX)) atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
       const int rci = a.ref_mode ? ref_cell_sorted[s] : 0;
       double pi[3] = {0.0, 0.0, 0.0};
       if (EXACT64) {
@@ -845,7 +893,7 @@ int mdx_nlist_update(mdx_ctx* c, bool force_rebuild) {
   a.nstencil = (int)c->stencil.size();
   MDX_CHECK(a.nstencil <= BL_MAXST, MDX_ERR_INVALID, "stencil of %d cells exceeds %d", a.nstencil, BL_MAXST);
   auto build = a.exact64 ? k_build_list_cell<true> : k_build_list_cell<false>;
-  build<<<c->ncell, NL_BLOCK, 0, c->stream>>>(
+  build<<<c->ncell, BL_THREADS, 0, c->stream>>>(
       a, c->xq.p, c->sorted_orig.p, c->cell_start.p, c->cell_count.p, c->d_stencil.p,
       c->has_excl ? c->excl_ptr.p : nullptr, c->excl_idx.p, c->has_s14 ? c->s14_ptr.p : nullptr, c->s14_idx.p,
       c->ref_cell_sorted.p, c->pos, c->nbr_count.p, c->nbr_list.p, st, thr2);

@@ -122,7 +122,7 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, const float4* 
           in = within_cutoff64(pos, oi, sorted_orig[e[u] & 0x3fffffffu], a.Lxd, a.Lyd, a.Lzd, a.rcd);
         }
         const float r2 = in ? r2raw : 1.0f;
-        const float rinv = rsqrtf(r2);
+        const float rinv = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32: r2 >= ~0.5 A^2, never denormal
         const float r2inv = rinv * rinv;
         float fr = 0.f;
         if (LJ) {
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, const float4* 
           const float qq = xi.w * xj[u].w;
           const float x = a.beta * (r2 * rinv);
           const float ex = __expf(-x * x);
-          const float tt = __frcp_rn(1.f + ERFC_P * x);
+          const float tt = __builtin_amdgcn_rcpf(1.f + ERFC_P * x);  // v_rcp_f32 (1 ulp)
           const float poly =
               ERFC_C1 + tt * (ERFC_C2 + tt * (ERFC_C3 + tt * (ERFC_C4 + tt * (ERFC_C5 + tt * (ERFC_C6 + tt * ERFC_C7)))));
           const float erfcx = tt * poly * ex;

@@ -236,10 +236,12 @@ struct PmeArgs {
   int n;
   int Kx, Ky, Kz;
   double Lx, Ly, Lz;
+  double iLx, iLy, iLz;
 };
 
-__device__ __forceinline__ void frac_coord(double x, double L, int K, int* base, float* w) {
-  double s = x / L;
+// fractional grid coordinate of x along an axis with inverse edge iL and K points
+__device__ __forceinline__ void frac_coord(double x, double iL, int K, int* base, float* w) {
+  double s = x * iL;
   s -= floor(s);
   double u = s * K;
   double f = floor(u);
@@ -261,9 +263,9 @@ __global__ __launch_bounds__(PME_BLOCK) void k_spread(PmeArgs a, const double* _
   if (q == 0.f) return;
   int bx, by, bz;
   float wx, wy, wz;
-  frac_coord(pos[3 * i], a.Lx, a.Kx, &bx, &wx);
-  frac_coord(pos[3 * i + 1], a.Ly, a.Ky, &by, &wy);
-  frac_coord(pos[3 * i + 2], a.Lz, a.Kz, &bz, &wz);
+  frac_coord(pos[3 * i], a.iLx, a.Kx, &bx, &wx);
+  frac_coord(pos[3 * i + 1], a.iLy, a.Ky, &by, &wy);
+  frac_coord(pos[3 * i + 2], a.iLz, a.Kz, &bz, &wz);
   float tx[4], ty[4], tz[4];
   bspline4(wx, tx, nullptr);
   bspline4(wy, ty, nullptr);
@@ -291,9 +293,9 @@ __global__ __launch_bounds__(PME_BLOCK) void k_gather(PmeArgs a, const double* _
   if (q == 0.f) return;
   int bx, by, bz;
   float wx, wy, wz;
-  frac_coord(pos[3 * i], a.Lx, a.Kx, &bx, &wx);
-  frac_coord(pos[3 * i + 1], a.Ly, a.Ky, &by, &wy);
-  frac_coord(pos[3 * i + 2], a.Lz, a.Kz, &bz, &wz);
+  frac_coord(pos[3 * i], a.iLx, a.Kx, &bx, &wx);
+  frac_coord(pos[3 * i + 1], a.iLy, a.Ky, &by, &wy);
+  frac_coord(pos[3 * i + 2], a.iLz, a.Kz, &bz, &wz);
   float tx[4], ty[4], tz[4], dx[4], dy[4], dz[4];
   bspline4(wx, tx, dx);
   bspline4(wy, ty, dy);
@@ -331,12 +333,13 @@ __global__ __launch_bounds__(PME_BLOCK) void k_gather(PmeArgs a, const double* _
 // contiguous ranges of sorted slots, and no atom is more than skin/2 away from its slice.
 // Phase 1 (one lane per candidate) compacts the touching atoms into LDS records; phase 2 gives
 // each record 16 lanes, one per (ky, kz) point of its 4x4 stencil in the plane.
-#define PLANE_THREADS 512
+#define PLANE_THREADS 1024
 
 struct PlaneArgs {
   int n;
   int Kx, Ky, Kz;
   double Lx, Ly, Lz;
+  double iLx, iLy, iLz;
   int gnx, gny, gnz;  // neighbour-list grid
   double wx;          // its cell width along x
   double margin;      // list skin/2 plus rounding slack
@@ -457,15 +460,15 @@ __global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_fwd(PlaneArgs a, Ff
         if (sl[u] >= 0) {
           int bx;
           float wx;
-          frac_coord(px[u], a.Lx, a.Kx, &bx, &wx);
+          frac_coord(px[u], a.iLx, a.Kx, &bx, &wx);
           int kx = bx - x0;
           kx += kx < 0 ? a.Kx : 0;
           if (kx <= 3) {
             const int s = sl[u];
             const float q = xq[s].w;
             int by, bz;
-            frac_coord(pos_sorted[3 * s + 1], a.Ly, a.Ky, &by, &r.wy);
-            frac_coord(pos_sorted[3 * s + 2], a.Lz, a.Kz, &bz, &r.wz);
+            frac_coord(pos_sorted[3 * s + 1], a.iLy, a.Ky, &by, &r.wy);
+            frac_coord(pos_sorted[3 * s + 2], a.iLz, a.Kz, &bz, &r.wz);
             r.yz = by | (bz << 12);
             r.qx = q * bsp4(wx, kx);
             keep = q != 0.f;
@@ -562,7 +565,7 @@ __global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_inv_gather(
         GatherRec r;
         if (sl[u] >= 0) {
           int bx;
-          frac_coord(px[u], a.Lx, a.Kx, &bx, &r.wx);
+          frac_coord(px[u], a.iLx, a.Kx, &bx, &r.wx);
           int kx = bx - x0;
           kx += kx < 0 ? a.Kx : 0;
           if (kx <= 3) {
@@ -575,8 +578,8 @@ __global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_inv_gather(
               atomicAdd(&bcount, 1);
             } else {
               int by, bz;
-              frac_coord(pos_sorted[3 * s + 1], a.Ly, a.Ky, &by, &r.wy);
-              frac_coord(pos_sorted[3 * s + 2], a.Lz, a.Kz, &bz, &r.wz);
+              frac_coord(pos_sorted[3 * s + 1], a.iLy, a.Ky, &by, &r.wy);
+              frac_coord(pos_sorted[3 * s + 2], a.iLz, a.Kz, &bz, &r.wz);
               r.yzk = by | (bz << 12) | (kx << 24);
               keep = true;
             }
@@ -824,6 +827,9 @@ static int pme_enqueue_planes(mdx_ctx* c) {
   a.Lx = c->box[0];
   a.Ly = c->box[1];
   a.Lz = c->box[2];
+  a.iLx = 1.0 / c->box[0];
+  a.iLy = 1.0 / c->box[1];
+  a.iLz = 1.0 / c->box[2];
   a.gnx = c->gn[0];
   a.gny = c->gn[1];
   a.gnz = c->gn[2];
@@ -868,6 +874,9 @@ int mdx_pme_enqueue(mdx_ctx* c, bool energy) {
   a.Lx = c->box[0];
   a.Ly = c->box[1];
   a.Lz = c->box[2];
+  a.iLx = 1.0 / c->box[0];
+  a.iLy = 1.0 / c->box[1];
+  a.iLz = 1.0 / c->box[2];
   k_spread<<<div_up(16LL * c->n, PME_BLOCK), PME_BLOCK, 0, c->stream>>>(a, c->pos, c->charge_s.p, c->qfix.p);
   MDX_LAUNCH_CHECK();
   int rc;
