@@ -139,7 +139,11 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, const float4* 
           const float qq = xi.w * xj[u].w;
           const float x = a.beta * (r2 * rinv);
           const float ex = __expf(-x * x);
-          const float tt = __builtin_amdgcn_rcpf(1.f + ERFC_P * x);  // v_rcp_f32 (1 ulp)
+          // 1/(1 + p x): v_rcp_f32 plus one Newton step (the raw 1-ulp reciprocal is biased, and the
+          // direct-space energy sum cancels strongly enough to show it)
+          const float den = 1.f + ERFC_P * x;
+          const float r0 = __builtin_amdgcn_rcpf(den);
+          const float tt = fmaf(r0, fmaf(-den, r0, 1.f), r0);
           const float poly =
               ERFC_C1 + tt * (ERFC_C2 + tt * (ERFC_C3 + tt * (ERFC_C4 + tt * (ERFC_C5 + tt * (ERFC_C6 + tt * ERFC_C7)))));
           const float erfcx = tt * poly * ex;
