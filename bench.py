@@ -55,34 +55,79 @@ def cpu_baseline(system, steps=2):
                       "%.1f s wall; steady-state step = 3 evaluations" % (steps, evals, wall)}
 
 
-def nve_leg(e, terms, steps=1000, dt=1.0, every=10):
-    """Energy drift of velocity Verlet from the thermalised state: slope of E_total(t) from a
-    linear fit, per atom, in kJ/mol/ns."""
-    from mdpy_amd import _lib as L
+def drift_fit(etot, dt, every, n_atoms, block=20):
+    """Slope of E_total(t) from a least-squares line, per atom, in kJ/mol/ns, with its standard
+    error from the residual variance inflated by sqrt(block) for correlated samples."""
     from mdpy_amd.unit import KJ_PER_MOL
-    e.configure_integrator(L.INT_VELOCITY_VERLET, dt=dt)
-    e.integrate(terms, steps, True, every)
-    h = e.energy_history()
-    etot = h[:, :L.NTERMS].sum(1) + h[:, L.E_KINETIC]
-    t_ns = np.arange(len(etot)) * every * dt * 1e-6
-    slope = np.polyfit(t_ns, etot, 1)[0] / KJ_PER_MOL / e.n
-    return {"steps": steps, "dt_fs": dt, "drift_kj_mol_ns_atom": float(slope),
-            "abs_drift_kj_mol_ns_atom": float(abs(slope)),
-            "etot_std_kj_mol_atom": float(etot.std() / KJ_PER_MOL / e.n), "device_flags": e.status()}
+    t = np.arange(len(etot)) * every * dt * 1e-6
+    A = np.stack([t, np.ones_like(t)], 1)
+    coef = np.linalg.lstsq(A, etot, rcond=None)[0]
+    r = etot - A @ coef
+    se = np.sqrt(r.var() * 12.0 / len(t)) / (t[-1] - t[0]) * np.sqrt(block)
+    scale = KJ_PER_MOL * n_atoms
+    return float(coef[0] / scale), float(se / scale), float(etot.std() / scale)
 
 
-def chain_leg(steps=1000, warmup=100):
-    """ns/day of the solvated chain (1,200-atom chain in 10,000 waters, 68 A, r_c 12 A, PME 72^3)."""
+def nve_leg(e, terms, masses):
+    """Energy drift of velocity Verlet from a thermalised state (G-JF Langevin 3,000 steps at
+    gamma 0.05/fs, then 2,000 at 0.01/fs): 1,000 steps at 1 fs with energies every step, then
+    20,000 steps at 1 fs and at 0.5 fs with energies every 10 steps, all from the same state."""
+    from mdpy_amd import _lib as L
+    from mdpy_amd.unit import KB
+    for gamma, n in ((0.05, 3000), (0.01, 2000)):
+        e.configure_integrator(L.INT_LANGEVIN, dt=1.0, temperature=300.0, friction=gamma, seed=7)
+        e.integrate(terms, n, True, 100)
+    v = e.velocities()
+    t_start = float((masses[:, None] * v * v).sum() / (3 * e.n * KB))
+    x0, v0 = e.pos.clone(), e.vel.clone()
+    out = {"start_temperature_K": t_start, "runs": []}
+    for dt, steps, every in ((1.0, 1000, 1), (1.0, 20000, 10), (0.5, 20000, 10)):
+        e.set_positions(x0, wrap=False)
+        e.set_velocities(v0)
+        e.configure_integrator(L.INT_VELOCITY_VERLET, dt=dt)
+        e.integrate(terms, steps, True, every)
+        h = e.energy_history()
+        etot = h[:, :L.NTERMS].sum(1) + h[:, L.E_KINETIC]
+        slope, se, sd = drift_fit(etot, dt, every, e.n)
+        out["runs"].append({"dt_fs": dt, "steps": steps, "drift_kj_mol_ns_atom": slope,
+                            "stderr_kj_mol_ns_atom": se, "etot_std_kj_mol_atom": sd,
+                            "device_flags": e.status()})
+    first = out["runs"][0]
+    out["drift_kj_mol_ns_atom"] = first["drift_kj_mol_ns_atom"]
+    out["note"] = ("a 1,000-step fit resolves the slope only to its standard error (energy "
+                   "fluctuations of flexible water); the 20,000-step runs resolve it")
+    return out
+
+
+def chain_leg(steps=1000):
+    """ns/day of the solvated chain (1,200-atom chain in 10,000 waters, 68 A, r_c 12 A, PME 72^3).
+    Thermalised first (G-JF Langevin, gamma 0.05/fs then 0.01/fs) in 250-step segments that stop
+    the leg if the temperature leaves 200-600 K or a device flag is raised; the timed run is
+    G-JF at gamma 0.01/fs."""
     import torch
     from mdpy_amd import _lib as L, systems
     from mdpy_amd.engine import Engine
+    from mdpy_amd.unit import KB
     s = systems.solvated_chain()
     e = Engine(s, skin=1.5)
     e.set_positions(s.positions)
     e.set_velocities(maxwell_velocities(s.masses, 300.0, 0))
-    e.configure_integrator(L.INT_LANGEVIN, dt=1.0, temperature=300.0, friction=1e-3, seed=0)
     terms = L.TERM_LJ | L.TERM_PME | L.TERM_BOND | L.TERM_ANGLE | L.TERM_DIHEDRAL | L.TERM_IMPROPER
-    e.integrate(terms, warmup, True, 50)
+
+    def temperature():
+        v = e.velocities()
+        return float((s.masses[:, None] * v * v).sum() / (3 * s.num_atoms * KB))
+
+    temps = []
+    for gamma, n in ((0.05, 2000), (0.01, 1000)):
+        e.configure_integrator(L.INT_LANGEVIN, dt=1.0, temperature=300.0, friction=gamma, seed=0)
+        for _ in range(n // 250):
+            e.integrate(terms, 250, True, 250)
+            temps.append(temperature())
+            if e.status() or not 200.0 < temps[-1] < 600.0:
+                out = {"error": "thermalisation left bounds", "temperatures_K": temps, "device_flags": e.status()}
+                e.close()
+                return out
     lanes = e.tune_nonbonded(L.TERM_LJ | L.TERM_COUL_DIRECT, 0, reps=10)
     e.integrate(terms, 50, True, 50)
     torch.cuda.synchronize()
@@ -94,7 +139,7 @@ def chain_leg(steps=1000, warmup=100):
     nb_ms = e.time_kernel(0, L.TERM_LJ | L.TERM_COUL_DIRECT, reps=20)
     out = {"atoms": s.num_atoms, "ns_per_day": 0.0864 / sps, "ms_per_step": sps * 1e3, "nb_lanes": lanes,
            "nonbonded_ms": nb_ms, "unique_pairs": pairs, "pairs_per_s": pairs / (nb_ms * 1e-3),
-           "device_flags": e.status()}
+           "temperature_K": temperature(), "device_flags": e.status()}
     e.close()
     return out
 
@@ -179,7 +224,8 @@ def main():
                     help="also measure the 64 batched water replicas (aggregate atom-steps/s per batch mode)")
     ap.add_argument("--mlp", action="store_true",
                     help="also train and measure the MLP surrogate on this box (adds a few minutes)")
-    ap.add_argument("--nve", action="store_true", help="also run 1,000 velocity-Verlet steps and report the drift")
+    ap.add_argument("--no-nve", action="store_true",
+                    help="skip the NVE drift leg (thermalise, then 1,000 + 2 x 20,000 velocity-Verlet steps)")
     ap.add_argument("--chain", action="store_true", help="also measure the solvated 31k-atom chain (72^3 PME)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB (a kernel of known "
@@ -265,9 +311,9 @@ def main():
     def progress(msg):
         print("bench: " + msg, file=sys.stderr, flush=True)
 
-    if args.nve:
+    if not args.no_nve:
         progress("NVE drift leg")
-        out["nve"] = nve_leg(e, terms)
+        out["nve"] = nve_leg(e, terms, s.masses)
     if args.chain:
         progress("solvated chain leg")
         out["chain"] = chain_leg()

@@ -184,7 +184,10 @@ class Engine:
             L.call("mdx_wrap_positions", self.ctx)
 
     def set_velocities(self, v):
-        self.vel.copy_(self.torch.as_tensor(np.asarray(v, dtype=np.float64), device=self.device))
+        if isinstance(v, self.torch.Tensor):
+            self.vel.copy_(v.to(device=self.device, dtype=self.torch.float64))
+        else:
+            self.vel.copy_(self.torch.as_tensor(np.asarray(v, dtype=np.float64), device=self.device))
 
     def positions(self):
         return self.pos.cpu().numpy()

@@ -239,34 +239,56 @@ def _min_image(d, L):
     return d - L * np.round(d / L)
 
 
-def solvated_chain(seed=0, n_chain=1200, n_waters=10000, L=68.0, cutoff=12.0, pme_grid=(72, 72, 72)):
-    """Self-avoiding random-walk chain with CHARMM-form bonded terms, embedded in water."""
-    rng = np.random.default_rng(seed)
-    # --- chain geometry: bond lengths U(1.0,1.5); successive bond directions within 70 deg;
-    # every non-bonded pair (|i-j| > 2) kept >= 2.5 A apart under the minimum image.
+def _chain_molecule(rng, n_chain, L):
+    """Positions and CHARMM-form parameters of a self-avoiding random-walk chain in a cube of
+    edge L (periodic).
+
+    Geometry: bond lengths U(1.0, 1.5) A; successive bond directions 50-75 deg apart (bond
+    angles 105-130 deg, so no angle approaches 180 deg, where the dihedral normals vanish);
+    every non-bonded pair (|i-j| > 2) at least 2.5 A apart under the minimum image.
+    Parameters: bonds k U(200, 400) kcal/mol/A^2 at their generated length; angles k U(40, 80)
+    kcal/mol/rad^2 at their generated angle, 30% with a Urey-Bradley term k U(5, 20); 1-3
+    periodic dihedral terms per quad, k U(0, 2) kcal/mol, n in {1, 2, 3}, delta in {0, 180};
+    every tenth quad (moved forward to the next quad whose angle is within 90 deg of 0) carries a
+    harmonic improper k U(5, 20) near its generated angle. The improper energy k (psi - psi0)^2 is
+    not periodic (mdpy/constraint/charmm_improper_constraint.py:70,90), so psi0 is kept away from
+    +-180 deg, where a crossing would jump the energy. Charges U(-0.5, 0.5) e shifted to a
+    neutral chain; LJ eps U(0.05, 0.2) kcal/mol, Rmin/2 U(1.7, 2.1) A, 1-4 eps halved.
+    """
     pos = np.zeros((n_chain, 3))
     pos[0] = rng.uniform(0, L, 3)
-    prev_dir = None
-    i = 1
-    tries = 0
+    dirs = np.zeros((n_chain, 3))
+    c_lo, c_hi = math.cos(math.radians(75)), math.cos(math.radians(50))
+    i, fails, tries = 1, 0, 0
     while i < n_chain:
         tries += 1
-        if tries > 200000:
+        if tries > 400000:
             raise RuntimeError("chain generation failed")
-        v = rng.normal(size=3)
-        v /= np.linalg.norm(v)
-        if prev_dir is not None and np.dot(v, prev_dir) < math.cos(math.radians(70)):
-            continue
+        if i == 1:
+            v = rng.normal(size=3)
+            v /= np.linalg.norm(v)
+        else:  # uniform on the cone band around the previous direction
+            p = dirs[i - 1]
+            u = np.cross(p, [1.0, 0.0, 0.0] if abs(p[0]) < 0.9 else [0.0, 1.0, 0.0])
+            u /= np.linalg.norm(u)
+            w = np.cross(p, u)
+            c = rng.uniform(c_lo, c_hi)
+            az = rng.uniform(0, 2 * math.pi)
+            v = c * p + math.sqrt(1 - c * c) * (math.cos(az) * u + math.sin(az) * w)
         b = rng.uniform(1.0, 1.5)
         cand = pos[i - 1] + b * v
         if i > 2:
             d = _min_image(pos[: i - 2] - cand, L)
             if np.min(np.einsum("ij,ij->i", d, d)) < 2.5 ** 2:
+                fails += 1
+                if fails > 200:  # trapped: step back a few monomers
+                    i = max(1, i - 5)
+                    fails = 0
                 continue
         pos[i] = cand
-        prev_dir = v
+        dirs[i] = v
+        fails = 0
         i += 1
-    # --- chain parameters
     idx = np.arange(n_chain)
     bonds = np.stack([idx[:-1], idx[1:]], 1)
     blen = np.linalg.norm(pos[1:] - pos[:-1], axis=1)
@@ -285,18 +307,48 @@ def solvated_chain(seed=0, n_chain=1200, n_waters=10000, L=68.0, cutoff=12.0, pm
         for n in rng.choice([1, 2, 3], size=nterms, replace=False):
             dih_rows.append([a, a + 1, a + 2, a + 3])
             dih_params.append([rng.uniform(0, 2) * KCAL_PER_MOL, float(n), math.radians(rng.choice([0.0, 180.0]))])
-    dihedrals = np.array(dih_rows, dtype=np.int64)
-    dihedral_params = np.array(dih_params)
-    imp_rows = [[a, a + 1, a + 2, a + 3] for a in range(0, n_chain - 3, 10)]
-    impropers = np.array(imp_rows, dtype=np.int64)
-    psi = _dihedral_angles(pos, impropers, L)
+    dihedrals = np.array(dih_rows, dtype=np.int64).reshape(-1, 4)
+    dihedral_params = np.array(dih_params).reshape(-1, 3)
+    quads = np.array([[a, a + 1, a + 2, a + 3] for a in range(n_chain - 3)], dtype=np.int64).reshape(-1, 4)
+    psi_all = _dihedral_angles(pos, quads, L) if len(quads) else np.zeros(0)
+    imp_rows = []
+    for a0 in range(0, n_chain - 3, 10):
+        for a in range(a0, min(a0 + 10, n_chain - 3)):
+            if abs(psi_all[a]) < 0.5 * math.pi:
+                imp_rows.append(a)
+                break
+    impropers = quads[imp_rows].reshape(-1, 4)
     improper_params = np.stack([rng.uniform(5, 20, len(imp_rows)) * KCAL_PER_MOL,
-                                psi + rng.uniform(-0.1, 0.1, len(imp_rows))], 1)
-    q = rng.uniform(-0.6, 0.6, n_chain)
+                                psi_all[imp_rows] + rng.uniform(-0.1, 0.1, len(imp_rows))], 1).reshape(-1, 2)
+    q = rng.uniform(-0.5, 0.5, n_chain)
     q -= q.mean()
-    eps = rng.uniform(0.02, 0.2, n_chain) * KCAL_PER_MOL
-    sig = 2.0 * rng.uniform(1.4, 2.1, n_chain) * RMIN_TO_SIGMA_FACTOR
-    lj_chain = np.stack([eps, sig, 0.5 * eps, sig], 1)
+    eps = rng.uniform(0.05, 0.2, n_chain) * KCAL_PER_MOL
+    sig = 2.0 * rng.uniform(1.7, 2.1, n_chain) * RMIN_TO_SIGMA_FACTOR
+    lj = np.stack([eps, sig, 0.5 * eps, sig], 1)
+    return dict(pos=pos, bonds=bonds, bond_params=bond_params, angles=angles, angle_params=angle_params,
+                dihedrals=dihedrals, dihedral_params=dihedral_params, impropers=impropers,
+                improper_params=improper_params, charges=q, lj=lj)
+
+
+def chain_in_vacuum(seed=0, n_chain=120, L=200.0, cutoff=12.0):
+    """The chain molecule alone in a large periodic cube (no solvent)."""
+    rng = np.random.default_rng(seed)
+    c = _chain_molecule(rng, n_chain, L)
+    positions = c["pos"] - L * np.floor(c["pos"] / L)
+    return System(
+        name="chain%d_vacuum" % n_chain, positions=positions, box=np.array([L, L, L]),
+        masses=np.full(n_chain, 12.011), charges=c["charges"], lj=c["lj"],
+        types=["C%d" % a for a in range(n_chain)], bonds=c["bonds"], bond_params=c["bond_params"],
+        angles=c["angles"], angle_params=c["angle_params"], dihedrals=c["dihedrals"],
+        dihedral_params=c["dihedral_params"], impropers=c["impropers"], improper_params=c["improper_params"],
+        cutoff=cutoff, pme_grid=(64, 64, 64), meta={"n_chain": n_chain, "seed": seed})
+
+
+def solvated_chain(seed=0, n_chain=1200, n_waters=10000, L=68.0, cutoff=12.0, pme_grid=(72, 72, 72)):
+    """Self-avoiding random-walk chain with CHARMM-form bonded terms, embedded in water."""
+    rng = np.random.default_rng(seed)
+    c = _chain_molecule(rng, n_chain, L)
+    pos = c["pos"]
     # --- waters on a 23^3 lattice, minus sites that overlap the chain, then a seeded subset
     m = 23
     g = (np.arange(m) + 0.5) * (L / m)
@@ -317,14 +369,14 @@ def solvated_chain(seed=0, n_chain=1200, n_waters=10000, L=68.0, cutoff=12.0, pm
     return System(
         name="chain31200", positions=positions, box=np.array([L, L, L]),
         masses=np.concatenate([np.full(n_chain, 12.011), wm]),
-        charges=np.concatenate([q, wq]), lj=np.concatenate([lj_chain, wlj]),
+        charges=np.concatenate([c["charges"], wq]), lj=np.concatenate([c["lj"], wlj]),
         types=["C%d" % a for a in range(n_chain)] + wtypes,
-        bonds=np.concatenate([bonds, wb]).astype(np.int64),
-        bond_params=np.concatenate([bond_params, wbp]),
-        angles=np.concatenate([angles, wa]).astype(np.int64),
-        angle_params=np.concatenate([angle_params, wap]),
-        dihedrals=dihedrals, dihedral_params=dihedral_params,
-        impropers=impropers, improper_params=improper_params,
+        bonds=np.concatenate([c["bonds"], wb]).astype(np.int64),
+        bond_params=np.concatenate([c["bond_params"], wbp]),
+        angles=np.concatenate([c["angles"], wa]).astype(np.int64),
+        angle_params=np.concatenate([c["angle_params"], wap]),
+        dihedrals=c["dihedrals"], dihedral_params=c["dihedral_params"],
+        impropers=c["impropers"], improper_params=c["improper_params"],
         cutoff=cutoff, pme_grid=tuple(pme_grid),
         meta={"n_chain": n_chain, "n_waters": n_waters, "water_sites_deleted": int((~keep).sum()), "seed": seed})
 

@@ -149,7 +149,11 @@ def oracle_terms(s, xw, pairs, beta):
     Eb = EW.background_energy(s.charges, s.box, beta, ONE_OVER_4PI_EPS0)
     out["coulomb_direct"] = (Fd, Ed)
     out["coulomb_reciprocal"] = (Fr + Fx, Er + Ex + Es + Eb)
-    out["_scale"] = {"coulomb_reciprocal": abs(Er) + abs(Ex) + abs(Es) + abs(Eb)}
+    # sum of |pair energy| (the same sum over |q|): float32 pair arithmetic errs per pair, so the
+    # error of a total that cancels to a small value scales with this, not with the total
+    _, Ed_abs = EW.direct_terms(xw, np.abs(s.charges), pairs, s.box, beta, ONE_OVER_4PI_EPS0)
+    out["_scale"] = {"coulomb_reciprocal": abs(Er) + abs(Ex) + abs(Es) + abs(Eb),
+                     "coulomb_direct": max(abs(Ed), 1e-2 * Ed_abs)}
     out["bond"] = R.bond_terms(xw, s.bonds, s.bond_params, s.box)
     out["angle"] = R.angle_terms(xw, s.angles, s.angle_params, s.box)
     out["dihedral"] = R.dihedral_terms(xw, s.dihedrals, s.dihedral_params, s.box)
