@@ -120,6 +120,12 @@ int mdx_neighbor_pairs(mdx_ctx* ctx, int* d_pairs, int64_t capacity, int64_t* h_
 /* ---- PME (new; no reference counterpart, mdpy/constraint/__init__.py:22) ------------------ */
 /* grid K (3 ints), B-spline order (4), Ewald beta (1/A). */
 int mdx_pme_config(mdx_ctx* ctx, const int* h_grid, int order, double beta);
+/* Reciprocal space in one launch with two grid barriers (enable != 0) or in three launches
+ * (spread + 2D FFT, x-axis FFT, 2D inverse + gather; the default). Same forces. The one-launch
+ * kernel needs its Kx blocks resident together and is never used inside an mdx_batch; on
+ * MI355X its barriers (cache write-back and invalidation across the XCDs) cost more than the
+ * launch gaps they remove, so it stays off unless selected. */
+int mdx_pme_set_fused(mdx_ctx* ctx, int enable);
 
 /* ---- force evaluation (Constraint.update / Ensemble.update, ensemble.py:53-61) ------------ */
 /* Evaluate the terms in `terms` at the current positions. If d_force_out != NULL, write the
@@ -147,8 +153,8 @@ int mdx_set_energy_interval(mdx_ctx* ctx, int every);
  * h_out[k*(MDX_NTERMS+1) + t] for k < min(rows, h_cap); *h_count = rows. Synchronises. */
 int mdx_energy_history_fetch(mdx_ctx* ctx, double* h_out, int h_cap, int* h_count);
 /* Status flags raised on the device since the last call (bit 0 overflow, 1 particle loss,
- * 2 non-finite, 3 PME coverage: an atom missed by the reciprocal-space gather); clears them.
- * Synchronises. */
+ * 2 non-finite, 3 PME coverage: an atom missed by the reciprocal-space gather, 4 the fused
+ * PME kernel's grid barrier gave up waiting for its blocks); clears them. Synchronises. */
 int mdx_status_fetch(mdx_ctx* ctx, unsigned* h_flags);
 
 /* ---- batched replicas (new; no reference counterpart) ---------------------------------- */

@@ -850,6 +850,9 @@ extern "C" int mdx_status_fetch(mdx_ctx* c, unsigned* h_flags) {
   MDX_HIP(hipMemcpyAsync(h_flags, &c->dstate.p->flags, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
   MDX_HIP(hipMemsetAsync(&c->dstate.p->flags, 0, sizeof(unsigned), c->stream));
   MDX_HIP(hipStreamSynchronize(c->stream));
+  if (*h_flags & MDX_FLAG_SYNC) {  // a barrier gave up: re-arm its arrival count
+    MDX_HIP(hipMemset(&c->dstate.p->bar_count, 0, sizeof(unsigned)));
+  }
   return 0;
 }
 
@@ -1163,6 +1166,12 @@ extern "C" int mdx_batch_create(mdx_ctx** ctxs, int n, void* hip_stream, mdx_bat
   MDX_HIP(hipSetDevice(ctxs[0]->device));
   mdx_batch* b = new mdx_batch();
   b->ctx.assign(ctxs, ctxs + n);
+  // the one-launch PME kernel needs all its blocks resident at once; many contexts advancing
+  // concurrently could together exceed the device, so batched contexts use the three-kernel path
+  for (int i = 0; i < n; ++i) {
+    ctxs[i]->in_batch = true;
+    mdx_invalidate_graph(ctxs[i]);
+  }
   b->ext = (hipStream_t)hip_stream;
   b->ev_out.resize(n, nullptr);
   bool ok = hipStreamCreateWithFlags(&b->lead, hipStreamNonBlocking) == hipSuccess &&
@@ -1180,6 +1189,10 @@ extern "C" int mdx_batch_create(mdx_ctx** ctxs, int n, void* hip_stream, mdx_bat
 
 extern "C" int mdx_batch_destroy(mdx_batch* b) {
   if (!b) return 0;
+  for (mdx_ctx* c : b->ctx) {  // standalone again
+    c->in_batch = false;
+    mdx_invalidate_graph(c);
+  }
   for (hipEvent_t e : b->ev_out)
     if (e) hipEventDestroy(e);
   if (b->ev_in) hipEventDestroy(b->ev_in);

@@ -23,6 +23,7 @@ __device__ __forceinline__ long long to_fix_f(float v) { return __double2ll_rn((
 #define MDX_FLAG_LOSS 2u
 #define MDX_FLAG_NONFINITE 4u
 #define MDX_FLAG_PME_COVERAGE 8u   // reciprocal-space gather missed an atom (internal invariant)
+#define MDX_FLAG_SYNC 16u          // a grid barrier gave up waiting (blocks not co-resident)
 
 // ------------------------------------------------------------------ error handling
 void mdx_set_error(const char* fmt, ...);
@@ -92,6 +93,8 @@ struct DevState {
   int pme_slots;            // reciprocal force slots written in the current evaluation
   unsigned pme_ticket;      // finished blocks of the PME gather kernel
   unsigned step_ticket;     // finished blocks of the fused step prologue
+  unsigned bar_count;       // grid barrier of the fused PME kernel: blocks arrived
+  unsigned bar_gen;         // and its generation (advanced by the last arrival)
 };
 
 // float32 sorted-order copy of one atom: wrapped, centred position and q*sqrt(ke) (the layout
@@ -183,6 +186,10 @@ struct mdx_ctx {
   DevBuf<double> fpart;          // plane path: reciprocal force per (kx, atom), 4 x 3N
   bool pme_planes = false;       // plane kernels usable (list grid present, plane fits LDS)
   int pme_wf = 0, pme_wg = 0;    // candidate windows of the two plane kernels
+  int pme_fused_B = 0;           // x-axis lines per chunk of the one-launch PME kernel (0: not usable)
+  size_t pme_fused_lds = 0;      // its dynamic LDS
+  bool pme_fused_on = false;     // one-launch PME kernel selected (never inside a batch)
+  bool in_batch = false;         // advanced by an mdx_batch
   bool fpart_live = false;       // fpart belongs to the current evaluation
   // ---- integrator
   bool int_configured = false;

@@ -138,6 +138,59 @@ __device__ float2* lds_fft(float2* src, float2* dst, const FftPlan& pl, int B, c
   return src;
 }
 
+// ---- 64 x 64 planes: each wave transforms 4 whole lines, so the three radix-4 Stockham passes
+// of a 64-point line need only wave-level ordering (no workgroup barrier between passes), and
+// every index is a compile-time shift or mask. Planes are held with row stride 65 (PLANE64_LS)
+// so that both rows and columns are read across lanes without LDS bank conflicts.
+#define PLANE64_LS 65
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// one radix-4 pass (p = 4^S) of the 64-point line at X/Y + off, elements es apart; lane i of
+// the line's 16 computes butterfly i
+template <int DIR, int S>
+__device__ __forceinline__ void fft64_pass(const float2* X, float2* Y, int off, int es, int i,
+                                           const float2* __restrict__ tw) {
+  constexpr int p = 1 << (2 * S);
+  constexpr int tstep = 16 >> (2 * S);
+  const int k = i & (p - 1);
+  float2 u[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) u[r] = X[off + (i + 16 * r) * es];
+  if (S > 0) {
+#pragma unroll
+    for (int r = 1; r < 4; ++r) {
+      float2 w = tw[(r * k * tstep) & 63];
+      if (DIR > 0) w.y = -w.y;
+      u[r] = cmul(u[r], w);
+    }
+  }
+  dft4(u, (float)DIR);
+  const int j = ((i - k) << 2) + k;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) Y[off + (j + r * p) * es] = u[r];
+}
+
+// 64-point transforms of the 64 rows (rows = true: elements contiguous) or the 64 columns of a
+// padded plane in a, with b as the ping-pong buffer; the result ends in b. Wave w takes lines
+// w, w + 16, w + 32, w + 48 (1024 threads).
+template <int DIR>
+__device__ __forceinline__ void plane64_lines(float2* a, float2* b, bool rows, const float2* __restrict__ tw) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int line = w + 16 * (lane >> 4), i = lane & 15;
+  const int off = rows ? line * PLANE64_LS : line;
+  const int es = rows ? 1 : PLANE64_LS;
+  fft64_pass<DIR, 0>(a, b, off, es, i, tw);
+  wave_lds_sync();
+  fft64_pass<DIR, 1>(b, a, off, es, i, tw);
+  wave_lds_sync();
+  fft64_pass<DIR, 2>(a, b, off, es, i, tw);
+}
+
 __device__ __forceinline__ void load_twiddles(float2* dst, const float2* __restrict__ src, int K) {
   for (int i = threadIdx.x; i < K; i += blockDim.x) dst[i] = src[i];
 }
@@ -148,21 +201,22 @@ __device__ __forceinline__ long long line_base(const FftLines& ln, int l) {
 
 // MODE 0: complex in/out.  MODE 1: load int64 fixed grid (real), clear it.  MODE 2: store real
 // part to a float grid.  MODE 3: forward, multiply by B*C (+ energy), inverse, in place.
+// lines [l0, l0 + B) of one axis pass, by one workgroup, in the LDS region lds
 template <int DIR, int MODE>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(FftPlan pl, FftLines ln, const float2* __restrict__ tw,
-                                                        float2* __restrict__ cg, long long* __restrict__ qfix,
-                                                        float* __restrict__ rout, const float* __restrict__ bc,
-                                                        long long* __restrict__ eslot) {
-  extern __shared__ float2 lds[];
+__device__ void fft_axis_body(const FftPlan& pl, const FftLines& ln, const float2* __restrict__ tw,
+                              float2* __restrict__ cg, long long* __restrict__ qfix, float* __restrict__ rout,
+                              const float* __restrict__ bc, long long* __restrict__ eslot, int l0, float2* lds) {
   const int K = pl.K;
   const int B = ln.B;
-  const int l0 = blockIdx.x * B;
   const int nb = min(B, ln.nlines - l0);
-  float2* A = lds;
-  float2* Bf = lds + B * K;
-  float2* stw = Bf + B * K;
-  load_twiddles(stw, tw, K);
   const bool contiguous = ln.estride == 1;
+  // lines of strided axes are loaded line-across-lanes: pad the LDS line stride to K + 1 so
+  // consecutive lanes fall on different banks
+  const int ls = contiguous ? K : K + 1;
+  float2* A = lds;
+  float2* Bf = lds + B * ls;
+  float2* stw = Bf + B * ls;
+  load_twiddles(stw, tw, K);
   // load
   for (int idx = threadIdx.x; idx < nb * K; idx += blockDim.x) {
     int l, e;
@@ -177,10 +231,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(FftPlan pl, FftLines l
     } else {
       v = cg[g];
     }
-    A[l * K + e] = v;
+    A[l * ls + e] = v;
   }
   __syncthreads();
-  float2* res = lds_fft<DIR>(A, Bf, pl, nb, stw);
+  float2* res = lds_fft<DIR>(A, Bf, pl, nb, stw, ls);
   if (MODE == 3) {
     float2* other = (res == A) ? Bf : A;
     double en = 0.0;
@@ -188,20 +242,20 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(FftPlan pl, FftLines l
       int l = idx / K, e = idx - l * K;
       long long g = line_base(ln, l0 + l) + (long long)e * ln.estride;
       float f = bc[g];
-      float2 v = res[idx];
+      float2 v = res[l * ls + e];
       en += 0.5 * (double)f * ((double)v.x * v.x + (double)v.y * v.y);
-      res[idx] = cscale(v, f);
+      res[l * ls + e] = cscale(v, f);
     }
     __syncthreads();
-    res = lds_fft<-DIR>(res, other, pl, nb, stw);
+    res = lds_fft<-DIR>(res, other, pl, nb, stw, ls);
     // block energy
-    __shared__ double sm[FFT_THREADS / 64];
+    __shared__ double sm[16];
     for (int off = 32; off > 0; off >>= 1) en += __shfl_xor(en, off, 64);
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = en;
     __syncthreads();
     if (threadIdx.x == 0) {
       double sacc = 0;
-      for (int w = 0; w < FFT_THREADS / 64; ++w) sacc += sm[w];
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) sacc += sm[w];
       atomicAdd((unsigned long long*)eslot, (unsigned long long)to_fix(sacc));
     }
   }
@@ -211,10 +265,19 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(FftPlan pl, FftLines l
     if (contiguous) { l = idx / K; e = idx - l * K; }
     else { e = idx / nb; l = idx - e * nb; }
     long long g = line_base(ln, l0 + l) + (long long)e * ln.estride;
-    float2 v = res[l * K + e];
+    float2 v = res[l * ls + e];
     if (MODE == 2) rout[g] = v.x;
     else cg[g] = v;
   }
+}
+
+template <int DIR, int MODE>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(FftPlan pl, FftLines ln, const float2* __restrict__ tw,
+                                                        float2* __restrict__ cg, long long* __restrict__ qfix,
+                                                        float* __restrict__ rout, const float* __restrict__ bc,
+                                                        long long* __restrict__ eslot) {
+  extern __shared__ float2 lds[];
+  fft_axis_body<DIR, MODE>(pl, ln, tw, cg, qfix, rout, bc, eslot, blockIdx.x * ln.B, lds);
 }
 
 // ---------------------------------------------------------------- spread / gather
@@ -416,21 +479,17 @@ __device__ __forceinline__ int reserve_slot(bool keep, int* counter) {
 // result is independent of the order of the atomics), then forward FFT along z and y; the
 // plane goes to cg[x0][y][z].  LDS: twiddles (Kz + Ky), two complex planes; the records
 // overlay the second plane and what follows it.
-__global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_fwd(PlaneArgs a, FftPlan plz, FftPlan ply,
-                                                                 const float2* __restrict__ twz_g,
-                                                                 const float2* __restrict__ twy_g,
-                                                                 const int* __restrict__ cell_start,
-                                                                 const double* __restrict__ pos_sorted,
-                                                                 const float4* __restrict__ xq,
-                                                                 float2* __restrict__ cg) {
-  extern __shared__ float2 lds[];
+__device__ void plane_fwd_body(const PlaneArgs& a, const FftPlan& plz, const FftPlan& ply,
+                               const float2* __restrict__ twz_g, const float2* __restrict__ twy_g,
+                               const int* __restrict__ cell_start, const double* __restrict__ pos_sorted,
+                               const float4* __restrict__ xq, float2* __restrict__ cg, int x0, float2* lds) {
   __shared__ int nrec;
-  const int x0 = blockIdx.x;
   const int P = a.Ky * a.Kz;
+  const bool p64 = a.Ky == 64 && a.Kz == 64;  // wave-level transforms on a padded plane
   float2* twz = lds;
   float2* twy = lds + a.Kz;
   float2* A = twy + a.Ky;
-  float2* Bf = A + P;
+  float2* Bf = A + (p64 ? 64 * PLANE64_LS : P);
   long long* plane = (long long*)A;
   SpreadRec* rec = (SpreadRec*)Bf;
   load_twiddles(twz, twz_g, a.Kz);
@@ -502,6 +561,18 @@ __global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_fwd(PlaneArgs a, Ff
     __syncthreads();
   }
   __syncthreads();
+  if (p64) {
+    float2* out = cg + (long long)x0 * P;
+    for (int i = threadIdx.x; i < P; i += blockDim.x)
+      Bf[(i >> 6) * PLANE64_LS + (i & 63)] = make_float2((float)((double)plane[i] * kFixInv), 0.f);
+    __syncthreads();
+    plane64_lines<-1>(Bf, A, true, twz);   // z lines (rows) -> A
+    __syncthreads();
+    plane64_lines<-1>(A, Bf, false, twy);  // y lines (columns) -> Bf
+    __syncthreads();
+    for (int i = threadIdx.x; i < P; i += blockDim.x) out[i] = Bf[(i >> 6) * PLANE64_LS + (i & 63)];
+    return;
+  }
   // in place: lane i reads int64 element i and writes complex element i (same 8 bytes)
   for (int i = threadIdx.x; i < P; i += blockDim.x) A[i] = make_float2((float)((double)plane[i] * kFixInv), 0.f);
   __syncthreads();
@@ -512,36 +583,58 @@ __global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_fwd(PlaneArgs a, Ff
   for (int i = threadIdx.x; i < P; i += blockDim.x) out[i] = r2[i];
 }
 
+__global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_fwd(PlaneArgs a, FftPlan plz, FftPlan ply,
+                                                                 const float2* __restrict__ twz_g,
+                                                                 const float2* __restrict__ twy_g,
+                                                                 const int* __restrict__ cell_start,
+                                                                 const double* __restrict__ pos_sorted,
+                                                                 const float4* __restrict__ xq,
+                                                                 float2* __restrict__ cg) {
+  extern __shared__ float2 lds[];
+  plane_fwd_body(a, plz, ply, twz_g, twy_g, cell_start, pos_sorted, xq, cg, blockIdx.x, lds);
+}
+
 // inverse FFT along y and z of plane x0, then for every atom touching the plane its share of
 // the reciprocal force, -q (dtheta_x theta_y theta_z, ...) . conv over this plane's 4x4 (y,z)
 // stencil, written to fpart[kx][3*atom + d] (each slot once per evaluation).
 // LDS: twiddles, real part of the convolved plane (floats), two complex planes that the
 // records overlay after the transforms.
-__global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_inv_gather(
-    PlaneArgs a, FftPlan plz, FftPlan ply, const float2* __restrict__ twz_g, const float2* __restrict__ twy_g,
-    const int* __restrict__ sorted_orig, const int* __restrict__ cell_start, const double* __restrict__ pos_sorted,
-    const float4* __restrict__ xq, const float2* __restrict__ cg, double* __restrict__ fpart, DevState* st) {
-  extern __shared__ float2 lds[];
+__device__ void plane_gather_body(const PlaneArgs& a, const FftPlan& plz, const FftPlan& ply,
+                                  const float2* __restrict__ twz_g, const float2* __restrict__ twy_g,
+                                  const int* __restrict__ sorted_orig, const int* __restrict__ cell_start,
+                                  const double* __restrict__ pos_sorted, const float4* __restrict__ xq,
+                                  const float2* __restrict__ cg, double* __restrict__ fpart, DevState* st, int x0,
+                                  float2* lds) {
   __shared__ int nrec;
   __shared__ int bcount;
-  const int x0 = blockIdx.x;
   const int P = a.Ky * a.Kz;
+  const bool p64 = a.Ky == 64 && a.Kz == 64;  // wave-level transforms on a padded plane
   float2* twz = lds;
   float2* twy = lds + a.Kz;
   float* convr = (float*)(twy + a.Ky);
   float2* A = (float2*)(convr + P + (P & 1));
-  float2* Bf = A + P;
+  float2* Bf = A + (p64 ? 64 * PLANE64_LS : P);
   GatherRec* rec = (GatherRec*)A;
   load_twiddles(twz, twz_g, a.Kz);
   load_twiddles(twy, twy_g, a.Ky);
   if (threadIdx.x == 0) bcount = 0;
   const float2* in = cg + (long long)x0 * P;
-  for (int i = threadIdx.x; i < P; i += blockDim.x) A[i] = in[i];
-  __syncthreads();
-  float2* r1 = lds_fft<1>(A, Bf, ply, a.Kz, twy, 1, a.Kz);        // y lines first (reverse order)
-  float2* o1 = (r1 == A) ? Bf : A;
-  float2* r2 = lds_fft<1>(r1, o1, plz, a.Ky, twz);                // z lines
-  for (int i = threadIdx.x; i < P; i += blockDim.x) convr[i] = r2[i].x;
+  if (p64) {
+    for (int i = threadIdx.x; i < P; i += blockDim.x) A[(i >> 6) * PLANE64_LS + (i & 63)] = in[i];
+    __syncthreads();
+    plane64_lines<1>(A, Bf, false, twy);  // y lines (columns) -> Bf
+    __syncthreads();
+    plane64_lines<1>(Bf, A, true, twz);   // z lines (rows) -> A
+    __syncthreads();
+    for (int i = threadIdx.x; i < P; i += blockDim.x) convr[i] = A[(i >> 6) * PLANE64_LS + (i & 63)].x;
+  } else {
+    for (int i = threadIdx.x; i < P; i += blockDim.x) A[i] = in[i];
+    __syncthreads();
+    float2* r1 = lds_fft<1>(A, Bf, ply, a.Kz, twy, 1, a.Kz);        // y lines first (reverse order)
+    float2* o1 = (r1 == A) ? Bf : A;
+    float2* r2 = lds_fft<1>(r1, o1, plz, a.Ky, twz);                // z lines
+    for (int i = threadIdx.x; i < P; i += blockDim.x) convr[i] = r2[i].x;
+  }
   int b0, e0, b1, e1;
   plane_ranges(a, cell_start, x0, &b0, &e0, &b1, &e1);
   const int len0 = e0 - b0, total = len0 + (e1 - b1);
@@ -651,14 +744,84 @@ __global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_inv_gather(
   }
 }
 
+__global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_inv_gather(
+    PlaneArgs a, FftPlan plz, FftPlan ply, const float2* __restrict__ twz_g, const float2* __restrict__ twy_g,
+    const int* __restrict__ sorted_orig, const int* __restrict__ cell_start, const double* __restrict__ pos_sorted,
+    const float4* __restrict__ xq, const float2* __restrict__ cg, double* __restrict__ fpart, DevState* st) {
+  extern __shared__ float2 lds[];
+  plane_gather_body(a, plz, ply, twz_g, twy_g, sorted_orig, cell_start, pos_sorted, xq, cg, fpart, st, blockIdx.x,
+                    lds);
+}
+
+// Grid-wide barrier of a kernel whose blocks are all resident at once (at most one block per
+// compute unit, grid no larger than the device). The last block to arrive resets the count and
+// advances the generation; the others wait for that with a bounded spin, so a launch whose
+// blocks cannot all be resident raises MDX_FLAG_SYNC and finishes instead of hanging. One
+// thread per block fences on each side (device-scope fences act on whole caches: a fence per
+// thread would repeat the same cache write-back a thousand times).
+__device__ void grid_barrier(DevState* st) {
+  __syncthreads();  // every thread's global stores issued and complete at workgroup scope
+  if (threadIdx.x == 0) {
+    __threadfence();  // release for the whole block: the cache write-back covers all its stores
+    const unsigned gen = atomicAdd(&st->bar_gen, 0u);
+    const unsigned arrived = atomicAdd(&st->bar_count, 1u);
+    if (arrived == gridDim.x - 1) {
+      atomicExch(&st->bar_count, 0u);
+      __threadfence();
+      atomicAdd(&st->bar_gen, 1u);
+    } else {
+      for (unsigned spin = 0; atomicAdd(&st->bar_gen, 0u) == gen; ++spin) {
+        if (spin > (1u << 18)) {
+          atomicOr(&st->flags, MDX_FLAG_SYNC);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __threadfence();  // acquire: invalidates this compute unit's caches for the whole block
+  }
+  __syncthreads();
+}
+
+// The whole reciprocal-space evaluation in one launch of Kx blocks: block x0 spreads and
+// transforms plane x0; after a grid barrier the blocks share the x-axis pass (pencils of the
+// Ky x Kz plane, influence function, energy); after a second barrier block x0 transforms plane
+// x0 back and gathers its forces. The blocks keep their compute units for the whole
+// evaluation, so kernels running concurrently on other streams cannot delay the later phases.
+__global__ __launch_bounds__(PLANE_THREADS) void k_pme_fused(
+    PlaneArgs a, FftPlan plz, FftPlan ply, FftPlan plx, FftLines lnx, const float2* __restrict__ twz_g,
+    const float2* __restrict__ twy_g, const float2* __restrict__ twx_g, const int* __restrict__ sorted_orig,
+    const int* __restrict__ cell_start, const double* __restrict__ pos_sorted, const float4* __restrict__ xq,
+    float2* __restrict__ cg, const float* __restrict__ bc, long long* __restrict__ eslot,
+    double* __restrict__ fpart, DevState* st, int w_fwd, int w_gather) {
+  extern __shared__ float2 lds[];
+  PlaneArgs af = a;
+  af.W = w_fwd;
+  plane_fwd_body(af, plz, ply, twz_g, twy_g, cell_start, pos_sorted, xq, cg, blockIdx.x, lds);
+  grid_barrier(st);
+  const int nchunks = (lnx.nlines + lnx.B - 1) / lnx.B;
+  for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    fft_axis_body<-1, 3>(plx, lnx, twx_g, cg, nullptr, nullptr, bc, eslot, ch * lnx.B, lds);
+    __syncthreads();
+  }
+  grid_barrier(st);
+  PlaneArgs ag = a;
+  ag.W = w_gather;
+  plane_gather_body(ag, plz, ply, twz_g, twy_g, sorted_orig, cell_start, pos_sorted, xq, cg, fpart, st, blockIdx.x,
+                    lds);
+}
+
 // LDS bytes of the two plane kernels for a record window of W candidates
+// (64 x 64 planes are held padded to 64 x PLANE64_LS complex values)
 static inline size_t plane_fwd_lds(int Ky, int Kz, int W) {
   const size_t P = (size_t)Ky * Kz;
-  return (size_t)(Ky + Kz) * 8 + std::max(2 * P * 8, P * 8 + (size_t)W * sizeof(SpreadRec));
+  const size_t Pp = (Ky == 64 && Kz == 64) ? 64 * PLANE64_LS : P;
+  return (size_t)(Ky + Kz) * 8 + std::max(2 * Pp * 8, Pp * 8 + (size_t)W * sizeof(SpreadRec));
 }
 static inline size_t plane_gather_lds(int Ky, int Kz, int W) {
   const size_t P = (size_t)Ky * Kz;
-  return (size_t)(Ky + Kz) * 8 + (P + (P & 1)) * 4 + std::max(2 * P * 8, (size_t)W * sizeof(GatherRec));
+  const size_t Pp = (Ky == 64 && Kz == 64) ? 64 * PLANE64_LS : P;
+  return (size_t)(Ky + Kz) * 8 + (P + (P & 1)) * 4 + std::max(2 * Pp * 8, (size_t)W * sizeof(GatherRec));
 }
 
 // ---------------------------------------------------------------- host
@@ -763,8 +926,31 @@ extern "C" int mdx_pme_config(mdx_ctx* c, const int* h_grid, int order, double b
     MDX_HIP(hipFuncSetAttribute((const void*)k_pme_plane_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)budget));
     MDX_HIP(hipFuncSetAttribute((const void*)k_pme_plane_inv_gather, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)budget));
+    MDX_HIP(hipFuncSetAttribute((const void*)k_pme_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)budget));
+  }
+  // one-launch kernel: Kx blocks that must all be resident together (one per compute unit);
+  // the x-axis pass takes ceil(Ky Kz / Kx) pencils per block if the LDS allows
+  int ncu = 0;
+  MDX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+  c->pme_fused_B = 0;
+  if (c->pme_planes && c->K[0] <= ncu) {
+    const int nl = c->K[1] * c->K[2];
+    int B = (nl + c->K[0] - 1) / c->K[0];
+    const size_t ls = c->K[0] + 1;  // strided lines: padded LDS line stride
+    while (B > 1 && (2 * (size_t)B * ls + c->K[0]) * sizeof(float2) > budget) B = (B + 1) / 2;
+    const size_t lds_x = (2 * (size_t)B * ls + c->K[0]) * sizeof(float2);
+    c->pme_fused_B = lds_x <= budget ? B : 0;
+    c->pme_fused_lds = std::max(lds_x, std::max(plane_fwd_lds(c->K[1], c->K[2], c->pme_wf),
+                                                plane_gather_lds(c->K[1], c->K[2], c->pme_wg)));
   }
   c->pme_configured = true;
+  return 0;
+}
+
+extern "C" int mdx_pme_set_fused(mdx_ctx* c, int enable) {
+  MDX_CHECK(c, MDX_ERR_INVALID, "mdx_pme_set_fused: null context");
+  c->pme_fused_on = enable != 0;
+  mdx_invalidate_graph(c);
   return 0;
 }
 
@@ -799,7 +985,8 @@ static int fft_pass(mdx_ctx* c, int axis, int dir, int mode) {
     ln.B = std::min(16, Kz);
   }
   const float2* tw = c->twiddle.p + (size_t)axis * Kmax;
-  size_t lds = (2 * (size_t)ln.B + 1) * pl.K * sizeof(float2);
+  const size_t ls = ln.estride == 1 ? pl.K : pl.K + 1;
+  size_t lds = (2 * (size_t)ln.B * ls + pl.K) * sizeof(float2);
   dim3 grid(div_up(ln.nlines, ln.B));
 #define FFT_L(D, M)                                                                                    \
   k_fft_axis<D, M><<<grid, FFT_THREADS, lds, c->stream>>>(pl, ln, tw, c->cgrid.p, c->qfix.p, c->conv.p, c->bc.p, \
@@ -841,6 +1028,23 @@ static int pme_enqueue_planes(mdx_ctx* c) {
   const int Kmax = std::max(Kx, std::max(Ky, Kz));
   const float2* twy = c->twiddle.p + (size_t)Kmax;
   const float2* twz = c->twiddle.p + 2 * (size_t)Kmax;
+  if (c->pme_fused_B > 0 && c->pme_fused_on && !c->in_batch) {
+    FftPlan plx;
+    factorize(Kx, &plx);
+    FftLines ln;
+    ln.nlines = Ky * Kz;
+    ln.inner = ln.nlines;
+    ln.inner_stride = 1;
+    ln.outer_stride = 0;
+    ln.estride = Ky * Kz;
+    ln.B = c->pme_fused_B;
+    a.W = 0;
+    k_pme_fused<<<Kx, PLANE_THREADS, c->pme_fused_lds, c->stream>>>(
+        a, plz, ply, plx, ln, twz, twy, c->twiddle.p, c->sorted_orig.p, c->cell_start.p, c->pos_sorted.p, c->xq.p,
+        c->cgrid.p, c->bc.p, c->efix.p + 2, c->fpart.p, c->dstate.p, c->pme_wf, c->pme_wg);
+    MDX_LAUNCH_CHECK();
+    return 0;
+  }
   a.W = c->pme_wf;
   k_pme_plane_fwd<<<Kx, PLANE_THREADS, plane_fwd_lds(Ky, Kz, a.W), c->stream>>>(
       a, plz, ply, twz, twy, c->cell_start.p, c->pos_sorted.p, c->xq.p, c->cgrid.p);

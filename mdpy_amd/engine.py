@@ -114,6 +114,10 @@ class Engine:
         L.call("mdx_pme_config", self.ctx, (ctypes.c_int * 3)(*self.pme_grid), int(order), self.beta)
 
     # ------------------------------------------------------------------ setup
+    def set_pme_fused(self, enable=True):
+        """Reciprocal space in one launch or in three (the default); the forces are the same."""
+        L.call("mdx_pme_set_fused", self.ctx, 1 if enable else 0)
+
     def _bind(self):
         box = (ctypes.c_double * 3)(*self.box.tolist())
         L.call("mdx_system_set", self.ctx, self.n, ctypes.c_void_p(self.pos.data_ptr()),

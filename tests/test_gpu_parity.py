@@ -219,6 +219,40 @@ def test_forces_deterministic(water23328):
     F2, E2 = e.compute(127)
     assert np.array_equal(F1, F2.cpu().numpy())
     assert E1 == E2
+    assert e.status() == 0
+    e.close()
+
+
+@pytest.mark.parametrize("name", ["water648", "water23328", "chain"])
+def test_pme_one_launch_equals_three_launches(name, request):
+    """The one-launch reciprocal kernel (grid barrier between its phases) and the three-kernel
+    path run the same arithmetic per plane and per pencil: forces bitwise equal, energies equal
+    up to the order of the per-block energy sums; no status flag (coverage, barrier timeout)."""
+    from mdpy_amd import _lib as L
+    from mdpy_amd.engine import Engine
+    s = request.getfixturevalue(name)
+    say("case: PME one launch vs three,", name)
+    e = Engine(s, skin=1.5)
+    e.set_positions(s.positions)
+    out = {}
+    for fused in (False, True, False, True):
+        e.set_pme_fused(fused)
+        F, en = e.compute(L.TERM_COUL_RECIP)
+        out.setdefault(fused, []).append((F.cpu().numpy().copy(), en["coulomb_reciprocal"]))
+        assert e.status() == 0, fused
+    (Fa, Ea), (Fb, Eb) = out[False][0], out[True][0]
+    assert np.array_equal(Fa, Fb)
+    # each block's energy sum is rounded once to the 2^-40 fixed-point quantum: at most one
+    # quantum per block of either launch shape (256 x-pass blocks at 64^3)
+    assert abs(Ea - Eb) <= 256 * 2.0 ** -40
+    for k in (False, True):  # repeated launches: the barrier re-arms itself
+        assert np.array_equal(out[k][0][0], out[k][1][0]) and out[k][0][1] == out[k][1][1]
+    # inside the integrator's graph, many replays
+    e.set_pme_fused(True)
+    e.set_velocities(np.zeros((e.n, 3)))
+    e.configure_integrator(L.INT_LANGEVIN, dt=1.0, temperature=300.0, friction=1e-2, seed=1)
+    e.integrate(L.TERM_LJ | L.TERM_PME | L.TERM_BOND | L.TERM_ANGLE, 200, True, 50)
+    assert e.status() == 0
     e.close()
 
 
