@@ -557,7 +557,7 @@ extern "C" int mdx_ctx_destroy(mdx_ctx* c) {
   c->d_stencil.release(); c->qfix.release(); c->cgrid.release(); c->conv.release(); c->bc.release();
   c->twiddle.release(); c->excl_pairs.release(); c->xa.release(); c->xb.release(); c->va.release();
   c->vb.release(); c->acc_prev.release(); c->acc_cur.release(); c->ehist.release(); c->tmp64.release();
-  c->pos_sorted.release(); c->fpart.release();
+  c->pos_sorted.release(); c->fpart.release(); c->pme_grec.release(); c->pme_grec_n.release();
   if (c->owns_stream) hipStreamDestroy(c->stream);
   if (c->side_stream) hipStreamDestroy(c->side_stream);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);

@@ -185,7 +185,10 @@ struct mdx_ctx {
   int n_excl_pairs = 0;
   DevBuf<double> fpart;          // plane path: reciprocal force per (kx, atom), 4 x 3N
   bool pme_planes = false;       // plane kernels usable (list grid present, plane fits LDS)
-  int pme_wf = 0, pme_wg = 0;    // candidate windows of the two plane kernels
+  int pme_wf = 0;                // candidate window of the forward plane kernel
+  int pme_rec_cap = 0;           // gather records per plane
+  DevBuf<int> pme_grec;          // gather records (6 words each), pme_rec_cap per plane
+  DevBuf<int> pme_grec_n;        // records per plane
   int pme_fused_B = 0;           // x-axis lines per chunk of the one-launch PME kernel (0: not usable)
   size_t pme_fused_lds = 0;      // its dynamic LDS
   bool pme_fused_on = false;     // one-launch PME kernel selected (never inside a batch)

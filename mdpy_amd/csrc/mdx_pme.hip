@@ -175,13 +175,13 @@ __device__ __forceinline__ void fft64_pass(const float2* X, float2* Y, int off, 
   for (int r = 0; r < 4; ++r) Y[off + (j + r * p) * es] = u[r];
 }
 
-// 64-point transforms of the 64 rows (rows = true: elements contiguous) or the 64 columns of a
-// padded plane in a, with b as the ping-pong buffer; the result ends in b. Wave w takes lines
-// w, w + 16, w + 32, w + 48 (1024 threads).
+// 64-point transforms of the 4 nw rows (rows = true: elements contiguous) or columns of a
+// padded plane in a (nw = waves of the block), with b as the ping-pong buffer; the result ends in
+// b. Wave w takes lines w, w + nw, w + 2 nw, w + 3 nw.
 template <int DIR>
 __device__ __forceinline__ void plane64_lines(float2* a, float2* b, bool rows, const float2* __restrict__ tw) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int line = w + 16 * (lane >> 4), i = lane & 15;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int line = w + nw * (lane >> 4), i = lane & 15;
   const int off = rows ? line * PLANE64_LS : line;
   const int es = rows ? 1 : PLANE64_LS;
   fft64_pass<DIR, 0>(a, b, off, es, i, tw);
@@ -234,7 +234,16 @@ __device__ void fft_axis_body(const FftPlan& pl, const FftLines& ln, const float
     A[l * ls + e] = v;
   }
   __syncthreads();
-  float2* res = lds_fft<DIR>(A, Bf, pl, nb, stw, ls);
+  // 64-point strided lines, four per wave: wave-level passes (padded stride 65)
+  const bool w64 = K == 64 && !contiguous && nb == B && B == 4 * (int)(blockDim.x >> 6);
+  float2* res;
+  if (w64) {
+    plane64_lines<DIR>(A, Bf, true, stw);
+    res = Bf;
+    __syncthreads();
+  } else {
+    res = lds_fft<DIR>(A, Bf, pl, nb, stw, ls);
+  }
   if (MODE == 3) {
     float2* other = (res == A) ? Bf : A;
     double en = 0.0;
@@ -247,7 +256,13 @@ __device__ void fft_axis_body(const FftPlan& pl, const FftLines& ln, const float
       res[l * ls + e] = cscale(v, f);
     }
     __syncthreads();
-    res = lds_fft<-DIR>(res, other, pl, nb, stw, ls);
+    if (w64) {
+      plane64_lines<-DIR>(res, other, true, stw);
+      res = other;
+      __syncthreads();
+    } else {
+      res = lds_fft<-DIR>(res, other, pl, nb, stw, ls);
+    }
     // block energy
     __shared__ double sm[16];
     for (int off = 32; off > 0; off >>= 1) en += __shfl_xor(en, off, 64);
@@ -407,17 +422,19 @@ struct PlaneArgs {
   double wx;          // its cell width along x
   double margin;      // list skin/2 plus rounding slack
   int W;              // candidates examined per round (bounded by the LDS record space)
+  int rec_cap;        // gather records per plane in global memory
 };
 
 struct SpreadRec {  // 16 B
   int yz;           // by | bz << 12
   float wy, wz, qx; // fractional offsets, q sqrt(ke) theta_kx(wx)
 };
-struct GatherRec {  // 24 B
+struct GatherRec {  // 24 B (stored in an int buffer, 6 words per record)
   int atom;
   int yzk;          // by | bz << 12 | kx << 24
   float wx, wy, wz, q;
 };
+static_assert(sizeof(GatherRec) == 6 * sizeof(int), "gather record layout");
 
 // weight k of the order-4 cardinal B-spline at offset w (grid point base - k), and its
 // derivative; branch-free so lanes with different k do not diverge
@@ -481,9 +498,12 @@ __device__ __forceinline__ int reserve_slot(bool keep, int* counter) {
 // overlay the second plane and what follows it.
 __device__ void plane_fwd_body(const PlaneArgs& a, const FftPlan& plz, const FftPlan& ply,
                                const float2* __restrict__ twz_g, const float2* __restrict__ twy_g,
-                               const int* __restrict__ cell_start, const double* __restrict__ pos_sorted,
-                               const float4* __restrict__ xq, float2* __restrict__ cg, int x0, float2* lds) {
+                               const int* __restrict__ sorted_orig, const int* __restrict__ cell_start,
+                               const double* __restrict__ pos_sorted, const float4* __restrict__ xq,
+                               float2* __restrict__ cg, GatherRec* __restrict__ grec, int* __restrict__ grec_n,
+                               DevState* st, int x0, float2* lds) {
   __shared__ int nrec;
+  __shared__ int ngrec;
   const int P = a.Ky * a.Kz;
   const bool p64 = a.Ky == 64 && a.Kz == 64;  // wave-level transforms on a padded plane
   float2* twz = lds;
@@ -495,6 +515,8 @@ __device__ void plane_fwd_body(const PlaneArgs& a, const FftPlan& plz, const Fft
   load_twiddles(twz, twz_g, a.Kz);
   load_twiddles(twy, twy_g, a.Ky);
   for (int i = threadIdx.x; i < P; i += blockDim.x) plane[i] = 0;
+  if (threadIdx.x == 0) ngrec = 0;
+  GatherRec* gout = grec + (long long)x0 * a.rec_cap;
   int b0, e0, b1, e1;
   plane_ranges(a, cell_start, x0, &b0, &e0, &b1, &e1);
   const int len0 = e0 - b0, total = len0 + (e1 - b1);
@@ -514,8 +536,9 @@ __device__ void plane_fwd_body(const PlaneArgs& a, const FftPlan& plz, const Fft
       }
 #pragma unroll
       for (int u = 0; u < PLANE_U; ++u) {
-        bool keep = false;
+        bool keep = false, touch = false;
         SpreadRec r;
+        GatherRec g;
         if (sl[u] >= 0) {
           int bx;
           float wx;
@@ -531,10 +554,21 @@ __device__ void plane_fwd_body(const PlaneArgs& a, const FftPlan& plz, const Fft
             r.yz = by | (bz << 12);
             r.qx = q * bsp4(wx, kx);
             keep = q != 0.f;
+            // the same atom's record for the gather of this plane (zero charges included: the
+            // gather writes their slot, so every atom owns exactly four)
+            g.atom = sorted_orig[s];
+            g.yzk = by | (bz << 12) | (kx << 24);
+            g.wx = wx;
+            g.wy = r.wy;
+            g.wz = r.wz;
+            g.q = q;
+            touch = true;
           }
         }
         const int slot = reserve_slot(keep, &nrec);
         if (keep) rec[slot] = r;
+        const int gslot = reserve_slot(touch, &ngrec);
+        if (touch && gslot < a.rec_cap) gout[gslot] = g;
       }
     }
     __syncthreads();
@@ -561,6 +595,10 @@ __device__ void plane_fwd_body(const PlaneArgs& a, const FftPlan& plz, const Fft
     __syncthreads();
   }
   __syncthreads();
+  if (threadIdx.x == 0) {
+    grec_n[x0] = min(ngrec, a.rec_cap);
+    if (ngrec > a.rec_cap) atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
+  }
   if (p64) {
     float2* out = cg + (long long)x0 * P;
     for (int i = threadIdx.x; i < P; i += blockDim.x)
@@ -583,15 +621,14 @@ __device__ void plane_fwd_body(const PlaneArgs& a, const FftPlan& plz, const Fft
   for (int i = threadIdx.x; i < P; i += blockDim.x) out[i] = r2[i];
 }
 
-__global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_fwd(PlaneArgs a, FftPlan plz, FftPlan ply,
-                                                                 const float2* __restrict__ twz_g,
-                                                                 const float2* __restrict__ twy_g,
-                                                                 const int* __restrict__ cell_start,
-                                                                 const double* __restrict__ pos_sorted,
-                                                                 const float4* __restrict__ xq,
-                                                                 float2* __restrict__ cg) {
+__global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_fwd(
+    PlaneArgs a, FftPlan plz, FftPlan ply, const float2* __restrict__ twz_g, const float2* __restrict__ twy_g,
+    const int* __restrict__ sorted_orig, const int* __restrict__ cell_start, const double* __restrict__ pos_sorted,
+    const float4* __restrict__ xq, float2* __restrict__ cg, GatherRec* __restrict__ grec, int* __restrict__ grec_n,
+    DevState* st) {
   extern __shared__ float2 lds[];
-  plane_fwd_body(a, plz, ply, twz_g, twy_g, cell_start, pos_sorted, xq, cg, blockIdx.x, lds);
+  plane_fwd_body(a, plz, ply, twz_g, twy_g, sorted_orig, cell_start, pos_sorted, xq, cg, grec, grec_n, st,
+                 blockIdx.x, lds);
 }
 
 // inverse FFT along y and z of plane x0, then for every atom touching the plane its share of
@@ -603,9 +640,9 @@ __device__ void plane_gather_body(const PlaneArgs& a, const FftPlan& plz, const 
                                   const float2* __restrict__ twz_g, const float2* __restrict__ twy_g,
                                   const int* __restrict__ sorted_orig, const int* __restrict__ cell_start,
                                   const double* __restrict__ pos_sorted, const float4* __restrict__ xq,
-                                  const float2* __restrict__ cg, double* __restrict__ fpart, DevState* st, int x0,
+                                  const float2* __restrict__ cg, const GatherRec* __restrict__ grec,
+                                  const int* __restrict__ grec_n, double* __restrict__ fpart, DevState* st, int x0,
                                   float2* lds) {
-  __shared__ int nrec;
   __shared__ int bcount;
   const int P = a.Ky * a.Kz;
   const bool p64 = a.Ky == 64 && a.Kz == 64;  // wave-level transforms on a padded plane
@@ -614,10 +651,8 @@ __device__ void plane_gather_body(const PlaneArgs& a, const FftPlan& plz, const 
   float* convr = (float*)(twy + a.Ky);
   float2* A = (float2*)(convr + P + (P & 1));
   float2* Bf = A + (p64 ? 64 * PLANE64_LS : P);
-  GatherRec* rec = (GatherRec*)A;
   load_twiddles(twz, twz_g, a.Kz);
   load_twiddles(twy, twy_g, a.Ky);
-  if (threadIdx.x == 0) bcount = 0;
   const float2* in = cg + (long long)x0 * P;
   if (p64) {
     for (int i = threadIdx.x; i < P; i += blockDim.x) A[(i >> 6) * PLANE64_LS + (i & 63)] = in[i];
@@ -635,98 +670,57 @@ __device__ void plane_gather_body(const PlaneArgs& a, const FftPlan& plz, const 
     float2* r2 = lds_fft<1>(r1, o1, plz, a.Ky, twz);                // z lines
     for (int i = threadIdx.x; i < P; i += blockDim.x) convr[i] = r2[i].x;
   }
-  int b0, e0, b1, e1;
-  plane_ranges(a, cell_start, x0, &b0, &e0, &b1, &e1);
-  const int len0 = e0 - b0, total = len0 + (e1 - b1);
   const long long n3 = 3LL * a.n;
-  for (int w0 = 0; w0 < total; w0 += a.W) {
-    if (threadIdx.x == 0) nrec = 0;
-    __syncthreads();  // also orders convr writes and the end of the transforms before the records
-    const int wend = min(total, w0 + a.W);
-    for (int c0 = w0; c0 < wend; c0 += PLANE_U * blockDim.x) {
-      int sl[PLANE_U];
-      double px[PLANE_U];
+  // the records of the atoms touching this plane, written by the forward kernel
+  const int nr = grec_n[x0];
+  const GatherRec* gin = grec + (long long)x0 * a.rec_cap;
+  __syncthreads();  // convr complete
+  // four lanes per record (one per ky, 4 z points each); nr * 4 and blockDim.x are multiples
+  // of 4, so the four lanes of a record stay together through the reduction. Zero charges give
+  // zero forces through the same arithmetic (their slot is still written).
+  for (int e = threadIdx.x; e < nr * 4; e += blockDim.x) {
+    const GatherRec r = gin[e >> 2];
+    const int ky = e & 3;
+    int gy = (r.yzk & 4095) - ky;
+    gy += gy < 0 ? a.Ky : 0;
+    const float ty = bsp4(r.wy, ky), dy = dbsp4(r.wy, ky);
+    const float w = r.wz, w2 = w * w, w3 = w2 * w, om = 1.f - w;
+    const float tz[4] = {w3 * (1.f / 6.f), (-3.f * w3 + 3.f * w2 + 3.f * w + 1.f) * (1.f / 6.f),
+                         (3.f * w3 - 6.f * w2 + 4.f) * (1.f / 6.f), om * om * om * (1.f / 6.f)};
+    const float dz[4] = {0.5f * w2, 0.5f * (-3.f * w2 + 2.f * w + 1.f), 0.5f * (3.f * w2 - 4.f * w),
+                         -0.5f * om * om};
+    const int bz = (r.yzk >> 12) & 4095;
+    const float* row = convr + gy * a.Kz;
+    float c = 0.f, cz = 0.f;
 #pragma unroll
-      for (int u = 0; u < PLANE_U; ++u) {
-        const int ci = c0 + u * blockDim.x + threadIdx.x;
-        sl[u] = ci < wend ? (ci < len0 ? b0 + ci : b1 + (ci - len0)) : -1;
-        px[u] = sl[u] >= 0 ? pos_sorted[3 * sl[u]] : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < PLANE_U; ++u) {
-        bool keep = false;
-        GatherRec r;
-        if (sl[u] >= 0) {
-          int bx;
-          frac_coord(px[u], a.iLx, a.Kx, &bx, &r.wx);
-          int kx = bx - x0;
-          kx += kx < 0 ? a.Kx : 0;
-          if (kx <= 3) {
-            const int s = sl[u];
-            r.q = xq[s].w;
-            r.atom = sorted_orig[s];
-            if (r.q == 0.f) {
-              double* dst = fpart + kx * n3 + 3LL * r.atom;
-              dst[0] = dst[1] = dst[2] = 0.0;
-              atomicAdd(&bcount, 1);
-            } else {
-              int by, bz;
-              frac_coord(pos_sorted[3 * s + 1], a.iLy, a.Ky, &by, &r.wy);
-              frac_coord(pos_sorted[3 * s + 2], a.iLz, a.Kz, &bz, &r.wz);
-              r.yzk = by | (bz << 12) | (kx << 24);
-              keep = true;
-            }
-          }
-        }
-        const int slot = reserve_slot(keep, &nrec);
-        if (keep) rec[slot] = r;
-      }
+    for (int kz = 0; kz < 4; ++kz) {
+      int gz = bz - kz;
+      gz += gz < 0 ? a.Kz : 0;
+      const float cv = row[gz];
+      c += tz[kz] * cv;
+      cz += dz[kz] * cv;
     }
-    __syncthreads();
-    const int nr = nrec;
-    // four lanes per record (one per ky, 4 z points each); nr * 4 and blockDim.x are multiples
-    // of 4, so the four lanes of a record stay together through the reduction
-    for (int e = threadIdx.x; e < nr * 4; e += blockDim.x) {
-      const GatherRec r = rec[e >> 2];
-      const int ky = e & 3;
-      int gy = (r.yzk & 4095) - ky;
-      gy += gy < 0 ? a.Ky : 0;
-      const float ty = bsp4(r.wy, ky), dy = dbsp4(r.wy, ky);
-      const float w = r.wz, w2 = w * w, w3 = w2 * w, om = 1.f - w;
-      const float tz[4] = {w3 * (1.f / 6.f), (-3.f * w3 + 3.f * w2 + 3.f * w + 1.f) * (1.f / 6.f),
-                           (3.f * w3 - 6.f * w2 + 4.f) * (1.f / 6.f), om * om * om * (1.f / 6.f)};
-      const float dz[4] = {0.5f * w2, 0.5f * (-3.f * w2 + 2.f * w + 1.f), 0.5f * (3.f * w2 - 4.f * w),
-                           -0.5f * om * om};
-      const int bz = (r.yzk >> 12) & 4095;
-      const float* row = convr + gy * a.Kz;
-      float c = 0.f, cz = 0.f;
+    float sx = ty * c, sy = dy * c, sz = ty * cz;
 #pragma unroll
-      for (int kz = 0; kz < 4; ++kz) {
-        int gz = bz - kz;
-        gz += gz < 0 ? a.Kz : 0;
-        const float cv = row[gz];
-        c += tz[kz] * cv;
-        cz += dz[kz] * cv;
-      }
-      float sx = ty * c, sy = dy * c, sz = ty * cz;
-#pragma unroll
-      for (int off = 2; off > 0; off >>= 1) {
-        sx += __shfl_xor(sx, off, 64);
-        sy += __shfl_xor(sy, off, 64);
-        sz += __shfl_xor(sz, off, 64);
-      }
-      if (ky == 0) {
-        const int kx = r.yzk >> 24;
-        const float tx = bsp4(r.wx, kx), dx = dbsp4(r.wx, kx);
-        double* dst = fpart + kx * n3 + 3LL * r.atom;
+    for (int off = 2; off > 0; off >>= 1) {
+      sx += __shfl_xor(sx, off, 64);
+      sy += __shfl_xor(sy, off, 64);
+      sz += __shfl_xor(sz, off, 64);
+    }
+    if (ky == 0) {
+      const int kx = r.yzk >> 24;
+      const float tx = bsp4(r.wx, kx), dx = dbsp4(r.wx, kx);
+      double* dst = fpart + kx * n3 + 3LL * r.atom;
+      if (r.q == 0.f) {
+        dst[0] = dst[1] = dst[2] = 0.0;
+      } else {
         dst[0] = -(double)r.q * (double)(dx * sx) * (a.Kx / a.Lx);
         dst[1] = -(double)r.q * (double)(tx * sy) * (a.Ky / a.Ly);
         dst[2] = -(double)r.q * (double)(tx * sz) * (a.Kz / a.Lz);
       }
     }
-    if (threadIdx.x == 0) bcount += nr;
-    __syncthreads();
   }
+  if (threadIdx.x == 0) bcount = nr;
   // every atom owns exactly one slot per kx: the grid-wide count must be 4 n. The last block
   // to finish checks it and re-arms the counters for the next evaluation.
   __syncthreads();
@@ -747,10 +741,11 @@ __device__ void plane_gather_body(const PlaneArgs& a, const FftPlan& plz, const 
 __global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_inv_gather(
     PlaneArgs a, FftPlan plz, FftPlan ply, const float2* __restrict__ twz_g, const float2* __restrict__ twy_g,
     const int* __restrict__ sorted_orig, const int* __restrict__ cell_start, const double* __restrict__ pos_sorted,
-    const float4* __restrict__ xq, const float2* __restrict__ cg, double* __restrict__ fpart, DevState* st) {
+    const float4* __restrict__ xq, const float2* __restrict__ cg, const GatherRec* __restrict__ grec,
+    const int* __restrict__ grec_n, double* __restrict__ fpart, DevState* st) {
   extern __shared__ float2 lds[];
-  plane_gather_body(a, plz, ply, twz_g, twy_g, sorted_orig, cell_start, pos_sorted, xq, cg, fpart, st, blockIdx.x,
-                    lds);
+  plane_gather_body(a, plz, ply, twz_g, twy_g, sorted_orig, cell_start, pos_sorted, xq, cg, grec, grec_n, fpart, st,
+                    blockIdx.x, lds);
 }
 
 // Grid-wide barrier of a kernel whose blocks are all resident at once (at most one block per
@@ -793,11 +788,10 @@ __global__ __launch_bounds__(PLANE_THREADS) void k_pme_fused(
     const float2* __restrict__ twy_g, const float2* __restrict__ twx_g, const int* __restrict__ sorted_orig,
     const int* __restrict__ cell_start, const double* __restrict__ pos_sorted, const float4* __restrict__ xq,
     float2* __restrict__ cg, const float* __restrict__ bc, long long* __restrict__ eslot,
-    double* __restrict__ fpart, DevState* st, int w_fwd, int w_gather) {
+    GatherRec* __restrict__ grec, int* __restrict__ grec_n, double* __restrict__ fpart, DevState* st) {
   extern __shared__ float2 lds[];
-  PlaneArgs af = a;
-  af.W = w_fwd;
-  plane_fwd_body(af, plz, ply, twz_g, twy_g, cell_start, pos_sorted, xq, cg, blockIdx.x, lds);
+  plane_fwd_body(a, plz, ply, twz_g, twy_g, sorted_orig, cell_start, pos_sorted, xq, cg, grec, grec_n, st,
+                 blockIdx.x, lds);
   grid_barrier(st);
   const int nchunks = (lnx.nlines + lnx.B - 1) / lnx.B;
   for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
@@ -805,23 +799,21 @@ __global__ __launch_bounds__(PLANE_THREADS) void k_pme_fused(
     __syncthreads();
   }
   grid_barrier(st);
-  PlaneArgs ag = a;
-  ag.W = w_gather;
-  plane_gather_body(ag, plz, ply, twz_g, twy_g, sorted_orig, cell_start, pos_sorted, xq, cg, fpart, st, blockIdx.x,
-                    lds);
+  plane_gather_body(a, plz, ply, twz_g, twy_g, sorted_orig, cell_start, pos_sorted, xq, cg, grec, grec_n, fpart, st,
+                    blockIdx.x, lds);
 }
 
-// LDS bytes of the two plane kernels for a record window of W candidates
+// LDS bytes of the two plane kernels (the forward one for a record window of W candidates)
 // (64 x 64 planes are held padded to 64 x PLANE64_LS complex values)
 static inline size_t plane_fwd_lds(int Ky, int Kz, int W) {
   const size_t P = (size_t)Ky * Kz;
   const size_t Pp = (Ky == 64 && Kz == 64) ? 64 * PLANE64_LS : P;
   return (size_t)(Ky + Kz) * 8 + std::max(2 * Pp * 8, Pp * 8 + (size_t)W * sizeof(SpreadRec));
 }
-static inline size_t plane_gather_lds(int Ky, int Kz, int W) {
+static inline size_t plane_gather_lds(int Ky, int Kz) {
   const size_t P = (size_t)Ky * Kz;
   const size_t Pp = (Ky == 64 && Kz == 64) ? 64 * PLANE64_LS : P;
-  return (size_t)(Ky + Kz) * 8 + (P + (P & 1)) * 4 + std::max(2 * Pp * 8, (size_t)W * sizeof(GatherRec));
+  return (size_t)(Ky + Kz) * 8 + (P + (P & 1)) * 4 + 2 * Pp * 8;
 }
 
 // ---------------------------------------------------------------- host
@@ -916,12 +908,15 @@ extern "C" int mdx_pme_config(mdx_ctx* c, const int* h_grid, int order, double b
   int lds_max = 0;
   MDX_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
   const size_t budget = lds_max > 1024 ? (size_t)lds_max - 1024 : 0;  // static LDS of the kernels
-  c->pme_wf = c->pme_wg = 0;
+  c->pme_wf = 0;
   for (int W = 8192; W >= PLANE_THREADS; W -= PLANE_THREADS)
     if (!c->pme_wf && plane_fwd_lds(c->K[1], c->K[2], W) <= budget) c->pme_wf = W;
-  for (int W = 8192; W >= PLANE_THREADS; W -= PLANE_THREADS)
-    if (!c->pme_wg && plane_gather_lds(c->K[1], c->K[2], W) <= budget) c->pme_wg = W;
-  c->pme_planes = c->pme_wf > 0 && c->pme_wg > 0 && c->K[1] < 4096 && c->K[2] < 4096;
+  c->pme_planes = c->pme_wf > 0 && plane_gather_lds(c->K[1], c->K[2]) <= budget && c->K[1] < 4096 &&
+                  c->K[2] < 4096;
+  // gather records: each atom touches 4 of the Kx planes; room for twice the mean plus slack
+  c->pme_rec_cap = (int)std::min<long long>(c->n, 8LL * c->n / c->K[0] + 4096);
+  if ((rc = c->pme_grec.ensure((size_t)c->K[0] * c->pme_rec_cap * 6)) || (rc = c->pme_grec_n.ensure(c->K[0])))
+    return rc;
   if (c->pme_planes) {  // the whole budget, so contexts with other grids stay launchable
     MDX_HIP(hipFuncSetAttribute((const void*)k_pme_plane_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)budget));
     MDX_HIP(hipFuncSetAttribute((const void*)k_pme_plane_inv_gather, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -941,7 +936,7 @@ extern "C" int mdx_pme_config(mdx_ctx* c, const int* h_grid, int order, double b
     const size_t lds_x = (2 * (size_t)B * ls + c->K[0]) * sizeof(float2);
     c->pme_fused_B = lds_x <= budget ? B : 0;
     c->pme_fused_lds = std::max(lds_x, std::max(plane_fwd_lds(c->K[1], c->K[2], c->pme_wf),
-                                                plane_gather_lds(c->K[1], c->K[2], c->pme_wg)));
+                                                plane_gather_lds(c->K[1], c->K[2])));
   }
   c->pme_configured = true;
   return 0;
@@ -1022,6 +1017,9 @@ static int pme_enqueue_planes(mdx_ctx* c) {
   a.gnz = c->gn[2];
   a.wx = c->box[0] / c->gn[0];
   a.margin = 0.5 * c->skin + 1e-6 * c->box[0] + 1e-9;
+  a.rec_cap = c->pme_rec_cap;
+  MDX_CHECK(c->pme_grec.n >= (size_t)Kx * a.rec_cap * 6, MDX_ERR_INVALID, "PME configured for another system size");
+  GatherRec* grec = (GatherRec*)c->pme_grec.p;
   FftPlan plz, ply;
   factorize(Kz, &plz);
   factorize(Ky, &ply);
@@ -1038,23 +1036,23 @@ static int pme_enqueue_planes(mdx_ctx* c) {
     ln.outer_stride = 0;
     ln.estride = Ky * Kz;
     ln.B = c->pme_fused_B;
-    a.W = 0;
+    a.W = c->pme_wf;
     k_pme_fused<<<Kx, PLANE_THREADS, c->pme_fused_lds, c->stream>>>(
         a, plz, ply, plx, ln, twz, twy, c->twiddle.p, c->sorted_orig.p, c->cell_start.p, c->pos_sorted.p, c->xq.p,
-        c->cgrid.p, c->bc.p, c->efix.p + 2, c->fpart.p, c->dstate.p, c->pme_wf, c->pme_wg);
+        c->cgrid.p, c->bc.p, c->efix.p + 2, grec, c->pme_grec_n.p, c->fpart.p, c->dstate.p);
     MDX_LAUNCH_CHECK();
     return 0;
   }
   a.W = c->pme_wf;
   k_pme_plane_fwd<<<Kx, PLANE_THREADS, plane_fwd_lds(Ky, Kz, a.W), c->stream>>>(
-      a, plz, ply, twz, twy, c->cell_start.p, c->pos_sorted.p, c->xq.p, c->cgrid.p);
+      a, plz, ply, twz, twy, c->sorted_orig.p, c->cell_start.p, c->pos_sorted.p, c->xq.p, c->cgrid.p, grec,
+      c->pme_grec_n.p, c->dstate.p);
   MDX_LAUNCH_CHECK();
   int rc;
   if ((rc = fft_pass(c, 0, -1, 3))) return rc;  // x forward, * B C, energy, x inverse
-  a.W = c->pme_wg;
-  k_pme_plane_inv_gather<<<Kx, PLANE_THREADS, plane_gather_lds(Ky, Kz, a.W), c->stream>>>(
-      a, plz, ply, twz, twy, c->sorted_orig.p, c->cell_start.p, c->pos_sorted.p, c->xq.p, c->cgrid.p, c->fpart.p,
-      c->dstate.p);
+  k_pme_plane_inv_gather<<<Kx, PLANE_THREADS, plane_gather_lds(Ky, Kz), c->stream>>>(
+      a, plz, ply, twz, twy, c->sorted_orig.p, c->cell_start.p, c->pos_sorted.p, c->xq.p, c->cgrid.p, grec,
+      c->pme_grec_n.p, c->fpart.p, c->dstate.p);
   MDX_LAUNCH_CHECK();
   return 0;
 }
