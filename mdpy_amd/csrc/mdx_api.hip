@@ -118,7 +118,7 @@ __global__ void k_forces_out(const long long* __restrict__ ffix, const double* _
 }
 
 __device__ void block_add_energy(double e, long long* slot) {
-  __shared__ double sm[4];
+  __shared__ double sm[16];
   for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = e;
   __syncthreads();
@@ -261,7 +261,20 @@ struct StepPro {
   int* cell_count;
   int ncell;
   double thr2;
+  // binning by the last workgroup (bin != 0): every lane stores its atom's list-grid cell; on a
+  // rebuild step the last workgroup counts, scans and scatters the atoms into cell order
+  int bin;
+  int* atom_cell;
+  int* cell_start;
+  int* sorted_orig;
+  int gn[3];
+  double gw[3];
+  int n;
 };
+
+#define PRO_THREADS 512
+#define PRO_SORT_MAX 1024         // largest cell population the per-cell sort accepts (as k_cell_scan)
+#define PRO_BIN_LDS_MAX (60 * 1024)  // dynamic LDS for last-workgroup binning (2 ints per cell)
 
 __device__ void prologue_atom(const IntArgs& A, const StepPro& P, DevState* st, int i, const double* xn, double* d2,
                               bool* bad) {
@@ -274,11 +287,67 @@ __device__ void prologue_atom(const IntArgs& A, const StepPro& P, DevState* st, 
   }
   const int s = P.orig_sorted[i];
   pack_atom(xn[0], xn[1], xn[2], P.qs ? P.qs[i] : 0.f, A.L[0], A.L[1], A.L[2], &P.xq[s], &P.pos_sorted[3 * s]);
+  if (P.bin) {  // the list-grid cell, as k_grid_bin computes it
+    int c[3];
+    for (int d = 0; d < 3; ++d) {
+      double x = xn[d];
+      x -= A.L[d] * floor(x / A.L[d]);
+      const int k = (int)(x / P.gw[d]);
+      c[d] = k < 0 ? 0 : (k >= P.gn[d] ? P.gn[d] - 1 : k);
+    }
+    P.atom_cell[i] = (c[0] * P.gn[1] + c[1]) * P.gn[2] + c[2];
+  }
   (void)st;
 }
 
+// rebuild step, last workgroup: count the atoms per cell in LDS, exclusive scan into cell_start,
+// scatter the atom ids into their cells (the per-cell sort that follows makes the order
+// deterministic). LDS: 2 ncell ints.
+__device__ void last_block_bin(const StepPro& P, DevState* st) {
+  extern __shared__ int pro_lds[];
+  int* cnt = pro_lds;
+  int* start = pro_lds + P.ncell;
+  __shared__ int part[PRO_THREADS];
+  for (int k = threadIdx.x; k < P.ncell; k += blockDim.x) cnt[k] = 0;
+  __syncthreads();
+  for (int j = threadIdx.x; j < P.n; j += blockDim.x) atomicAdd(&cnt[P.atom_cell[j]], 1);
+  __syncthreads();
+  const int per = (P.ncell + blockDim.x - 1) / blockDim.x;
+  const int b = threadIdx.x * per, e = min(P.ncell, b + per);
+  int sum = 0;
+  for (int k = b; k < e; ++k) sum += cnt[k];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int t = 0; t < (int)blockDim.x; ++t) {
+      const int v = part[t];
+      part[t] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  int run = part[threadIdx.x];
+  bool over = false;
+  for (int k = b; k < e; ++k) {
+    const int v = cnt[k];
+    over |= v > PRO_SORT_MAX;
+    start[k] = run;
+    P.cell_start[k] = run;
+    P.cell_count[k] = v;
+    run += v;
+    cnt[k] = 0;  // becomes the scatter cursor
+  }
+  if (over) atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
+  __syncthreads();
+  for (int j = threadIdx.x; j < P.n; j += blockDim.x) {
+    const int c = P.atom_cell[j];
+    P.sorted_orig[start[c] + atomicAdd(&cnt[c], 1)] = j;
+  }
+}
+
 __device__ void prologue_block_end(const StepPro& P, DevState* st, double d2, bool bad) {
-  __shared__ double sm[4];
+  __shared__ double sm[16];
   __shared__ int last;
   if (bad) atomicOr(&st->flags, MDX_FLAG_NONFINITE);
   for (int off = 32; off > 0; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off, 64));
@@ -293,11 +362,15 @@ __device__ void prologue_block_end(const StepPro& P, DevState* st, double d2, bo
   }
   __syncthreads();
   if (!last) return;
-  __threadfence();
+  if (threadIdx.x == 0) __threadfence();  // acquire for the block (device-scope fences act on whole caches)
+  __syncthreads();
   const bool rebuild = atomicAdd(&st->rebuild, 0) != 0 ||
                        dec_double_order(atomicAdd(&st->max_disp2_bits, 0ull)) > P.thr2;
-  if (rebuild)
-    for (int k = threadIdx.x; k < P.ncell; k += blockDim.x) P.cell_count[k] = 0;
+  if (rebuild) {
+    if (P.bin) last_block_bin(P, st);
+    else
+      for (int k = threadIdx.x; k < P.ncell; k += blockDim.x) P.cell_count[k] = 0;
+  }
   if (threadIdx.x == 0) {
     if (P.record) {
       const long long k = st->step - st->hist_base;
@@ -311,43 +384,65 @@ __device__ void prologue_block_end(const StepPro& P, DevState* st, double d2, bo
   }
 }
 
-__global__ __launch_bounds__(256) void k_gjf_a_pro(IntArgs A, StepPro P, double* __restrict__ x,
-                                                   const double* __restrict__ v, const double* __restrict__ im,
-                                                   double* __restrict__ acc_prev, DevState* st) {
+// The step prologue: with FUSE_B it first completes the previous step (the velocity half of
+// G-JF or velocity Verlet with the forces of the last evaluation, plus the kinetic energy; the
+// same arithmetic as k_gjf_b / k_vv_b), then runs this step's position half (as k_gjf_a /
+// k_vv_a) and the list bookkeeping of prologue_atom / prologue_block_end.
+template <bool GJF, bool FUSE_B>
+__global__ __launch_bounds__(PRO_THREADS) void k_step_pro(IntArgs A, StepPro P, double* __restrict__ x,
+                                                          double* __restrict__ v, const double* __restrict__ im,
+                                                          const double* __restrict__ m, double* __restrict__ acc_prev,
+                                                          long long* __restrict__ ke_slot, DevState* st) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double d2 = 0.0;
+  double d2 = 0.0, ke = 0.0;
   bool bad = false;
   if (i < A.n) {
-    double g[3], xn[3];
-    noise3(A.ns, st->step + 1, i, A.n, g);
-    const double sm = sqrt(im[i]);
+    double vv[3], xn[3], f[3];
     for (int d = 0; d < 3; ++d) {
-      const double acc = fix_force(P.ffix, A, i, d) * im[i];
-      acc_prev[3 * i + d] = acc;
-      xn[d] = x[3 * i + d] + A.b * A.dt * v[3 * i + d] + A.b * A.dt * A.dt * 0.5 * acc +
-              A.b * A.sigma * A.dt * sqrt(A.dt) * 0.5 * g[d] * sm;
-      x[3 * i + d] = xn[d];
+      f[d] = fix_force(P.ffix, A, i, d);
+      vv[d] = v[3 * i + d];
+    }
+    if (FUSE_B) {
+      if (GJF) {
+        double g[3];
+        noise3(A.ns, st->step, i, A.n, g);
+        const double sm = sqrt(im[i]);
+        for (int d = 0; d < 3; ++d) {
+          const double acc = f[d] * im[i];
+          vv[d] = A.a * vv[d] + 0.5 * A.dt * (A.a * acc_prev[3 * i + d] + acc) + A.b * A.sigma * sqrt(A.dt) * g[d] * sm;
+          ke += vv[d] * vv[d];
+        }
+      } else {
+        for (int d = 0; d < 3; ++d) {
+          vv[d] = vv[d] + 0.5 * A.dt * f[d] * im[i];
+          ke += vv[d] * vv[d];
+        }
+      }
+      ke *= 0.5 * m[i];
+    }
+    if (GJF) {
+      double g[3];
+      noise3(A.ns, st->step + 1, i, A.n, g);
+      const double sm = sqrt(im[i]);
+      for (int d = 0; d < 3; ++d) {
+        const double acc = f[d] * im[i];
+        acc_prev[3 * i + d] = acc;
+        xn[d] = x[3 * i + d] + A.b * A.dt * vv[d] + A.b * A.dt * A.dt * 0.5 * acc +
+                A.b * A.sigma * A.dt * sqrt(A.dt) * 0.5 * g[d] * sm;
+        x[3 * i + d] = xn[d];
+        if (FUSE_B) v[3 * i + d] = vv[d];
+      }
+    } else {
+      for (int d = 0; d < 3; ++d) {
+        const double vh = vv[d] + 0.5 * A.dt * f[d] * im[i];
+        v[3 * i + d] = vh;
+        xn[d] = x[3 * i + d] + A.dt * vh;
+        x[3 * i + d] = xn[d];
+      }
     }
     prologue_atom(A, P, st, i, xn, &d2, &bad);
   }
-  prologue_block_end(P, st, d2, bad);
-}
-
-__global__ __launch_bounds__(256) void k_vv_a_pro(IntArgs A, StepPro P, double* __restrict__ x, double* __restrict__ v,
-                                                  const double* __restrict__ im, DevState* st) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double d2 = 0.0;
-  bool bad = false;
-  if (i < A.n) {
-    double xn[3];
-    for (int d = 0; d < 3; ++d) {
-      const double vv = v[3 * i + d] + 0.5 * A.dt * fix_force(P.ffix, A, i, d) * im[i];
-      v[3 * i + d] = vv;
-      xn[d] = x[3 * i + d] + A.dt * vv;
-      x[3 * i + d] = xn[d];
-    }
-    prologue_atom(A, P, st, i, xn, &d2, &bad);
-  }
+  if (FUSE_B) block_add_energy(ke, ke_slot);  // before this block's ticket: the last block records it
   prologue_block_end(P, st, d2, bad);
 }
 
@@ -951,36 +1046,64 @@ static StepPro step_pro(mdx_ctx* c) {
   P.ncell = c->ncell;
   const double half = 0.5 * c->skin;
   P.thr2 = c->skin > 0 ? half * half : -1.0;
+  // the last workgroup bins the atoms itself when the grid's counts fit its LDS
+  P.bin = 2 * (size_t)c->ncell * sizeof(int) <= PRO_BIN_LDS_MAX ? 1 : 0;
+  P.atom_cell = c->atom_cell.p;
+  P.cell_start = c->cell_start.p;
+  P.sorted_orig = c->sorted_orig.p;
+  for (int d = 0; d < 3; ++d) {
+    P.gn[d] = c->gn[d];
+    P.gw[d] = c->box[d] / c->gn[d];
+  }
+  P.n = c->n;
   return P;
 }
 
-int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy) {
+// The step prologue applies (G-JF or velocity Verlet) when the step's forces walk the list grid
+// (pair terms or the PME plane kernels) of an exact-pair-set context.
+static bool fused_steps(const mdx_ctx* c, unsigned terms) {
+  const bool grid_used = c->nb_configured && c->pair_mode == MDX_PAIRS_EXACT &&
+                         (terms & (MDX_TERM_LJ | MDX_TERM_COUL_DIRECT | MDX_TERM_COUL_RECIP)) != 0 &&
+                         ((terms & (MDX_TERM_LJ | MDX_TERM_COUL_DIRECT)) || c->pme_planes);
+  return grid_used && (c->int_kind == MDX_INT_LANGEVIN || c->int_kind == MDX_INT_VELOCITY_VERLET);
+}
+
+// One integrator step. fuse_prev_b: this step's prologue also completes the previous step's
+// velocity half (the previous call to this function had emit_b = false); emit_b: finish this
+// step with the standalone velocity half.
+int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy, bool fuse_prev_b, bool emit_b) {
   IntArgs A = int_args(c);
   dim3 g(div_up(c->n, 256)), b(256);
   int rc;
   long long* ke = c->efix.p + MDX_E_KINETIC;
-  // the fused prologue applies when this step's forces walk the list grid (pair terms or the
-  // PME plane kernels) of an exact-pair-set context
-  const bool grid_used = c->nb_configured && c->pair_mode == MDX_PAIRS_EXACT &&
-                         (terms & (MDX_TERM_LJ | MDX_TERM_COUL_DIRECT | MDX_TERM_COUL_RECIP)) != 0 &&
-                         ((terms & (MDX_TERM_LJ | MDX_TERM_COUL_DIRECT)) || c->pme_planes);
-  if (grid_used && (c->int_kind == MDX_INT_LANGEVIN || c->int_kind == MDX_INT_VELOCITY_VERLET)) {
+  if (fused_steps(c, terms)) {
     StepPro P = step_pro(c);
-    if (c->int_kind == MDX_INT_LANGEVIN)
-      k_gjf_a_pro<<<g, b, 0, c->stream>>>(A, P, c->pos, c->vel, c->inv_mass.p, c->acc_prev.p, c->dstate.p);
-    else
-      k_vv_a_pro<<<g, b, 0, c->stream>>>(A, P, c->pos, c->vel, c->inv_mass.p, c->dstate.p);
+    const size_t lds = P.bin ? 2 * (size_t)c->ncell * sizeof(int) : 0;
+    dim3 gp(div_up(c->n, PRO_THREADS)), bp(PRO_THREADS);
+    A.fpart = live_fpart(c);  // the forces the prologue reads are the last evaluation's
+    const bool gjf = c->int_kind == MDX_INT_LANGEVIN;
+#define PRO_L(G, F) \
+  k_step_pro<G, F><<<gp, bp, lds, c->stream>>>(A, P, c->pos, c->vel, c->inv_mass.p, c->mass.p, c->acc_prev.p, ke, c->dstate.p)
+    if (gjf && fuse_prev_b) PRO_L(true, true);
+    else if (gjf) PRO_L(true, false);
+    else if (fuse_prev_b) PRO_L(false, true);
+    else PRO_L(false, false);
+#undef PRO_L
     MDX_LAUNCH_CHECK();
     c->prologue_done = true;
+    c->prologue_binned = P.bin != 0;
     rc = mdx_enqueue_forces(c, terms, energy);
     c->prologue_done = false;
+    c->prologue_binned = false;
     if (rc) return rc;
-    A.fpart = live_fpart(c);
-    if (c->int_kind == MDX_INT_LANGEVIN)
-      k_gjf_b<<<g, b, 0, c->stream>>>(A, c->vel, c->inv_mass.p, c->mass.p, c->ffix.p, c->acc_prev.p, c->dstate.p, ke);
-    else
-      k_vv_b<<<g, b, 0, c->stream>>>(A, c->vel, c->inv_mass.p, c->mass.p, c->ffix.p, ke);
-    MDX_LAUNCH_CHECK();
+    if (emit_b) {
+      A.fpart = live_fpart(c);
+      if (gjf)
+        k_gjf_b<<<g, b, 0, c->stream>>>(A, c->vel, c->inv_mass.p, c->mass.p, c->ffix.p, c->acc_prev.p, c->dstate.p, ke);
+      else
+        k_vv_b<<<g, b, 0, c->stream>>>(A, c->vel, c->inv_mass.p, c->mass.p, c->ffix.p, ke);
+      MDX_LAUNCH_CHECK();
+    }
     return 0;
   }
   switch (c->int_kind) {
@@ -1091,14 +1214,32 @@ static int graph_steps_for(const mdx_ctx* c, int nsteps) {
   return std::min(nsteps, every >= 10 ? every : every * ((10 + every - 1) / every));
 }
 
-// direct (uncaptured) steps local0 .. local0+count-1 of the current call
-static int integ_direct(mdx_ctx* c, unsigned terms, int local0, int count) {
+// steps local0 .. local0+count-1 of the current call. With the step prologue every step but
+// the call's first folds the previous step's velocity half into its prologue (fuse_first for the
+// first of these), and the call ends with one standalone velocity half (integ_final_b), so graph
+// replays and direct launches run the same kernels in the same order.
+static int integ_steps(mdx_ctx* c, unsigned terms, int local0, int count, bool fuse_first) {
   int rc = 0;
-  for (int k = 0; k < count && !rc; ++k) rc = mdx_integrator_enqueue_step(c, terms, (local0 + k + 1) % c->energy_every == 0);
+  for (int k = 0; k < count && !rc; ++k)
+    rc = mdx_integrator_enqueue_step(c, terms, (local0 + k + 1) % c->energy_every == 0, k > 0 || fuse_first, false);
   return rc;
 }
 
-// the context's own graph of `gsteps` steps (captured on its stream; cached)
+static int integ_final_b(mdx_ctx* c, unsigned terms, int nsteps) {
+  if (nsteps <= 0 || !fused_steps(c, terms)) return 0;  // other integrators finish every step
+  IntArgs A = int_args(c);
+  dim3 g(div_up(c->n, 256)), b(256);
+  long long* ke = c->efix.p + MDX_E_KINETIC;
+  if (c->int_kind == MDX_INT_LANGEVIN)
+    k_gjf_b<<<g, b, 0, c->stream>>>(A, c->vel, c->inv_mass.p, c->mass.p, c->ffix.p, c->acc_prev.p, c->dstate.p, ke);
+  else
+    k_vv_b<<<g, b, 0, c->stream>>>(A, c->vel, c->inv_mass.p, c->mass.p, c->ffix.p, ke);
+  MDX_LAUNCH_CHECK();
+  return 0;
+}
+
+// the context's own graph of `gsteps` steps following a call's first step (captured on its
+// stream; cached)
 static int ensure_graph(mdx_ctx* c, unsigned terms, int gsteps) {
   const int every = c->energy_every;
   if (c->graph_exec && c->graph_terms == terms && c->graph_steps == gsteps && c->graph_kind == c->int_kind &&
@@ -1107,7 +1248,7 @@ static int ensure_graph(mdx_ctx* c, unsigned terms, int gsteps) {
   mdx_invalidate_graph(c);
   hipGraph_t graph;
   MDX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  int rc = integ_direct(c, terms, 0, gsteps);
+  int rc = integ_steps(c, terms, 1, gsteps, true);
   hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
   if (rc) return rc;
   if (e2 != hipSuccess) { mdx_set_error("graph capture failed: %s", hipGetErrorString(e2)); return MDX_ERR_HIP; }
@@ -1127,19 +1268,23 @@ extern "C" int mdx_integrate(mdx_ctx* c, unsigned terms, int nsteps, int use_gra
   int rc = integ_begin(c, terms, nsteps);
   if (rc) LEAVE_RET(rc);
   // potential energies are evaluated on steps that are multiples of `every` (the prime counts
-  // as step 0); the graph holds a whole number of energy intervals
-  if (use_graph && nsteps > 0) {
-    const int gsteps = graph_steps_for(c, nsteps);
+  // as step 0); step 0 of the call is launched directly, then the graph (a whole number of
+  // energy intervals) covers the following steps, then the remainder and the final velocity half
+  if (use_graph && nsteps > 1) {
+    const int rest = nsteps - 1;
+    const int gsteps = graph_steps_for(c, rest);
     if ((rc = ensure_graph(c, terms, gsteps))) LEAVE_RET(rc);
-    const int full = nsteps / gsteps, rem = nsteps % gsteps;
+    if ((rc = integ_steps(c, terms, 0, 1, false))) LEAVE_RET(rc);
+    const int full = rest / gsteps, rem = rest % gsteps;
     for (int k = 0; k < full; ++k) {
       hipError_t e = hipGraphLaunch(c->graph_exec, c->stream);
       if (e != hipSuccess) { mdx_set_error("hipGraphLaunch: %s", hipGetErrorString(e)); LEAVE_RET(MDX_ERR_HIP); }
     }
-    rc = integ_direct(c, terms, full * gsteps, rem);
+    rc = integ_steps(c, terms, 1 + full * gsteps, rem, true);
   } else {
-    rc = integ_direct(c, terms, 0, nsteps);
+    rc = integ_steps(c, terms, 0, nsteps, false);
   }
+  if (!rc) rc = integ_final_b(c, terms, nsteps);
   if (rc) LEAVE_RET(rc);
   LEAVE_RET(integ_end(c, nsteps));
 }
@@ -1230,11 +1375,18 @@ extern "C" int mdx_batch_integrate(mdx_batch* b, unsigned terms, int nsteps, int
   int rc = 0;
   for (mdx_ctx* c : b->ctx)
     if ((rc = integ_begin(c, terms, nsteps))) return rc;
-  const int gsteps = graph_steps_for(b->ctx[0], nsteps);
-  const int full = gsteps > 0 ? nsteps / gsteps : 0, rem = gsteps > 0 ? nsteps % gsteps : 0;
-  if (full > 0) {
+  // same schedule as mdx_integrate: step 0 directly, graphs for the following steps, remainder,
+  // final velocity half
+  const int rest = nsteps > 0 ? nsteps - 1 : 0;
+  const int gsteps = graph_steps_for(b->ctx[0], rest);
+  const int full = gsteps > 0 ? rest / gsteps : 0, rem = gsteps > 0 ? rest % gsteps : 0;
+  if (full > 0)
     for (mdx_ctx* c : b->ctx)
       if ((rc = ensure_graph(c, terms, gsteps))) return rc;
+  if (nsteps > 0)
+    for (mdx_ctx* c : b->ctx)
+      if ((rc = integ_steps(c, terms, 0, 1, false))) return rc;
+  if (full > 0) {
     if (mode == MDX_BATCH_STREAMS) {  // interleaved launches: the replicas' graphs run side by side
       for (int k = 0; k < full; ++k)
         for (mdx_ctx* c : b->ctx) {
@@ -1253,7 +1405,8 @@ extern "C" int mdx_batch_integrate(mdx_batch* b, unsigned terms, int nsteps, int
     }
   }
   for (mdx_ctx* c : b->ctx) {
-    if ((rc = integ_direct(c, terms, full * gsteps, rem))) return rc;
+    if (nsteps > 0 && (rc = integ_steps(c, terms, 1 + full * gsteps, rem, true))) return rc;
+    if ((rc = integ_final_b(c, terms, nsteps))) return rc;
     if ((rc = integ_end(c, nsteps))) return rc;
   }
   // leave: the caller's stream waits for every replica
@@ -1368,7 +1521,7 @@ extern "C" int mdx_time_kernel(mdx_ctx* c, int which, unsigned terms, int reps, 
       rc = mdx_enqueue_zero_accumulators(c);
       if (!rc) rc = mdx_enqueue_forces(c, terms, true);
     } else if (which == 2) rc = mdx_pme_enqueue(c, true);
-    else if (which == 3) rc = mdx_integrator_enqueue_step(c, terms, false);
+    else if (which == 3) rc = mdx_integrator_enqueue_step(c, terms, false, false, true);
     else rc = MDX_ERR_INVALID;
   }
   MDX_HIP(hipEventRecord(e1, c->stream));

@@ -169,6 +169,7 @@ struct mdx_ctx {
   int list_stride = 0;
   bool list_valid = false;
   bool prologue_done = false;    // the integrator's fused kernel already checked/zeroed/packed this step
+  bool prologue_binned = false;  // ... and (on rebuild steps) counted, scanned and scattered the atoms
   std::vector<int> stencil;      // packed stencil offsets (dx,dy,dz) of the list grid
   DevBuf<int> d_stencil;
   // ---- PME
@@ -282,6 +283,6 @@ int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy);
 #define MDX_E_MLP 8  // energy slot of MDX_TERM_MLP (its bit index)
 int mdx_mlp_enqueue(mdx_ctx* c, bool energy);
 // integrator
-int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy);
+int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy, bool fuse_prev_b, bool emit_b);
 int mdx_integrator_prime(mdx_ctx* c, unsigned terms);
 int mdx_integrator_finish(mdx_ctx* c);

@@ -862,13 +862,16 @@ int mdx_nlist_update(mdx_ctx* c, bool force_rebuild) {
     int rc = ref_bin_enqueue(c, c->cell_cutoff, nullptr);
     if (rc) return rc;
   }
-  k_grid_bin<<<nb, NL_BLOCK, 0, c->stream>>>(c->pos, c->n, g, st, thr2, c->atom_cell.p, c->cell_count.p);
-  MDX_LAUNCH_CHECK();
-  k_cell_scan<<<1, 1024, 0, c->stream>>>(c->cell_count.p, c->cell_start.p, c->cell_cursor.p, c->ncell, st, st, thr2);
-  MDX_LAUNCH_CHECK();
-  k_grid_scatter<<<nb, NL_BLOCK, 0, c->stream>>>(c->atom_cell.p, c->n, c->cell_start.p, c->cell_cursor.p,
-                                                 c->sorted_orig.p, st, thr2);
-  MDX_LAUNCH_CHECK();
+  if (!c->prologue_binned) {  // else the step prologue's last workgroup binned the atoms
+    k_grid_bin<<<nb, NL_BLOCK, 0, c->stream>>>(c->pos, c->n, g, st, thr2, c->atom_cell.p, c->cell_count.p);
+    MDX_LAUNCH_CHECK();
+    k_cell_scan<<<1, 1024, 0, c->stream>>>(c->cell_count.p, c->cell_start.p, c->cell_cursor.p, c->ncell, st, st,
+                                           thr2);
+    MDX_LAUNCH_CHECK();
+    k_grid_scatter<<<nb, NL_BLOCK, 0, c->stream>>>(c->atom_cell.p, c->n, c->cell_start.p, c->cell_cursor.p,
+                                                   c->sorted_orig.p, st, thr2);
+    MDX_LAUNCH_CHECK();
+  }
   bool ref = c->pair_mode == MDX_PAIRS_REFERENCE;
   k_grid_sort_fill<<<c->ncell, 256, 0, c->stream>>>(
       c->cell_start.p, c->cell_count.p, c->sorted_orig.p, c->orig_sorted.p, c->pos, c->pos_at_build.p,
