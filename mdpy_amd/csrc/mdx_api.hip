@@ -653,6 +653,7 @@ extern "C" int mdx_ctx_destroy(mdx_ctx* c) {
   c->twiddle.release(); c->excl_pairs.release(); c->xa.release(); c->xb.release(); c->va.release();
   c->vb.release(); c->acc_prev.release(); c->acc_cur.release(); c->ehist.release(); c->tmp64.release();
   c->pos_sorted.release(); c->fpart.release(); c->pme_grec.release(); c->pme_grec_n.release();
+  c->ex_win.release(); c->s4_win.release(); c->win_bad.release();
   if (c->owns_stream) hipStreamDestroy(c->stream);
   if (c->side_stream) hipStreamDestroy(c->side_stream);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
@@ -777,12 +778,37 @@ static int set_csr(mdx_ctx* c, const int* ptr, const int* idx, DevBuf<int>& dp, 
   return 0;
 }
 
+// index-window masks of one partner relation (bit = 0 exclusions, 1 scaled 1-4 pairs)
+static int set_window_masks(mdx_ctx* c, const int* h_ptr, const int* h_idx, int bit) {
+  const int n = c->n;
+  std::vector<unsigned long long>& m = bit == 0 ? c->h_ex_win : c->h_s4_win;
+  m.assign(n, 0ull);
+  if (c->h_win_bad.size() != (size_t)n) c->h_win_bad.assign(n, 0);
+  if (bit == 0 && c->h_s4_win.size() != (size_t)n) c->h_s4_win.assign(n, 0ull);
+  if (bit == 1 && c->h_ex_win.size() != (size_t)n) c->h_ex_win.assign(n, 0ull);
+  for (int i = 0; i < n; ++i) {
+    c->h_win_bad[i] &= (unsigned char)~(1u << bit);
+    for (int k = h_ptr[i]; k < h_ptr[i + 1]; ++k) {
+      const int d = h_idx[k] - i + 32;
+      if (d >= 0 && d < 64) m[i] |= 1ull << d;
+      else c->h_win_bad[i] |= (unsigned char)(1u << bit);
+    }
+  }
+  int rc;
+  if ((rc = upload(c->ex_win, c->h_ex_win.data(), (size_t)n)) || (rc = upload(c->s4_win, c->h_s4_win.data(), (size_t)n)) ||
+      (rc = upload(c->win_bad, c->h_win_bad.data(), (size_t)n)))
+    return rc;
+  c->win_ready = true;
+  return 0;
+}
+
 extern "C" int mdx_set_exclusions(mdx_ctx* c, const int* h_ptr, const int* h_idx) {
   MDX_CHECK(c && c->n > 0, MDX_ERR_INVALID, "mdx_set_exclusions: no system");
   mdx_invalidate_graph(c);  // parameter buffers may move
   int rc = set_csr(c, h_ptr, h_idx, c->excl_ptr, c->excl_idx, &c->max_excl);
   if (rc) return rc;
   c->has_excl = h_ptr[c->n] > 0;
+  if ((rc = set_window_masks(c, h_ptr, h_idx, 0))) return rc;
   // unique excluded pairs (i < j) for the PME erf correction
   std::vector<int> pairs;
   for (int i = 0; i < c->n; ++i)
@@ -805,6 +831,7 @@ extern "C" int mdx_set_scaling(mdx_ctx* c, const int* h_ptr, const int* h_idx) {
   int rc = set_csr(c, h_ptr, h_idx, c->s14_ptr, c->s14_idx, &c->max_s14);
   if (rc) return rc;
   c->has_s14 = h_ptr[c->n] > 0;
+  if ((rc = set_window_masks(c, h_ptr, h_idx, 1))) return rc;
   int one = 1;
   MDX_HIP(hipMemcpy(&c->dstate.p->rebuild, &one, sizeof(int), hipMemcpyHostToDevice));
   return 0;

@@ -170,6 +170,13 @@ struct mdx_ctx {
   bool list_valid = false;
   bool prologue_done = false;    // the integrator's fused kernel already checked/zeroed/packed this step
   bool prologue_binned = false;  // ... and (on rebuild steps) counted, scanned and scattered the atoms
+  // exclusion / 1-4 partners as 64-bit masks over the index window i-32 .. i+31 (list build);
+  // win_bad[i] bit 0 / 1: atom i has an exclusion / 1-4 partner outside the window
+  DevBuf<unsigned long long> ex_win, s4_win;
+  DevBuf<unsigned char> win_bad;
+  std::vector<unsigned long long> h_ex_win, h_s4_win;
+  std::vector<unsigned char> h_win_bad;
+  bool win_ready = false;
   std::vector<int> stencil;      // packed stencil offsets (dx,dy,dz) of the list grid
   DevBuf<int> d_stencil;
   // ---- PME

@@ -401,6 +401,7 @@ struct ListArgs {
   int ref_mode;       // REFERENCE stencil filter
   int rnx, rny, rnz;  // reference grid
   int nstencil;
+  int shifted;        // staged candidates are shifted into the home frame (no minimum image)
 };
 
 __device__ __forceinline__ int cyc_dist(int a, int b, int n) {
@@ -408,144 +409,44 @@ __device__ __forceinline__ int cyc_dist(int a, int b, int n) {
   return min(d, n - d);
 }
 
-// NB_G lanes build one atom's list: for every stencil cell the group scans the cell's atoms
-// NB_G at a time (contiguous loads), and a wave ballot turns the accepted lanes into
-// consecutive, deterministic slots of the atom's row-major list.
-__global__ __launch_bounds__(NL_BLOCK) void k_build_list(
-    ListArgs a, const float4* __restrict__ xq, const int* __restrict__ sorted_orig,
-    const int* __restrict__ cell_start, const int* __restrict__ cell_count, const int* __restrict__ stencil,
-    const int* __restrict__ excl_ptr, const int* __restrict__ excl_idx, const int* __restrict__ s14_ptr,
-    const int* __restrict__ s14_idx, const int* __restrict__ ref_cell_sorted, const double* __restrict__ pos,
-    int* __restrict__ nbr_count, unsigned* __restrict__ nbr_list, DevState* st, double thr2) {
-  if (!need_rebuild(st, thr2)) return;
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = gt / NB_G;
-  const int g = gt & (NB_G - 1);
-  if (s >= a.n) return;  // whole groups leave together
-  const int lane = threadIdx.x & 63;
-  const unsigned long long gmask = (NB_G == 64 ? ~0ull : ((1ull << NB_G) - 1ull)) << (lane & ~(NB_G - 1));
-  const unsigned long long below = (1ull << lane) - 1ull;
-  const GridArgs& gr = a.g;
-  const int o = sorted_orig[s];
-  const float4 xi = xq[s];
-  const float Lx = (float)gr.Lx, Ly = (float)gr.Ly, Lz = (float)gr.Lz;
-  const float ix = 1.0f / Lx, iy = 1.0f / Ly, iz = 1.0f / Lz;
-  double px = pos[3 * o + 0], py = pos[3 * o + 1], pz = pos[3 * o + 2];
-  double wx = px - gr.Lx * floor(px / gr.Lx), wy = py - gr.Ly * floor(py / gr.Ly), wz = pz - gr.Lz * floor(pz / gr.Lz);
-  int cx = min(gr.nx - 1, (int)(wx / gr.wx)), cy = min(gr.ny - 1, (int)(wy / gr.wy)), cz = min(gr.nz - 1, (int)(wz / gr.wz));
-  const int e0 = excl_ptr ? excl_ptr[o] : 0, e1 = excl_ptr ? excl_ptr[o + 1] : 0;
-  const int t0s = s14_ptr ? s14_ptr[o] : 0, t1s = s14_ptr ? s14_ptr[o + 1] : 0;
-  const int rci = a.ref_mode ? ref_cell_sorted[s] : 0;
-  unsigned* row = nbr_list + (long long)s * a.cap;
-  // the first few excluded / 1-4 partners live in registers (water: 2 and 0)
-  constexpr int NREG = 6;
-  int exr[NREG], s4r[NREG];
-#pragma unroll
-  for (int q = 0; q < NREG; ++q) {
-    exr[q] = e0 + q < e1 ? excl_idx[e0 + q] : -1;
-    s4r[q] = t0s + q < t1s ? s14_idx[t0s + q] : -1;
-  }
-  int cnt = 0;
-  for (int k = 0; k < a.nstencil; ++k) {
-    int p = stencil[k];
-    int dx = ((p >> 16) & 0xff) - 128, dy = ((p >> 8) & 0xff) - 128, dz = (p & 0xff) - 128;
-    int nx = cx + dx, ny = cy + dy, nz = cz + dz;
-    nx += nx < 0 ? gr.nx : (nx >= gr.nx ? -gr.nx : 0);
-    ny += ny < 0 ? gr.ny : (ny >= gr.ny ? -gr.ny : 0);
-    nz += nz < 0 ? gr.nz : (nz >= gr.nz ? -gr.nz : 0);
-    const int cell = (nx * gr.ny + ny) * gr.nz + nz;
-    const int b = cell_start[cell], e = b + cell_count[cell];
-    for (int tb = b; tb < e; tb += NB_G) {
-      const int t = tb + g;
-      bool acc = false, s14 = false;
-      if (t < e && t != s) {
-        const float4 xj = xq[t];
-        const int jo = sorted_orig[t];
-        float ddx = xj.x - xi.x, ddy = xj.y - xi.y, ddz = xj.z - xi.z;
-        ddx -= Lx * rintf(ddx * ix);
-        ddy -= Ly * rintf(ddy * iy);
-        ddz -= Lz * rintf(ddz * iz);
-        float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-        if (r2 < a.rl2) {
-          acc = true;
-          if (a.exact64) {  // float64 decision with the reference's sequence (pbc.py:40-44)
-            double d[3];
-            double L[3] = {gr.Lx, gr.Ly, gr.Lz};
-            for (int q = 0; q < 3; ++q) {
-              double v = pos[3 * jo + q] - pos[3 * o + q];
-              double sv = v * (1.0 / L[q]);
-              sv -= rint(sv);
-              d[q] = sv * L[q];
-            }
-            acc = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) <= a.rc;
-          }
-#pragma unroll
-          for (int q = 0; q < NREG; ++q) acc = acc && (exr[q] != jo);
-          for (int q = e0 + NREG; q < e1; ++q) acc = acc && (excl_idx[q] != jo);
-          if (acc && a.ref_mode) {
-            int rcj = ref_cell_sorted[t];
-            int ax = cyc_dist(rci >> 20, rcj >> 20, a.rnx);
-            int ay = cyc_dist((rci >> 10) & 1023, (rcj >> 10) & 1023, a.rny);
-            int az = cyc_dist(rci & 1023, rcj & 1023, a.rnz);
-            acc = ax <= 1 && ay <= 1 && az <= 1;
-          }
-          if (acc) {
-#pragma unroll
-            for (int q = 0; q < NREG; ++q) s14 |= (s4r[q] == jo);
-            for (int q = t0s + NREG; q < t1s; ++q) s14 |= (s14_idx[q] == jo);
-          }
-        }
-      }
-      const unsigned long long m = __ballot(acc) & gmask;
-      const int slot = cnt + __popcll(m & below);
-      if (acc && slot < a.cap) row[slot] = (unsigned)t | (s14 ? 0x80000000u : 0u);
-      cnt += __popcll(m);
-    }
-  }
-  if (g == 0) {
-    nbr_count[s] = min(cnt, a.cap);
-    if (cnt > a.cap) atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
-  }
-}
-
 // One workgroup per home cell: the candidates of every stencil cell (identical for all atoms of
 // the home cell) are staged in LDS in chunks, then each NB_G-lane group scans them for one home
 // atom at a time and compacts the accepted ones with a wave ballot. Same candidate order, hence
 // the same lists, as a per-atom walk of the stencil; global memory is read once per cell.
-// The scan loop reads LDS only (the home atom's exclusions and 1-4 partners are staged there
-// too) and evaluates its tests without short-circuit branches, so an iteration waits for LDS
-// once and never for global memory; the float64 decision of skin-0 lists is a separate
-// instantiation.
+// Staged candidates are shifted into the home cell's periodic frame (when the stencil holds each
+// periodic cell once, on no more than one side), so the scan needs no minimum-image arithmetic.
+// Exclusions and 1-4 partners within 32 atom indices of the home atom are bits of two 64-bit
+// masks; an atom with a partner outside that window scans its CSR lists (staged in LDS) instead.
+// Each lane tests BL_U candidates per pass, all LDS reads issued first; the float64 decision of
+// skin-0 lists is a separate instantiation.
 #define BL_CH 768
 #define BL_MAXST 512
 #define BL_MAXEX 32  // exclusions (and 1-4 partners) per atom
 #define BL_HMAX 192  // home atoms staged in LDS (more: their data is read from global memory)
 #define BL_POOL 1024 // exclusion / 1-4 entries of a cell's home atoms staged in LDS
 #define BL_THREADS 512
+#define BL_U 4       // candidates per lane per pass
+#define BL_WIN 32    // index window of the exclusion masks: partners i-32 .. i+31
 struct CandTest {
   float4 xi;
   float Lx, Ly, Lz, ix, iy, iz, rl2;
 };
-__device__ __forceinline__ bool cand_in_range(const CandTest& T, float4 xj) {
-  float dx = xj.x - T.xi.x, dy = xj.y - T.xi.y, dz = xj.z - T.xi.z;
-  dx -= T.Lx * rintf(dx * T.ix);
-  dy -= T.Ly * rintf(dy * T.iy);
-  dz -= T.Lz * rintf(dz * T.iz);
-  return dx * dx + dy * dy + dz * dz < T.rl2;
-}
 
 template <bool EXACT64>
 __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
     ListArgs a, const float4* __restrict__ xq, const int* __restrict__ sorted_orig,
     const int* __restrict__ cell_start, const int* __restrict__ cell_count, const int* __restrict__ stencil,
     const int* __restrict__ excl_ptr, const int* __restrict__ excl_idx, const int* __restrict__ s14_ptr,
-    const int* __restrict__ s14_idx, const int* __restrict__ ref_cell_sorted, const double* __restrict__ pos,
-    int* __restrict__ nbr_count, unsigned* __restrict__ nbr_list, DevState* st, double thr2) {
+    const int* __restrict__ s14_idx, const unsigned long long* __restrict__ ex_win,
+    const unsigned long long* __restrict__ s4_win, const unsigned char* __restrict__ win_bad,
+    const int* __restrict__ ref_cell_sorted, const double* __restrict__ pos, int* __restrict__ nbr_count,
+    unsigned* __restrict__ nbr_list, DevState* st, double thr2) {
   if (!need_rebuild(st, thr2)) return;
   __shared__ float4 cxq[BL_CH];
   __shared__ int2 cto[BL_CH];      // (sorted slot, original atom id)
   __shared__ int cr[BL_CH];
   __shared__ int sb[BL_MAXST], sp[BL_MAXST + 1];
+  __shared__ unsigned char shc[BL_MAXST];  // periodic shift of each stencil cell: 2 bits per axis
   __shared__ int acnt[SORT_MAX];
   __shared__ int exl[BL_THREADS / NB_G][BL_MAXEX], s4l[BL_THREADS / NB_G][BL_MAXEX];
   // home atoms staged once per block: position, atom id and their exclusion / 1-4 lists
@@ -562,6 +463,12 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
   for (int k = tid; k < a.nstencil; k += blockDim.x) {
     int p = stencil[k];
     int nx = hx + ((p >> 16) & 0xff) - 128, ny = hy + ((p >> 8) & 0xff) - 128, nz = hz + (p & 0xff) - 128;
+    // a neighbour reached across the box edge is the periodic image one box length away
+    // (code per axis: 0 = -L, 1 = 0, 2 = +L)
+    const int cx = nx < 0 ? 0 : (nx >= gr.nx ? 2 : 1);
+    const int cy = ny < 0 ? 0 : (ny >= gr.ny ? 2 : 1);
+    const int cz = nz < 0 ? 0 : (nz >= gr.nz ? 2 : 1);
+    shc[k] = (unsigned char)(cx | (cy << 2) | (cz << 4));
     nx += nx < 0 ? gr.nx : (nx >= gr.nx ? -gr.nx : 0);
     ny += ny < 0 ? gr.ny : (ny >= gr.ny ? -gr.ny : 0);
     nz += nz < 0 ? gr.nz : (nz >= gr.nz ? -gr.nz : 0);
@@ -632,7 +539,14 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
         else hi = mid - 1;
       }
       const int t = sb[lo] + (idx - sp[lo]);
-      cxq[c] = xq[t];
+      float4 v = xq[t];
+      if (a.shifted) {
+        const int code = shc[lo];
+        v.x += (float)((code & 3) - 1) * T.Lx;
+        v.y += (float)(((code >> 2) & 3) - 1) * T.Ly;
+        v.z += (float)(((code >> 4) & 3) - 1) * T.Lz;
+      }
+      cxq[c] = v;
       cto[c] = make_int2(t, sorted_orig[t]);
       if (a.ref_mode) cr[c] = ref_cell_sorted[t];
     }
@@ -640,24 +554,33 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
     for (int h = q; h < hn; h += ngroups) {  // uniform per group
       const int s = hb + h;
       int o, nex_all, ns4_all;
+      const int* src_ex;
+      const int* src_s4;
       if (fast) {  // everything from LDS
         o = ho[h];
         T.xi = hxi[h];
         nex_all = hex0[h + 1] - hex0[h];
         ns4_all = hs40[h + 1] - hs40[h];
-        if (g < nex_all && g < BL_MAXEX) my_ex[g] = pool_ex[hex0[h] + g];
-        if (g < ns4_all && g < BL_MAXEX) my_s4[g] = pool_s4[hs40[h] + g];
+        src_ex = pool_ex + hex0[h];
+        src_s4 = pool_s4 + hs40[h];
       } else {     // crowded cell: read the home atom's data from global memory
         o = sorted_orig[s];
         T.xi = xq[s];
         const int e0 = excl_ptr ? excl_ptr[o] : 0, t0s = s14_ptr ? s14_ptr[o] : 0;
         nex_all = excl_ptr ? excl_ptr[o + 1] - e0 : 0;
         ns4_all = s14_ptr ? s14_ptr[o + 1] - t0s : 0;
-        if (g < nex_all && g < BL_MAXEX) my_ex[g] = excl_idx[e0 + g];
-        if (g < ns4_all && g < BL_MAXEX) my_s4[g] = s14_idx[t0s + g];
+        src_ex = excl_idx + e0;
+        src_s4 = s14_idx + t0s;
       }
-      const int nex = min(nex_all, BL_MAXEX), ns4 = min(ns4_all, BL_MAXEX);
-      if (g == 0 && (nex_all > BL_MAXEX || ns4_all > BL_MAXEX)) atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
+      // index-window masks, or the CSR lists for an atom with a partner outside the window
+      const bool masked = !(win_bad && win_bad[o]);
+      const unsigned long long exm = ex_win ? ex_win[o] : 0ull, s4m = s4_win ? s4_win[o] : 0ull;
+      const int nex = masked ? 0 : min(nex_all, BL_MAXEX), ns4 = masked ? 0 : min(ns4_all, BL_MAXEX);
+      if (!masked) {
+        if (g < nex) my_ex[g] = src_ex[g];
+        if (g < ns4) my_s4[g] = src_s4[g];
+        if (g == 0 && (nex_all > BL_MAXEX || ns4_all > BL_MAXEX)) atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
+      }
       const int rci = a.ref_mode ? ref_cell_sorted[s] : 0;
       double pi[3] = {0.0, 0.0, 0.0};
       if (EXACT64) {
@@ -671,25 +594,39 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       unsigned* row = nbr_list + (long long)s * a.cap;
       int cnt = acnt[h];
-      for (int cb = 0; cb < nch; cb += 2 * NB_G) {
-        const int cc[2] = {cb + g, cb + NB_G + g};
-        float4 xj[2];
-        int2 tj[2];
-        int rcj[2] = {0, 0};
+      for (int cb = 0; cb < nch; cb += BL_U * NB_G) {
+        float4 xj[BL_U];
+        int2 tj[BL_U];
+        int rcj[BL_U];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int ci = min(cc[u], nch - 1);
+        for (int u = 0; u < BL_U; ++u) {
+          const int ci = min(cb + u * NB_G + g, nch - 1);
           xj[u] = cxq[ci];
           tj[u] = cto[ci];
-          if (a.ref_mode) rcj[u] = cr[ci];
+          rcj[u] = a.ref_mode ? cr[ci] : 0;
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < BL_U; ++u) {
           const int jo = tj[u].y;
-          bool acc = (cc[u] < nch) & (tj[u].x != s) & cand_in_range(T, xj[u]);
-          bool s14 = false;
-          for (int r = 0; r < nex; ++r) acc &= (my_ex[r] != jo);
-          for (int r = 0; r < ns4; ++r) s14 |= (my_s4[r] == jo);
+          float dx = xj[u].x - T.xi.x, dy = xj[u].y - T.xi.y, dz = xj[u].z - T.xi.z;
+          if (!a.shifted) {
+            dx -= T.Lx * rintf(dx * T.ix);
+            dy -= T.Ly * rintf(dy * T.iy);
+            dz -= T.Lz * rintf(dz * T.iz);
+          }
+          bool acc = (cb + u * NB_G + g < nch) & (tj[u].x != s) & (dx * dx + dy * dy + dz * dz < T.rl2);
+          bool s14;
+          if (masked) {
+            const int dw = jo - o + BL_WIN;
+            const bool win = (unsigned)dw < 2u * BL_WIN;
+            const int sh = dw & (2 * BL_WIN - 1);
+            acc &= !(win && ((exm >> sh) & 1ull));
+            s14 = win && ((s4m >> sh) & 1ull);
+          } else {
+            s14 = false;
+            for (int r = 0; r < nex; ++r) acc &= (my_ex[r] != jo);
+            for (int r = 0; r < ns4; ++r) s14 |= (my_s4[r] == jo);
+          }
           if (a.ref_mode) {
             const int ax = cyc_dist(rci >> 20, rcj[u] >> 20, a.rnx);
             const int ay = cyc_dist((rci >> 10) & 1023, (rcj[u] >> 10) & 1023, a.rny);
@@ -895,11 +832,18 @@ int mdx_nlist_update(mdx_ctx* c, bool force_rebuild) {
   a.rnz = c->ref_n[2];
   a.nstencil = (int)c->stencil.size();
   MDX_CHECK(a.nstencil <= BL_MAXST, MDX_ERR_INVALID, "stencil of %d cells exceeds %d", a.nstencil, BL_MAXST);
+  // each periodic cell appears once in the stencil, on one side only: a fixed shift per stencil
+  // cell maps its atoms to the image nearest the home cell
+  a.shifted = 1;
+  for (int d = 0; d < 3; ++d)
+    if (2 * c->reach[d] + 1 >= c->gn[d]) a.shifted = 0;
+  const bool win = c->win_ready && c->h_win_bad.size() == (size_t)c->n;
   auto build = a.exact64 ? k_build_list_cell<true> : k_build_list_cell<false>;
   build<<<c->ncell, BL_THREADS, 0, c->stream>>>(
       a, c->xq.p, c->sorted_orig.p, c->cell_start.p, c->cell_count.p, c->d_stencil.p,
       c->has_excl ? c->excl_ptr.p : nullptr, c->excl_idx.p, c->has_s14 ? c->s14_ptr.p : nullptr, c->s14_idx.p,
-      c->ref_cell_sorted.p, c->pos, c->nbr_count.p, c->nbr_list.p, st, thr2);
+      win ? c->ex_win.p : nullptr, win ? c->s4_win.p : nullptr, win ? c->win_bad.p : nullptr, c->ref_cell_sorted.p,
+      c->pos, c->nbr_count.p, c->nbr_list.p, st, thr2);
   MDX_LAUNCH_CHECK();
   return 0;
 }
