@@ -898,7 +898,12 @@ int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy) {
     MDX_CHECK(!lj || c->has_lj, MDX_ERR_INVALID, "LJ requested but no LJ parameters set");
     MDX_CHECK(!cd || (c->has_charges && c->pme_configured), MDX_ERR_INVALID,
               "PME direct space requested but charges/PME not configured");
-    if ((rc = mdx_nlist_update(c, false))) return rc;
+    // inside an integrator step the bonded terms share the list-fill launch
+    c->fill_bonded_terms = c->prologue_done ? terms : 0u;
+    c->bonded_done = false;
+    rc = mdx_nlist_update(c, false);
+    c->fill_bonded_terms = 0;
+    if (rc) return rc;
   }
   // the PME plane kernels only read what the list update produced: run them on the side
   // stream, concurrently with the nonbonded and bonded kernels, and join before returning
@@ -936,7 +941,8 @@ int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy) {
   if (terms & MDX_TERM_MLP) {
     if ((rc = mdx_mlp_enqueue(c, energy))) return rc;
   }
-  if ((rc = mdx_bonded_enqueue(c, terms))) return rc;
+  if (!c->bonded_done && (rc = mdx_bonded_enqueue(c, terms))) return rc;
+  c->bonded_done = false;
   if (fork_pme) MDX_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   return 0;
 }

@@ -221,8 +221,8 @@ struct BondedArgs {
   double ke, beta, e_const;
 };
 
-__global__ __launch_bounds__(BD_BLOCK) void k_bonded(BondedArgs a) {
-  int seg = 0, blk = blockIdx.x;
+__device__ __forceinline__ void bonded_block(const BondedArgs& a, int blk) {
+  int seg = 0;
   while (seg < 4 && blk >= a.blocks[seg]) blk -= a.blocks[seg++];
   const int t = blk * BD_BLOCK + threadIdx.x;
   double e = 0.0;
@@ -239,8 +239,21 @@ __global__ __launch_bounds__(BD_BLOCK) void k_bonded(BondedArgs a) {
   block_energy(e, a.efix + (seg < 4 ? 3 + seg : 2));
 }
 
+__global__ __launch_bounds__(BD_BLOCK) void k_bonded(BondedArgs a) { bonded_block(a, blockIdx.x); }
+
+// The list fill (blocks below f.ncell, only on rebuild steps) and the bonded terms (the blocks
+// after) in one launch: the bonded terms need only the positions, so they run as soon as the
+// step prologue has moved the atoms, in the launch that precedes the list build.
+__global__ __launch_bounds__(BD_BLOCK) void k_fill_bonded(FillArgs f, BondedArgs a) {
+  if ((int)blockIdx.x < f.ncell) {
+    if (need_rebuild(f.st, f.thr2)) grid_sort_fill_block(f, blockIdx.x);
+    return;
+  }
+  bonded_block(a, blockIdx.x - f.ncell);
+}
+
 // bonded terms of `terms` plus, with MDX_TERM_COUL_RECIP, the PME exclusion/self/background part
-int mdx_bonded_enqueue(mdx_ctx* c, unsigned terms) {
+int mdx_bonded_enqueue(mdx_ctx* c, unsigned terms, const FillArgs* fill) {
   BondedArgs a;
   a.n = c->n;
   for (int q = 0; q < 3; ++q) {
@@ -279,6 +292,11 @@ int mdx_bonded_enqueue(mdx_ctx* c, unsigned terms) {
     const double e_self = -a.ke * c->beta / sqrt(M_PI) * c->q2sum;
     const double e_bg = -a.ke * M_PI * c->qsum * c->qsum / (2.0 * c->beta * c->beta * V);
     a.e_const = e_self + e_bg;
+  }
+  if (fill) {
+    k_fill_bonded<<<fill->ncell + total, BD_BLOCK, 0, c->stream>>>(*fill, a);
+    MDX_LAUNCH_CHECK();
+    return 0;
   }
   if (total == 0) return 0;
   k_bonded<<<total, BD_BLOCK, 0, c->stream>>>(a);

@@ -170,6 +170,8 @@ struct mdx_ctx {
   bool list_valid = false;
   bool prologue_done = false;    // the integrator's fused kernel already checked/zeroed/packed this step
   bool prologue_binned = false;  // ... and (on rebuild steps) counted, scanned and scattered the atoms
+  unsigned fill_bonded_terms = 0;  // bonded terms to launch together with the list fill (0: none)
+  bool bonded_done = false;        // this evaluation's bonded terms were launched with the fill
   // exclusion / 1-4 partners as 64-bit masks over the index window i-32 .. i+31 (list build);
   // win_bad[i] bit 0 / 1: atom i has an exclusion / 1-4 partner outside the window
   DevBuf<unsigned long long> ex_win, s4_win;
@@ -251,6 +253,86 @@ static inline double dec_double_order_host(unsigned long long e) {
   return v;
 }
 
+// ---- list rebuild, per-cell sort and fill (shared by the fill kernel and the fill + bonded launch)
+// decision shared by every rebuild kernel: rebuild requested, or an atom moved > skin/2
+__device__ __forceinline__ bool need_rebuild(const DevState* st, double thr2) {
+  return st->rebuild != 0 || dec_double_order(st->max_disp2_bits) > thr2;
+}
+
+struct FillArgs {
+  const int* cell_start;
+  const int* cell_count;
+  int* sorted_orig;
+  int* orig_sorted;
+  const double* pos;
+  double* pos_build;
+  const double* lj64;
+  float2* ljs;
+  float2* ljs14;
+  const int* ref_lin;
+  int* ref_cell_sorted;
+  int rny, rnz;
+  const DevState* st;
+  double thr2;
+  const float* qs;
+  float4* xq;
+  double* pos_sorted;
+  double L[3];
+  int ncell;
+};
+
+#ifndef SORT_MAX
+#define SORT_MAX 1024
+#endif
+// per cell: sort members ascending (deterministic order), then fill the per-slot arrays
+__device__ inline void grid_sort_fill_block(const FillArgs& f, int cell) {
+  __shared__ int buf[SORT_MAX];
+  int* sorted_orig = f.sorted_orig;
+  int start = f.cell_start[cell], cnt = f.cell_count[cell];
+  if (cnt > SORT_MAX) return;  // flagged in k_cell_scan
+  int P = 1;
+  while (P < cnt) P <<= 1;
+  for (int t = threadIdx.x; t < P; t += blockDim.x) buf[t] = t < cnt ? sorted_orig[start + t] : 0x7fffffff;
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < P; t += blockDim.x) {
+        int u = t ^ j;
+        if (u > t) {
+          bool up = (t & k) == 0;
+          int a = buf[t], b = buf[u];
+          if ((a > b) == up) {
+            buf[t] = b;
+            buf[u] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+    int o = buf[t];
+    int s = start + t;
+    sorted_orig[s] = o;
+    f.orig_sorted[o] = s;
+    for (int d = 0; d < 3; ++d) f.pos_build[3 * o + d] = f.pos[3 * o + d];
+    pack_atom(f.pos[3 * o], f.pos[3 * o + 1], f.pos[3 * o + 2], f.qs ? f.qs[o] : 0.f, f.L[0], f.L[1], f.L[2], &f.xq[s],
+              &f.pos_sorted[3 * s]);
+    if (f.lj64) {
+      double e = f.lj64[4 * o + 0], sg = f.lj64[4 * o + 1], e14 = f.lj64[4 * o + 2], s14 = f.lj64[4 * o + 3];
+      f.ljs[s] = make_float2((float)(0.5 * sg), (float)sqrt(e));
+      f.ljs14[s] = make_float2((float)(0.5 * s14), (float)sqrt(e14));
+    }
+    if (f.ref_lin) {
+      int lin = f.ref_lin[o];
+      int cz = lin % f.rnz, cy = (lin / f.rnz) % f.rny, cx = lin / (f.rnz * f.rny);
+      f.ref_cell_sorted[s] = (cx << 20) | (cy << 10) | cz;
+    }
+  }
+}
+
+
+
 // lanes cooperating on one atom's neighbour list (nonbonded kernel and list build)
 #define NB_G 32
 
@@ -283,7 +365,7 @@ int mdx_enqueue_request_rebuild(mdx_ctx* c);
 int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy);
 int mdx_enqueue_coulomb_plain(mdx_ctx* c);
 int mdx_pme_enqueue(mdx_ctx* c, bool energy);
-int mdx_bonded_enqueue(mdx_ctx* c, unsigned terms);
+int mdx_bonded_enqueue(mdx_ctx* c, unsigned terms, const FillArgs* fill = nullptr);
 int mdx_enqueue_zero_accumulators(mdx_ctx* c);
 int mdx_enqueue_forces_out(mdx_ctx* c, double* d_out);
 int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy);
@@ -291,5 +373,6 @@ int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy);
 int mdx_mlp_enqueue(mdx_ctx* c, bool energy);
 // integrator
 int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy, bool fuse_prev_b, bool emit_b);
+FillArgs fill_args(mdx_ctx* c, double thr2);
 int mdx_integrator_prime(mdx_ctx* c, unsigned terms);
 int mdx_integrator_finish(mdx_ctx* c);

@@ -15,7 +15,6 @@
 #include <cmath>
 
 #define NL_BLOCK 256
-#define SORT_MAX 1024
 
 // ============================================================== wrap / reference binning
 __global__ void k_wrap(double* __restrict__ pos, int n, double Lx, double Ly, double Lz, double ix,
@@ -226,10 +225,6 @@ extern "C" int mdx_celllist_table(mdx_ctx* c, int* d_table, int maxpc) {
 }
 
 // ============================================================== neighbour-list grid
-// decision shared by every rebuild kernel: rebuild requested, or an atom moved > skin/2
-__device__ __forceinline__ bool need_rebuild(const DevState* st, double thr2) {
-  return st->rebuild != 0 || dec_double_order(st->max_disp2_bits) > thr2;
-}
 
 __global__ void k_check_disp(const double* __restrict__ pos, const double* __restrict__ pos_build, int n,
                              double Lx, double Ly, double Lz, DevState* st) {
@@ -326,58 +321,37 @@ __global__ void k_grid_scatter(const int* __restrict__ atom_cell, int n, const i
   sorted_orig[cell_start[c] + slot] = i;
 }
 
-// per cell: sort members ascending (deterministic order), then fill the per-slot arrays
-__global__ void k_grid_sort_fill(const int* __restrict__ cell_start, const int* __restrict__ cell_count,
-                                 int* __restrict__ sorted_orig, int* __restrict__ orig_sorted,
-                                 const double* __restrict__ pos, double* __restrict__ pos_build,
-                                 const double* __restrict__ lj64, float2* __restrict__ ljs,
-                                 float2* __restrict__ ljs14, const int* __restrict__ ref_lin,
-                                 int* __restrict__ ref_cell_sorted, int rny, int rnz, const DevState* st,
-                                 double thr2, const float* __restrict__ qs, float4* __restrict__ xq,
-                                 double* __restrict__ pos_sorted, double Lx, double Ly, double Lz) {
-  if (!need_rebuild(st, thr2)) return;
-  __shared__ int buf[SORT_MAX];
-  int cell = blockIdx.x;
-  int start = cell_start[cell], cnt = cell_count[cell];
-  if (cnt > SORT_MAX) return;  // flagged in k_cell_scan
-  int P = 1;
-  while (P < cnt) P <<= 1;
-  for (int t = threadIdx.x; t < P; t += blockDim.x) buf[t] = t < cnt ? sorted_orig[start + t] : 0x7fffffff;
-  __syncthreads();
-  for (int k = 2; k <= P; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int t = threadIdx.x; t < P; t += blockDim.x) {
-        int u = t ^ j;
-        if (u > t) {
-          bool up = (t & k) == 0;
-          int a = buf[t], b = buf[u];
-          if ((a > b) == up) {
-            buf[t] = b;
-            buf[u] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-  for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-    int o = buf[t];
-    int s = start + t;
-    sorted_orig[s] = o;
-    orig_sorted[o] = s;
-    for (int d = 0; d < 3; ++d) pos_build[3 * o + d] = pos[3 * o + d];
-    pack_atom(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], qs ? qs[o] : 0.f, Lx, Ly, Lz, &xq[s], &pos_sorted[3 * s]);
-    if (lj64) {
-      double e = lj64[4 * o + 0], sg = lj64[4 * o + 1], e14 = lj64[4 * o + 2], s14 = lj64[4 * o + 3];
-      ljs[s] = make_float2((float)(0.5 * sg), (float)sqrt(e));
-      ljs14[s] = make_float2((float)(0.5 * s14), (float)sqrt(e14));
-    }
-    if (ref_lin) {
-      int lin = ref_lin[o];
-      int cz = lin % rnz, cy = (lin / rnz) % rny, cx = lin / (rnz * rny);
-      ref_cell_sorted[s] = (cx << 20) | (cy << 10) | cz;
-    }
-  }
+__global__ void k_grid_sort_fill(FillArgs f) {
+  if (!need_rebuild(f.st, f.thr2)) return;
+  grid_sort_fill_block(f, blockIdx.x);
+}
+
+FillArgs fill_args(mdx_ctx* c, double thr2) {
+  FillArgs f;
+  f.cell_start = c->cell_start.p;
+  f.cell_count = c->cell_count.p;
+  f.sorted_orig = c->sorted_orig.p;
+  f.orig_sorted = c->orig_sorted.p;
+  f.pos = c->pos;
+  f.pos_build = c->pos_at_build.p;
+  f.lj64 = c->has_lj ? c->lj64.p : nullptr;
+  f.ljs = c->ljs.p;
+  f.ljs14 = c->ljs14.p;
+  const bool ref = c->pair_mode == MDX_PAIRS_REFERENCE;
+  f.ref_lin = ref ? c->ref_cell_lin.p : nullptr;
+  f.ref_cell_sorted = c->ref_cell_sorted.p;
+  f.rny = c->ref_n[1];
+  f.rnz = c->ref_n[2];
+  f.st = c->dstate.p;
+  f.thr2 = thr2;
+  f.qs = c->has_charges ? c->charge_s.p : nullptr;
+  f.xq = c->xq.p;
+  f.pos_sorted = c->pos_sorted.p;
+  f.L[0] = c->box[0];
+  f.L[1] = c->box[1];
+  f.L[2] = c->box[2];
+  f.ncell = c->ncell;
+  return f;
 }
 
 // every evaluation: float32 positions (wrapped, centred) and scaled charges in sorted order,
@@ -810,12 +784,15 @@ int mdx_nlist_update(mdx_ctx* c, bool force_rebuild) {
     MDX_LAUNCH_CHECK();
   }
   bool ref = c->pair_mode == MDX_PAIRS_REFERENCE;
-  k_grid_sort_fill<<<c->ncell, 256, 0, c->stream>>>(
-      c->cell_start.p, c->cell_count.p, c->sorted_orig.p, c->orig_sorted.p, c->pos, c->pos_at_build.p,
-      c->has_lj ? c->lj64.p : nullptr, c->ljs.p, c->ljs14.p, ref ? c->ref_cell_lin.p : nullptr,
-      c->ref_cell_sorted.p, c->ref_n[1], c->ref_n[2], st, thr2, c->has_charges ? c->charge_s.p : nullptr, c->xq.p,
-      c->pos_sorted.p, c->box[0], c->box[1], c->box[2]);
-  MDX_LAUNCH_CHECK();
+  const FillArgs fa = fill_args(c, thr2);
+  if (c->fill_bonded_terms) {  // the bonded terms of this evaluation share the fill launch
+    int rc0 = mdx_bonded_enqueue(c, c->fill_bonded_terms, &fa);
+    if (rc0) return rc0;
+    c->bonded_done = true;
+  } else {
+    k_grid_sort_fill<<<c->ncell, 256, 0, c->stream>>>(fa);
+    MDX_LAUNCH_CHECK();
+  }
   int rc = pro ? 0 : mdx_nlist_enqueue_pack(c);
   if (rc) return rc;
   ListArgs a;
