@@ -307,27 +307,36 @@ __device__ void last_block_bin(const StepPro& P, DevState* st) {
   extern __shared__ int pro_lds[];
   int* cnt = pro_lds;
   int* start = pro_lds + P.ncell;
-  __shared__ int part[PRO_THREADS];
-  for (int k = threadIdx.x; k < P.ncell; k += blockDim.x) cnt[k] = 0;
+  __shared__ int wsum[PRO_THREADS / 64];
+  constexpr int U = 8;  // independent loads in flight per lane
+  const int tid = threadIdx.x, bs = blockDim.x;
+  for (int k = tid; k < P.ncell; k += bs) cnt[k] = 0;
   __syncthreads();
-  for (int j = threadIdx.x; j < P.n; j += blockDim.x) atomicAdd(&cnt[P.atom_cell[j]], 1);
-  __syncthreads();
-  const int per = (P.ncell + blockDim.x - 1) / blockDim.x;
-  const int b = threadIdx.x * per, e = min(P.ncell, b + per);
-  int sum = 0;
-  for (int k = b; k < e; ++k) sum += cnt[k];
-  part[threadIdx.x] = sum;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int run = 0;
-    for (int t = 0; t < (int)blockDim.x; ++t) {
-      const int v = part[t];
-      part[t] = run;
-      run += v;
-    }
+  for (int j0 = tid; j0 < P.n; j0 += U * bs) {
+    int c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = j0 + u * bs < P.n ? P.atom_cell[j0 + u * bs] : -1;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (c[u] >= 0) atomicAdd(&cnt[c[u]], 1);
   }
   __syncthreads();
-  int run = part[threadIdx.x];
+  // exclusive scan: per-lane chunk sums, wave-level inclusive scan, then the wave totals
+  const int per = (P.ncell + bs - 1) / bs;
+  const int b = tid * per, e = min(P.ncell, b + per);
+  int sum = 0;
+  for (int k = b; k < e; ++k) sum += cnt[k];
+  int incl = sum;
+  const int lane = tid & 63;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
+  }
+  if (lane == 63) wsum[tid >> 6] = incl;
+  __syncthreads();
+  int wbase = 0;
+  for (int w = 0; w < (tid >> 6); ++w) wbase += wsum[w];
+  int run = wbase + incl - sum;
   bool over = false;
   for (int k = b; k < e; ++k) {
     const int v = cnt[k];
@@ -336,13 +345,18 @@ __device__ void last_block_bin(const StepPro& P, DevState* st) {
     P.cell_start[k] = run;
     P.cell_count[k] = v;
     run += v;
-    cnt[k] = 0;  // becomes the scatter cursor
   }
   if (over) atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
   __syncthreads();
-  for (int j = threadIdx.x; j < P.n; j += blockDim.x) {
-    const int c = P.atom_cell[j];
-    P.sorted_orig[start[c] + atomicAdd(&cnt[c], 1)] = j;
+  for (int k = tid; k < P.ncell; k += bs) cnt[k] = 0;  // scatter cursors
+  __syncthreads();
+  for (int j0 = tid; j0 < P.n; j0 += U * bs) {
+    int c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = j0 + u * bs < P.n ? P.atom_cell[j0 + u * bs] : -1;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (c[u] >= 0) P.sorted_orig[start[c[u]] + atomicAdd(&cnt[c[u]], 1)] = j0 + u * bs;
   }
 }
 
