@@ -150,14 +150,20 @@ __device__ __forceinline__ void stage_neighbours(const MlAtoms& a, const MlConst
 }
 
 // ---------------------------------------------------------------- descriptors
-// One wave per atom; lane f owns radial feature f (element f/32, Gaussian f%32) and moment
-// components lane, lane+64, lane+128 (< 160) of the atom's moment tensors.
+// One wave per atom. Each chunk of 64 list neighbours is first evaluated with one lane per
+// neighbour: the in-range ones (r < r_d) are compacted in list order into LDS tables of their 32
+// radial Gaussians, 4 angular channel weights (all times the cutoff function) and 20 unit-vector
+// monomials. Then lane f accumulates radial feature f (element f/32, Gaussian f%32) and moment
+// components lane, lane+64, lane+128 (< 160) over the chunk's table rows in order.
 __global__ __launch_bounds__(64) void k_ml_desc(MlAtoms a, MlConst k, const float4* __restrict__ xq,
                                                const int* __restrict__ sorted_orig, const int* __restrict__ nbr_count,
                                                const unsigned* __restrict__ nbr_list, const int* __restrict__ row_of,
                                                const float* __restrict__ mean, const float* __restrict__ rstd,
                                                float* __restrict__ G, float* __restrict__ T, bf16_t* __restrict__ X) {
-  __shared__ NbStage S;
+  __shared__ float gr[64][ML_RAD + 1];
+  __shared__ float ga[64][ML_CH];
+  __shared__ float mo[64][ML_MOM + 1];
+  __shared__ int ge[64];
   __shared__ float sm[ML_T];
   const int s = blockIdx.x;
   const int lane = threadIdx.x;
@@ -166,41 +172,92 @@ __global__ __launch_bounds__(64) void k_ml_desc(MlAtoms a, MlConst k, const floa
   const int row = row_of[o];
   const int cnt = nbr_count[s];
   const unsigned* lrow = nbr_list + (long long)s * a.cap;
-  const int fe = lane >> 5;
-  const float mu = k.rad_mu0 + (float)(lane & 31) * k.rad_dmu;
+  const int fe = lane >> 5, fk = lane & 31;
   float rad = 0.f;
   float mom[3] = {0.f, 0.f, 0.f};
-  int me[3], mch[3], ma[3], mb[3], mc[3];
-  float mnu[3];
+  int me[3], mch[3], mcomp[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const int c = lane + 64 * q;
     const int cc = c < ML_T ? c : 0;
     me[q] = c < ML_T ? cc / 80 : -1;
     mch[q] = (cc % 80) / 20;
-    mono_exps(cc % 20, &ma[q], &mb[q], &mc[q]);
-    mnu[q] = k.ang_nu[mch[q]];
+    mcomp[q] = cc % 20;
   }
   for (int base = 0; base < cnt; base += 64) {
-    stage_neighbours(a, k, xi, xq, lrow, cnt, base, S);
-    __syncthreads();
-    const int m = min(64, cnt - base);
-    for (int jj = 0; jj < m; ++jj) {
-      const float r = S.r[jj];
-      if (r >= k.rd) continue;  // uniform over the wave
-      const int e = S.e[jj];
-      const float ux = S.ux[jj], uy = S.uy[jj], uz = S.uz[jj];
+    // lane = neighbour: geometry, in-range test, compaction
+    const int kk = base + lane;
+    float ux = 0.f, uy = 0.f, uz = 0.f, r = 1e30f;
+    int e = 0;
+    if (kk < cnt) {
+      const int t = (int)(lrow[kk] & 0x3fffffffu);
+      const float4 xj = xq[t];
+      float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+      dx -= a.Lx * rintf(dx * a.ix);
+      dy -= a.Ly * rintf(dy * a.iy);
+      dz -= a.Lz * rintf(dz * a.iz);
+      const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
+      if (rr < k.rd && rr > 0.f) {
+        r = rr;
+        const float ir = __builtin_amdgcn_rcpf(rr);
+        ux = dx * ir;
+        uy = dy * ir;
+        uz = dz * ir;
+      }
+      e = (int)xj.w;
+    }
+    const bool keep = r < k.rd;
+    const unsigned long long ball = __ballot(keep);
+    const int slot = __popcll(ball & ((1ull << lane) - 1ull));
+    const int m = __popcll(ball);
+    if (keep) {
       float fc, dfc;
       cutoff_fc(k, r, &fc, &dfc);
-      const float dr = r - mu;
-      if (e == fe) rad += __expf(-k.rad_eta * dr * dr) * fc;
+#pragma unroll 8
+      for (int q = 0; q < ML_RAD; ++q) {
+        const float dr = r - (k.rad_mu0 + (float)q * k.rad_dmu);
+        gr[slot][q] = __expf(-k.rad_eta * dr * dr) * fc;
+      }
+#pragma unroll
+      for (int ch = 0; ch < ML_CH; ++ch) {
+        const float dn = r - k.ang_nu[ch];
+        ga[slot][ch] = __expf(-k.ang_eta * dn * dn) * fc;
+      }
+      const float xx = ux * ux, yy = uy * uy, zz = uz * uz;
+      float* mm = mo[slot];
+      mm[0] = 1.f;
+      mm[1] = ux;
+      mm[2] = uy;
+      mm[3] = uz;
+      mm[4] = xx;
+      mm[5] = ux * uy;
+      mm[6] = ux * uz;
+      mm[7] = yy;
+      mm[8] = uy * uz;
+      mm[9] = zz;
+      mm[10] = xx * ux;
+      mm[11] = xx * uy;
+      mm[12] = xx * uz;
+      mm[13] = ux * yy;
+      mm[14] = ux * uy * uz;
+      mm[15] = ux * zz;
+      mm[16] = yy * uy;
+      mm[17] = yy * uz;
+      mm[18] = uy * zz;
+      mm[19] = zz * uz;
+      ge[slot] = e;
+    }
+    __syncthreads();
+    // lane = feature: accumulate over the chunk's in-range neighbours, in list order
+#pragma unroll 4
+    for (int jj = 0; jj < m; ++jj) {
+      const int ej = ge[jj];
+      const float g = gr[jj][fk];
+      rad += ej == fe ? g : 0.f;
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        if (me[q] == e) {
-          const float dn = r - mnu[q];
-          const float g = __expf(-k.ang_eta * dn * dn) * fc;
-          mom[q] += g * pw3(ux, ma[q]) * pw3(uy, mb[q]) * pw3(uz, mc[q]);
-        }
+        const float w = ga[jj][mch[q]] * mo[jj][mcomp[q]];
+        mom[q] += me[q] == ej ? w : 0.f;
       }
     }
     __syncthreads();
@@ -216,7 +273,7 @@ __global__ __launch_bounds__(64) void k_ml_desc(MlAtoms a, MlConst k, const floa
   __syncthreads();
   // features: radial f = e*48 + k (lane < 64), angular f = e*48 + 32 + ch*4 + l (lanes 0..31)
   {
-    const int f = fe * ML_PER + (lane & 31);
+    const int f = fe * ML_PER + fk;
     G[(long long)row * ML_F + f] = rad;
     X[(long long)row * ML_F + f] = f2bf((rad - mean[f]) * rstd[f]);
   }
@@ -269,7 +326,10 @@ __device__ __forceinline__ void pair_force(const MlConst& k, const float* gG, co
 }
 
 // ---------------------------------------------------------------- forces
-// One wave per atom i; lane per neighbour j: v = dE_i/dr_ij, F_j -= v (atomic), F_i += v.
+// One wave per atom i. The in-range neighbours (r < r_d) of the list are first compacted into LDS
+// in list order, then evaluated one lane per neighbour: v = dE_i/dr_ij, F_j -= v (atomic),
+// F_i += v. Compaction keeps every lane busy (about a third of the list is inside r_d).
+#define MLF_MAX 256
 __global__ __launch_bounds__(64) void k_ml_force(MlAtoms a, MlConst k, const float4* __restrict__ xq,
                                                 const int* __restrict__ sorted_orig, const int* __restrict__ nbr_count,
                                                 const unsigned* __restrict__ nbr_list, const int* __restrict__ row_of,
@@ -277,6 +337,8 @@ __global__ __launch_bounds__(64) void k_ml_force(MlAtoms a, MlConst k, const flo
                                                 const float* __restrict__ T, long long* __restrict__ ffix) {
   __shared__ float gG[ML_F];
   __shared__ float tt[ML_T];
+  __shared__ float4 nb[MLF_MAX];  // unit vector, r
+  __shared__ int nt[MLF_MAX];     // neighbour slot | element << 30
   const int s = blockIdx.x;
   const int lane = threadIdx.x;
   const float4 xi = xq[s];
@@ -284,29 +346,56 @@ __global__ __launch_bounds__(64) void k_ml_force(MlAtoms a, MlConst k, const flo
   const int row = row_of[o];
   for (int f = lane; f < ML_F; f += 64) gG[f] = GX[(long long)row * ML_F + f] * rstd[f];
   for (int c = lane; c < ML_T; c += 64) tt[c] = T[(long long)row * ML_T + c];
-  __syncthreads();
   const int cnt = nbr_count[s];
   const unsigned* lrow = nbr_list + (long long)s * a.cap;
   float fx = 0.f, fy = 0.f, fz = 0.f;
-  for (int kk = lane; kk < cnt; kk += 64) {
-    const int t = (int)(lrow[kk] & 0x3fffffffu);
-    const float4 xj = xq[t];
-    float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-    dx -= a.Lx * rintf(dx * a.ix);
-    dy -= a.Ly * rintf(dy * a.iy);
-    dz -= a.Lz * rintf(dz * a.iz);
-    const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-    if (r >= k.rd || r <= 0.f) continue;
-    const float ir = __builtin_amdgcn_rcpf(r);
-    float vx, vy, vz;
-    pair_force(k, gG, tt, (int)xj.w, dx * ir, dy * ir, dz * ir, r, &vx, &vy, &vz);
-    const int oj = sorted_orig[t];
-    atomicAdd((unsigned long long*)&ffix[oj], (unsigned long long)to_fix_f(-vx));
-    atomicAdd((unsigned long long*)&ffix[a.n + oj], (unsigned long long)to_fix_f(-vy));
-    atomicAdd((unsigned long long*)&ffix[2LL * a.n + oj], (unsigned long long)to_fix_f(-vz));
-    fx += vx;
-    fy += vy;
-    fz += vz;
+  int done = 0;  // list entries consumed
+  while (done < cnt) {
+    // compact up to MLF_MAX in-range neighbours, in list order
+    int m = 0;
+    while (done < cnt && m + 64 <= MLF_MAX) {
+      const int kk = done + lane;
+      bool keep = false;
+      float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
+      int te = 0;
+      if (kk < cnt) {
+        const int t = (int)(lrow[kk] & 0x3fffffffu);
+        const float4 xj = xq[t];
+        float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        dx -= a.Lx * rintf(dx * a.ix);
+        dy -= a.Ly * rintf(dy * a.iy);
+        dz -= a.Lz * rintf(dz * a.iz);
+        const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+        keep = r < k.rd && r > 0.f;
+        const float ir = __builtin_amdgcn_rcpf(r);
+        u = make_float4(dx * ir, dy * ir, dz * ir, r);
+        te = t | ((int)xj.w << 30);
+      }
+      const unsigned long long ball = __ballot(keep);
+      if (keep) {
+        const int slot = m + __popcll(ball & ((1ull << lane) - 1ull));
+        nb[slot] = u;
+        nt[slot] = te;
+      }
+      m += __popcll(ball);
+      done += 64;
+    }
+    __syncthreads();
+    for (int j = lane; j < m; j += 64) {
+      const float4 u = nb[j];
+      const int te = nt[j];
+      const int t = te & 0x3fffffff, e = (int)((unsigned)te >> 30);
+      float vx, vy, vz;
+      pair_force(k, gG, tt, e, u.x, u.y, u.z, u.w, &vx, &vy, &vz);
+      const int oj = sorted_orig[t];
+      atomicAdd((unsigned long long*)&ffix[oj], (unsigned long long)to_fix_f(-vx));
+      atomicAdd((unsigned long long*)&ffix[a.n + oj], (unsigned long long)to_fix_f(-vy));
+      atomicAdd((unsigned long long*)&ffix[2LL * a.n + oj], (unsigned long long)to_fix_f(-vz));
+      fx += vx;
+      fy += vy;
+      fz += vz;
+    }
+    __syncthreads();
   }
   for (int off = 32; off > 0; off >>= 1) {
     fx += __shfl_xor(fx, off, 64);
@@ -414,9 +503,35 @@ struct GemmArgs {
   float* DD;                  // EPI_BIAS_SILU: writes SiLU''(z) when not null
 };
 #define GK 32
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// 8 consecutive bf16 of a tile row from global memory (16 B when all 8 are in range and the
+// address is 16-byte aligned, element by element with zero fill otherwise)
+__device__ __forceinline__ void load8(const bf16_t* __restrict__ base, long long off, long long step, int nvalid,
+                                      bf16_t* out) {
+  const bf16_t* p = base + off;
+  if (nvalid >= 8 && step == 1 && (((unsigned long long)p) & 15ull) == 0) {
+    const u32x4 v = *(const u32x4*)p;
+    const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      out[2 * i] = (bf16_t)(w[i] & 0xffffu);
+      out[2 * i + 1] = (bf16_t)(w[i] >> 16);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) out[i] = i < nvalid ? p[(long long)i * step] : (bf16_t)0;
+  }
+}
+
+// A_KC: A's k index is contiguous in memory (a_k == 1), else its row index is (a_m == 1);
+// B_KC: B's k index is contiguous (b_k == 1), else its column index is (b_n == 1). Each k-step
+// every lane moves one 8-element vector of A and one of B into LDS, where both tiles are
+// stored k-contiguous per row / column so each MFMA operand is one 8-byte LDS read.
+template <bool A_KC, bool B_KC>
 __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g, int epi) {
-  __shared__ bf16_t As[ML_TILE][GK + 8];
-  __shared__ bf16_t Bs[GK][ML_TILE + 8];
+  __shared__ __attribute__((aligned(16))) bf16_t As[ML_TILE][GK + 8];
+  __shared__ __attribute__((aligned(16))) bf16_t Bs[ML_TILE][GK + 8];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
   const int row0 = blockIdx.y * ML_TILE, col0 = blockIdx.x * ML_TILE;
@@ -426,27 +541,44 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g, int epi) {
   f32x16 acc;
   for (int i = 0; i < 16; ++i) acc[i] = 0.f;
   for (int k0 = kb; k0 < ke; k0 += GK) {
-    for (int e = tid; e < ML_TILE * GK; e += 256) {
-      const int r = e / GK, kk = e % GK;
-      const int m = row0 + r, kx = k0 + kk;
-      As[r][kk] = (m < g.M && kx < ke) ? g.A[(long long)m * g.a_m + (long long)kx * g.a_k] : (bf16_t)0;
+    bf16_t va[8], vb[8];
+    if (A_KC) {  // row r, k k0+kv..+7
+      const int r = tid >> 2, kv = (tid & 3) * 8;
+      const int m = row0 + r, kx = k0 + kv;
+      const int nv = m < g.M ? max(0, min(8, ke - kx)) : 0;
+      load8(g.A, (long long)m * g.a_m + kx, 1, nv, va);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) As[r][kv + i] = va[i];
+    } else {     // k k0+kk, rows mv..mv+7
+      const int kk = tid >> 3, mv = (tid & 7) * 8;
+      const int m = row0 + mv, kx = k0 + kk;
+      const int nv = kx < ke ? max(0, min(8, g.M - m)) : 0;
+      load8(g.A, (long long)kx * g.a_k + m, 1, nv, va);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) As[mv + i][kk] = va[i];
     }
-    for (int e = tid; e < GK * ML_TILE; e += 256) {
-      const int kk = e / ML_TILE, c = e % ML_TILE;
-      const int n = col0 + c, kx = k0 + kk;
-      Bs[kk][c] = (n < g.N && kx < ke) ? B[(long long)kx * g.b_k + (long long)n * g.b_n] : (bf16_t)0;
+    if (B_KC) {  // column c, k k0+kv..+7
+      const int c = tid >> 2, kv = (tid & 3) * 8;
+      const int n = col0 + c, kx = k0 + kv;
+      const int nv = n < g.N ? max(0, min(8, ke - kx)) : 0;
+      load8(B, (long long)n * g.b_n + kx, 1, nv, vb);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) Bs[c][kv + i] = vb[i];
+    } else {     // k k0+kk, columns nv..nv+7
+      const int kk = tid >> 3, nc = (tid & 7) * 8;
+      const int n = col0 + nc, kx = k0 + kk;
+      const int nv = kx < ke ? max(0, min(8, g.N - n)) : 0;
+      load8(B, (long long)kx * g.b_k + n, 1, nv, vb);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) Bs[nc + i][kk] = vb[i];
     }
     __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < GK; ks += 8) {
-      bf16x4 av, bv;
       const int ar = wm * 32 + (lane & 31), kq = ks + (lane >> 5) * 4;
       const int bc = wn * 32 + (lane & 31);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        av[j] = (short)As[ar][kq + j];
-        bv[j] = (short)Bs[kq + j][bc];
-      }
+      const bf16x4 av = *(const bf16x4*)&As[ar][kq];
+      const bf16x4 bv = *(const bf16x4*)&Bs[bc][kq];
       acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
     }
     __syncthreads();
@@ -466,7 +598,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g, int epi) {
       const float z = v + (second ? g.bias1 : g.bias0)[n];
       const float sg = 1.f / (1.f + __expf(-z));
       const long long idx = (long long)m * g.ldc + n;
-      g.C[idx] = z;
+      if (g.C) g.C[idx] = z;
       g.H[idx] = f2bf(z * sg);
       g.S[idx] = sg * (1.f + z * (1.f - sg));
       if (g.DD) g.DD[idx] = sg * (1.f - sg) * (2.f + z * (1.f - 2.f * sg));
@@ -533,13 +665,24 @@ __global__ void k_ml_top(int rows, int row_split, const float* __restrict__ a3, 
   dz16[i] = f2bf(v);
   dz32[i] = v;
 }
-// out[c] (+)= sum over rows [r0, r1) of in[row][c] (fixed order), one lane per column
-__global__ void k_ml_colsum(const float* __restrict__ in, int r0, int r1, int ncol, float* __restrict__ out, int acc) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+// column sums over rows [r0, r1) of in[row][c] in a fixed order: block b sums its chunk of rows
+// (one lane per column, coalesced), then one lane per column adds the chunk sums in order
+#define COLSUM_CHUNKS 128
+__global__ void k_ml_colsum_part(const float* __restrict__ in, int r0, int r1, int ncol, float* __restrict__ part) {
+  const int c = threadIdx.x;
+  const int len = (r1 - r0 + COLSUM_CHUNKS - 1) / COLSUM_CHUNKS;
+  const int a = r0 + blockIdx.x * len, b = min(r1, a + len);
   if (c >= ncol) return;
   float s = 0.f;
-  for (int r = r0; r < r1; ++r) s += in[(long long)r * ncol + c];
-  out[c] = acc ? out[c] + s : s;
+  for (int r = a; r < b; ++r) s += in[(long long)r * ncol + c];
+  part[blockIdx.x * ncol + c] = s;
+}
+__global__ void k_ml_colsum_fin(const float* __restrict__ part, int ncol, float* __restrict__ out) {
+  const int c = threadIdx.x;
+  if (c >= ncol) return;
+  float s = 0.f;
+  for (int k = 0; k < COLSUM_CHUNKS; ++k) s += part[k * ncol + c];
+  out[c] = s;
 }
 // sum of split-K partials [z][rows][cols] into out (fixed order)
 __global__ void k_ml_reduce_splits(const float* __restrict__ part, int nz, long long count, float* __restrict__ out) {
@@ -626,7 +769,7 @@ struct mdx_mlp {
   MlNet net[2];
   MlConst k;
   // activations (rows x width)
-  DevBuf<float> G, T, GX, Z1, Z2, Z3, S1, S2, S3, D1, D2, D3, HB1, HB2, A1, A2, A3, C1, C2, T4, DZ1, DZ2, DZ3, DH1, DH2;
+  DevBuf<float> G, T, GX, S1, S2, S3, D1, D2, D3, HB1, HB2, A1, A2, A3, C1, C2, T4, DZ1, DZ2, DZ3, DH1, DH2;
   DevBuf<bf16_t> X, H1, H2, H3, G1, G2, G3, U, B1, B2, DZ16_1, DZ16_2, DZ16_3;
   DevBuf<float> part;           // split-K partials
   DevBuf<double> w;             // loss weights (3N)
@@ -654,8 +797,7 @@ static int ml_alloc(mdx_mlp* m, bool training) {
   int rc;
   if (!m->G.p) {
     if ((rc = m->G.ensure(R * ML_F)) || (rc = m->T.ensure(R * ML_T)) || (rc = m->GX.ensure(R * ML_F)) ||
-        (rc = m->X.ensure(R * ML_F)) || (rc = m->Z1.ensure(R * ML_H)) || (rc = m->Z2.ensure(R * ML_H)) ||
-        (rc = m->Z3.ensure(R * ML_H)) || (rc = m->S1.ensure(R * ML_H)) || (rc = m->S2.ensure(R * ML_H)) ||
+        (rc = m->X.ensure(R * ML_F)) || (rc = m->S1.ensure(R * ML_H)) || (rc = m->S2.ensure(R * ML_H)) ||
         (rc = m->S3.ensure(R * ML_H)) || (rc = m->H1.ensure(R * ML_H)) || (rc = m->H2.ensure(R * ML_H)) ||
         (rc = m->H3.ensure(R * ML_H)) || (rc = m->G1.ensure(R * ML_H)) || (rc = m->G2.ensure(R * ML_H)) ||
         (rc = m->G3.ensure(R * ML_H)) || (rc = m->scal.ensure(2)))
@@ -778,7 +920,7 @@ extern "C" int mdx_mlp_destroy(mdx_mlp* m) {
   hipStreamSynchronize(m->host->stream);
   if (m->host->mlp == m) m->host->mlp = nullptr;
   mdx_ctx_destroy(m->nl);
-  for (DevBuf<float>* b : {&m->mean, &m->rstd, &m->G, &m->T, &m->GX, &m->Z1, &m->Z2, &m->Z3, &m->S1, &m->S2, &m->S3,
+  for (DevBuf<float>* b : {&m->mean, &m->rstd, &m->G, &m->T, &m->GX, &m->S1, &m->S2, &m->S3,
                            &m->D1, &m->D2, &m->D3, &m->HB1, &m->HB2, &m->A1, &m->A2, &m->A3, &m->C1, &m->C2, &m->T4,
                            &m->DZ1, &m->DZ2, &m->DZ3, &m->DH1, &m->DH2, &m->part})
     b->release();
@@ -877,7 +1019,13 @@ static int gemm(mdx_mlp* m, int epi, int M, int N, int K, const bf16_t* A, long 
   g.S = S;
   g.DD = DD;
   dim3 grid(div_up(N, ML_TILE), div_up(M, ML_TILE), splits);
-  k_gemm_bf16<<<grid, 256, 0, m->host->stream>>>(g, epi);
+  MDX_CHECK((a_k == 1 || a_m == 1) && (b_k == 1 || b_n == 1), MDX_ERR_INVALID,
+            "gemm: one stride of each operand must be 1");
+  hipStream_t st = m->host->stream;
+  if (a_k == 1 && b_k == 1) k_gemm_bf16<true, true><<<grid, 256, 0, st>>>(g, epi);
+  else if (a_k == 1) k_gemm_bf16<true, false><<<grid, 256, 0, st>>>(g, epi);
+  else if (b_k == 1) k_gemm_bf16<false, true><<<grid, 256, 0, st>>>(g, epi);
+  else k_gemm_bf16<false, false><<<grid, 256, 0, st>>>(g, epi);
   MDX_LAUNCH_CHECK();
   return 0;
 }
@@ -896,11 +1044,11 @@ static int ml_forward_backward(mdx_mlp* m, long long* eslot, bool keep_training_
   float* DD1 = keep_training_state ? m->D1.p : nullptr;
   float* DD2 = keep_training_state ? m->D2.p : nullptr;
   float* DD3 = keep_training_state ? m->D3.p : nullptr;
-  if ((rc = gemm(m, EPI_BIAS_SILU, R, ML_H, ML_F, m->X.p, ML_F, 1, N0.w1.p, N1.w1.p, ML_H, 1, rs, m->Z1.p, ML_H, 1,
+  if ((rc = gemm(m, EPI_BIAS_SILU, R, ML_H, ML_F, m->X.p, ML_F, 1, N0.w1.p, N1.w1.p, ML_H, 1, rs, nullptr, ML_H, 1,
                  N0.p.p + P_B1, N1.p.p + P_B1, m->H1.p, m->S1.p, DD1)) ||
-      (rc = gemm(m, EPI_BIAS_SILU, R, ML_H, ML_H, m->H1.p, ML_H, 1, N0.w2.p, N1.w2.p, ML_H, 1, rs, m->Z2.p, ML_H, 1,
+      (rc = gemm(m, EPI_BIAS_SILU, R, ML_H, ML_H, m->H1.p, ML_H, 1, N0.w2.p, N1.w2.p, ML_H, 1, rs, nullptr, ML_H, 1,
                  N0.p.p + P_B2, N1.p.p + P_B2, m->H2.p, m->S2.p, DD2)) ||
-      (rc = gemm(m, EPI_BIAS_SILU, R, ML_H, ML_H, m->H2.p, ML_H, 1, N0.w3.p, N1.w3.p, ML_H, 1, rs, m->Z3.p, ML_H, 1,
+      (rc = gemm(m, EPI_BIAS_SILU, R, ML_H, ML_H, m->H2.p, ML_H, 1, N0.w3.p, N1.w3.p, ML_H, 1, rs, nullptr, ML_H, 1,
                  N0.p.p + P_B3, N1.p.p + P_B3, m->H3.p, m->S3.p, DD3)))
     return rc;
   k_ml_head<<<R, 64, 0, st>>>(R, rs, m->H3.p, m->S3.p, N0.p.p, N1.p.p, m->atom_of_row.p, m->G3.p, nullptr, eslot);
@@ -1029,10 +1177,13 @@ static int ml_weight_grads(mdx_mlp* m, int e, int r0, int r1) {
       (rc = wgrad(ML_H, ML_H, m->B1.p, m->G2.p, m->H1.p, m->DZ16_2.p, N.grad.p + P_W2)) ||
       (rc = wgrad(ML_H, ML_H, m->B2.p, m->G3.p, m->H2.p, m->DZ16_3.p, N.grad.p + P_W3)))
     return rc;
-  k_ml_colsum<<<1, ML_H, 0, st>>>(m->DZ1.p, r0, r1, ML_H, N.grad.p + P_B1, 0);
-  k_ml_colsum<<<1, ML_H, 0, st>>>(m->DZ2.p, r0, r1, ML_H, N.grad.p + P_B2, 0);
-  k_ml_colsum<<<1, ML_H, 0, st>>>(m->DZ3.p, r0, r1, ML_H, N.grad.p + P_B3, 0);
-  k_ml_colsum<<<1, ML_H, 0, st>>>(m->T4.p, r0, r1, ML_H, N.grad.p + P_W4, 0);
+  // bias and head gradients: column sums (the split-K partial buffer is free again here)
+  const float* cols_in[4] = {m->DZ1.p, m->DZ2.p, m->DZ3.p, m->T4.p};
+  const int cols_out[4] = {P_B1, P_B2, P_B3, P_W4};
+  for (int q = 0; q < 4; ++q) {
+    k_ml_colsum_part<<<COLSUM_CHUNKS, ML_H, 0, st>>>(cols_in[q], r0, r1, ML_H, m->part.p);
+    k_ml_colsum_fin<<<1, ML_H, 0, st>>>(m->part.p, ML_H, N.grad.p + cols_out[q]);
+  }
   k_ml_fill_f32<<<1, 64, 0, st>>>(1, N.grad.p + P_B4, 0.f);
   MDX_LAUNCH_CHECK();
   return 0;
