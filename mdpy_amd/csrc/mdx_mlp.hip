@@ -540,39 +540,37 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g, int epi) {
   const bf16_t* B = second ? g.B1 : g.B0;
   f32x16 acc;
   for (int i = 0; i < 16; ++i) acc[i] = 0.f;
-  for (int k0 = kb; k0 < ke; k0 += GK) {
-    bf16_t va[8], vb[8];
-    if (A_KC) {  // row r, k k0+kv..+7
-      const int r = tid >> 2, kv = (tid & 3) * 8;
-      const int m = row0 + r, kx = k0 + kv;
-      const int nv = m < g.M ? max(0, min(8, ke - kx)) : 0;
-      load8(g.A, (long long)m * g.a_m + kx, 1, nv, va);
-#pragma unroll
-      for (int i = 0; i < 8; ++i) As[r][kv + i] = va[i];
-    } else {     // k k0+kk, rows mv..mv+7
-      const int kk = tid >> 3, mv = (tid & 7) * 8;
-      const int m = row0 + mv, kx = k0 + kk;
-      const int nv = kx < ke ? max(0, min(8, g.M - m)) : 0;
-      load8(g.A, (long long)kx * g.a_k + m, 1, nv, va);
-#pragma unroll
-      for (int i = 0; i < 8; ++i) As[mv + i][kk] = va[i];
+  // this lane's tile vectors: A (row ra / k offset ka, or k ka / rows ra..+7), B likewise
+  const int ra = A_KC ? (tid >> 2) : (tid & 7) * 8, ka = A_KC ? (tid & 3) * 8 : (tid >> 3);
+  const int rb = B_KC ? (tid >> 2) : (tid & 7) * 8, kbv = B_KC ? (tid & 3) * 8 : (tid >> 3);
+  auto fetch = [&](int k0, bf16_t* va, bf16_t* vb) {
+    if (A_KC) {
+      const int m = row0 + ra, kx = k0 + ka;
+      load8(g.A, (long long)m * g.a_m + kx, 1, m < g.M ? max(0, min(8, ke - kx)) : 0, va);
+    } else {
+      const int m = row0 + ra, kx = k0 + ka;
+      load8(g.A, (long long)kx * g.a_k + m, 1, kx < ke ? max(0, min(8, g.M - m)) : 0, va);
     }
-    if (B_KC) {  // column c, k k0+kv..+7
-      const int c = tid >> 2, kv = (tid & 3) * 8;
-      const int n = col0 + c, kx = k0 + kv;
-      const int nv = n < g.N ? max(0, min(8, ke - kx)) : 0;
-      load8(B, (long long)n * g.b_n + kx, 1, nv, vb);
+    if (B_KC) {
+      const int n = col0 + rb, kx = k0 + kbv;
+      load8(B, (long long)n * g.b_n + kx, 1, n < g.N ? max(0, min(8, ke - kx)) : 0, vb);
+    } else {
+      const int n = col0 + rb, kx = k0 + kbv;
+      load8(B, (long long)kx * g.b_k + n, 1, kx < ke ? max(0, min(8, g.N - n)) : 0, vb);
+    }
+  };
+  bf16_t va[8], vb[8];
+  if (kb < ke) fetch(kb, va, vb);
+  for (int k0 = kb; k0 < ke; k0 += GK) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) Bs[c][kv + i] = vb[i];
-    } else {     // k k0+kk, columns nv..nv+7
-      const int kk = tid >> 3, nc = (tid & 7) * 8;
-      const int n = col0 + nc, kx = k0 + kk;
-      const int nv = kx < ke ? max(0, min(8, g.N - n)) : 0;
-      load8(B, (long long)kx * g.b_k + n, 1, nv, vb);
-#pragma unroll
-      for (int i = 0; i < 8; ++i) Bs[nc + i][kk] = vb[i];
+    for (int i = 0; i < 8; ++i) {
+      if (A_KC) As[ra][ka + i] = va[i];
+      else As[ra + i][ka] = va[i];
+      if (B_KC) Bs[rb][kbv + i] = vb[i];
+      else Bs[rb + i][kbv] = vb[i];
     }
     __syncthreads();
+    if (k0 + GK < ke) fetch(k0 + GK, va, vb);  // next step's loads in flight during the MFMAs
 #pragma unroll
     for (int ks = 0; ks < GK; ks += 8) {
       const int ar = wm * 32 + (lane & 31), kq = ks + (lane >> 5) * 4;
@@ -611,27 +609,47 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g, int epi) {
 }
 
 // ---------------------------------------------------------------- elementwise / reductions
-// per row: E_i = H3 . w4 + b4, G3 = w4 * S3 (bf16); one wave per row
-__global__ __launch_bounds__(64) void k_ml_head(int rows, int row_split, const bf16_t* __restrict__ H3,
-                                               const float* __restrict__ S3, const float* __restrict__ p0,
-                                               const float* __restrict__ p1, const int* __restrict__ atom_of_row,
-                                               bf16_t* __restrict__ G3, double* __restrict__ Eatom,
-                                               long long* __restrict__ eslot) {
-  const int row = blockIdx.x;
-  const int lane = threadIdx.x;
-  const float* p = row >= row_split ? p1 : p0;
-  float e = 0.f;
-  for (int c = lane; c < ML_H; c += 64) {
+static_assert(ML_H == 256 && P_W4 % 4 == 0, "head kernel layout: 64 lanes x 4 columns");
+// per row: E_i = H3 . w4 + b4, G3 = w4 * S3 (bf16). One wave per row at a time, each lane four
+// consecutive columns (8-byte bf16 and 16-byte float accesses); each workgroup covers HEAD_ROWS
+// rows and adds its energy with one fixed-point atomic (a per-row atomic on one address
+// serialises).
+#define HEAD_ROWS 32
+__global__ __launch_bounds__(256) void k_ml_head(int rows, int row_split, const bf16_t* __restrict__ H3,
+                                                const float* __restrict__ S3, const float* __restrict__ p0,
+                                                const float* __restrict__ p1, const int* __restrict__ atom_of_row,
+                                                bf16_t* __restrict__ G3, double* __restrict__ Eatom,
+                                                long long* __restrict__ eslot) {
+  __shared__ double wsum[4];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c = lane * 4;  // ML_H == 256 == 64 lanes x 4
+  double ew = 0.0;
+  const int r0 = blockIdx.x * HEAD_ROWS, r1 = min(rows, r0 + HEAD_ROWS);
+  for (int row = r0 + w; row < r1; row += 4) {
+    const float* p = row >= row_split ? p1 : p0;
     const long long idx = (long long)row * ML_H + c;
-    e += bf2f(H3[idx]) * p[P_W4 + c];
-    G3[idx] = f2bf(p[P_W4 + c] * S3[idx]);
+    const bf16x4 h = *(const bf16x4*)(H3 + idx);
+    const float4 sv = *(const float4*)(S3 + idx);
+    const float4 wv = *(const float4*)(p + P_W4 + c);
+    float e = bf2f((bf16_t)h[0]) * wv.x + bf2f((bf16_t)h[1]) * wv.y + bf2f((bf16_t)h[2]) * wv.z +
+              bf2f((bf16_t)h[3]) * wv.w;
+    bf16x4 g;
+    g[0] = (short)f2bf(wv.x * sv.x);
+    g[1] = (short)f2bf(wv.y * sv.y);
+    g[2] = (short)f2bf(wv.z * sv.z);
+    g[3] = (short)f2bf(wv.w * sv.w);
+    *(bf16x4*)(G3 + idx) = g;
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+    if (atom_of_row[row] >= 0) {
+      const double E = (double)e + (double)p[P_B4];
+      if (lane == 0 && Eatom) Eatom[row] = E;
+      ew += E;
+    }
   }
-  for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
-  if (lane == 0 && atom_of_row[row] >= 0) {
-    const double E = (double)e + (double)p[P_B4];
-    if (Eatom) Eatom[row] = E;
-    if (eslot) atomicAdd((unsigned long long*)eslot, (unsigned long long)to_fix(E));
-  }
+  if (lane == 0) wsum[w] = ew;
+  __syncthreads();
+  if (threadIdx.x == 0 && eslot)
+    atomicAdd((unsigned long long*)eslot, (unsigned long long)to_fix(wsum[0] + wsum[1] + wsum[2] + wsum[3]));
 }
 
 // tangent step of one layer: b = a * S (bf16), c = a * hb * DD (fp32)
@@ -1051,7 +1069,8 @@ static int ml_forward_backward(mdx_mlp* m, long long* eslot, bool keep_training_
       (rc = gemm(m, EPI_BIAS_SILU, R, ML_H, ML_H, m->H2.p, ML_H, 1, N0.w3.p, N1.w3.p, ML_H, 1, rs, nullptr, ML_H, 1,
                  N0.p.p + P_B3, N1.p.p + P_B3, m->H3.p, m->S3.p, DD3)))
     return rc;
-  k_ml_head<<<R, 64, 0, st>>>(R, rs, m->H3.p, m->S3.p, N0.p.p, N1.p.p, m->atom_of_row.p, m->G3.p, nullptr, eslot);
+  k_ml_head<<<div_up(R, HEAD_ROWS), 256, 0, st>>>(R, rs, m->H3.p, m->S3.p, N0.p.p, N1.p.p, m->atom_of_row.p,
+                                                   m->G3.p, nullptr, eslot);
   MDX_LAUNCH_CHECK();
   // input gradient: G2 = (G3 W3^T) * S2, G1 = (G2 W2^T) * S1, GX = G1 W1^T
   if ((rc = gemm(m, EPI_MUL_S, R, ML_H, ML_H, m->G3.p, ML_H, 1, N0.w3.p, N1.w3.p, 1, ML_H, rs,
