@@ -131,6 +131,10 @@ int mdx_pme_set_fused(mdx_ctx* ctx, int enable);
 /* Evaluate the terms in `terms` at the current positions. If d_force_out != NULL, write the
  * summed forces (N*3 f64). Energies accumulate on the device; read them with mdx_energies_fetch. */
 int mdx_compute_forces(mdx_ctx* ctx, unsigned terms, double* d_force_out);
+/* As mdx_compute_forces, with the kernels that integrator steps recording no energies run (the
+ * same forces to float32 rounding; the direct-space Coulomb force uses a tabulated Ewald
+ * correction). The energy accumulators are left at zero. */
+int mdx_compute_forces_only(mdx_ctx* ctx, unsigned terms, double* d_force_out);
 /* Copy the per-term energies of the last evaluation (+ kinetic energy at MDX_E_KINETIC) to
  * h_out[MDX_NTERMS + 1]. Synchronises. */
 int mdx_energies_fetch(mdx_ctx* ctx, double* h_out);

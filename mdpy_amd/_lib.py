@@ -80,6 +80,7 @@ SIGNATURES = {
     "mdx_pme_config": (ctypes.c_int, [_vp, _c_int_p, ctypes.c_int, ctypes.c_double]),
     "mdx_pme_set_fused": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mdx_compute_forces": (ctypes.c_int, [_vp, ctypes.c_uint, _vp]),
+    "mdx_compute_forces_only": (ctypes.c_int, [_vp, ctypes.c_uint, _vp]),
     "mdx_energies_fetch": (ctypes.c_int, [_vp, _c_dbl_p]),
     "mdx_integrator_config": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_double, ctypes.c_uint64]),

@@ -667,7 +667,7 @@ extern "C" int mdx_ctx_destroy(mdx_ctx* c) {
   c->twiddle.release(); c->excl_pairs.release(); c->xa.release(); c->xb.release(); c->va.release();
   c->vb.release(); c->acc_prev.release(); c->acc_cur.release(); c->ehist.release(); c->tmp64.release();
   c->pos_sorted.release(); c->fpart.release(); c->pme_grec.release(); c->pme_grec_n.release();
-  c->ex_win.release(); c->s4_win.release(); c->win_bad.release();
+  c->ex_win.release(); c->s4_win.release(); c->win_bad.release(); c->ctab.release();
   if (c->owns_stream) hipStreamDestroy(c->stream);
   if (c->side_stream) hipStreamDestroy(c->side_stream);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
@@ -887,7 +887,7 @@ extern "C" int mdx_nonbonded_config(mdx_ctx* c, double cutoff, double skin, int 
   c->cell_cutoff = cell_cutoff > 0 ? cell_cutoff : 12.0;
   mdx_invalidate_graph(c);
   int rc = mdx_nlist_prepare(c);
-  if (rc) return rc;
+  if (rc || (rc = mdx_coulomb_table_update(c))) return rc;
   c->nb_configured = true;
   return 0;
 }
@@ -915,12 +915,13 @@ int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy) {
     // inside an integrator step the bonded terms share the list-fill launch
     c->fill_bonded_terms = c->prologue_done ? terms : 0u;
     c->bonded_done = false;
-    rc = mdx_nlist_update(c, false);
+    rc = mdx_nlist_update_order(c, false);
     c->fill_bonded_terms = 0;
     if (rc) return rc;
   }
-  // the PME plane kernels only read what the list update produced: run them on the side
-  // stream, concurrently with the nonbonded and bonded kernels, and join before returning
+  // the PME plane kernels read only the cell-sorted order and packed positions, final after the
+  // first part of the list update: run them on the side stream, concurrently with the list build
+  // (all of a rebuild step's ~160 us), the nonbonded and the bonded kernels; join before returning
   const bool fork_pme = (terms & MDX_TERM_COUL_RECIP) && (lj || cd) && c->pme_planes && c->side_stream &&
                         c->has_charges && c->pme_configured;
   if (fork_pme) {
@@ -934,7 +935,7 @@ int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy) {
     MDX_HIP(hipEventRecord(c->ev_join, c->side_stream));
   }
   if (lj || cd) {
-    if ((rc = mdx_enqueue_nonbonded(c, lj, cd, energy))) return rc;
+    if ((rc = mdx_nlist_update_list(c)) || (rc = mdx_enqueue_nonbonded(c, lj, cd, energy))) return rc;
   }
   if (terms & MDX_TERM_COUL_RECIP) {
     if (!fork_pme) {
@@ -961,12 +962,12 @@ int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy) {
   return 0;
 }
 
-extern "C" int mdx_compute_forces(mdx_ctx* c, unsigned terms, double* d_force_out) {
+static int compute_forces(mdx_ctx* c, unsigned terms, double* d_force_out, bool energy) {
   MDX_CHECK(c && c->n > 0, MDX_ERR_INVALID, "mdx_compute_forces: no system");
   ENTER();
   int rc = mdx_enqueue_zero_accumulators(c);
-  if (!rc) rc = mdx_enqueue_forces(c, terms, true);
-  if (!rc && c->vel && c->has_mass) {
+  if (!rc) rc = mdx_enqueue_forces(c, terms, energy);
+  if (!rc && energy && c->vel && c->has_mass) {
     k_kinetic<<<div_up(c->n, 256), 256, 0, c->stream>>>(c->n, c->vel, c->mass.p, c->efix.p + MDX_E_KINETIC);
     if (hipGetLastError() != hipSuccess) rc = MDX_ERR_HIP;
   }
@@ -976,6 +977,14 @@ extern "C" int mdx_compute_forces(mdx_ctx* c, unsigned terms, double* d_force_ou
   }
   c->evals++;
   LEAVE_RET(rc);
+}
+
+extern "C" int mdx_compute_forces(mdx_ctx* c, unsigned terms, double* d_force_out) {
+  return compute_forces(c, terms, d_force_out, true);
+}
+
+extern "C" int mdx_compute_forces_only(mdx_ctx* c, unsigned terms, double* d_force_out) {
+  return compute_forces(c, terms, d_force_out, false);
 }
 
 extern "C" int mdx_energies_fetch(mdx_ctx* c, double* h_out) {

@@ -18,6 +18,14 @@ static constexpr double kFixInv = 1.0 / 1099511627776.0;
 __device__ __forceinline__ long long to_fix(double v) { return __double2ll_rn(v * kFixScale); }
 __device__ __forceinline__ long long to_fix_f(float v) { return __double2ll_rn((double)v * kFixScale); }
 
+// Orders one wave's LDS writes before its later LDS reads (and the reads before later writes):
+// a wave's LDS operations complete in program order, so only compiler motion must be stopped.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // status flag bits (device word)
 #define MDX_FLAG_OVERFLOW 1u
 #define MDX_FLAG_LOSS 2u
@@ -159,6 +167,9 @@ struct mdx_ctx {
   DevBuf<int> orig_sorted;       // original atom -> sorted slot
   DevBuf<float4> xq;             // sorted: x, y, z (centred, float), q*sqrt(ke)
   DevBuf<float2> ljs, ljs14;     // sorted: sigma/2, sqrt(eps)
+  DevBuf<float4> ctab;           // Ewald correction G(r^2) as cubic pieces (MDX_CTAB_N of them)
+  double ctab_beta = 0, ctab_rc = 0;
+  float ctab_scale = 0.f;        // pieces per A^2
   DevBuf<int> ref_cell_sorted;   // sorted: packed reference cell coords (REFERENCE mode)
   DevBuf<double> pos_at_build;   // original order, 3N
   DevBuf<double> pos_sorted;     // sorted order, 3N, refreshed with xq every evaluation
@@ -358,11 +369,16 @@ void mdx_invalidate_graph(mdx_ctx* c);
 // celllist / neighbour list
 int mdx_nlist_prepare(mdx_ctx* c);                    // allocate grid/list buffers for current config
 int mdx_nlist_update(mdx_ctx* c, bool force_rebuild);  // enqueue displacement check + conditional rebuild + pack
+int mdx_nlist_update_order(mdx_ctx* c, bool force_rebuild);  // its first part: up to the sorted arrays and pack
+int mdx_nlist_update_list(mdx_ctx* c);                       // its second part: the list build
 int mdx_nlist_enqueue_pack(mdx_ctx* c);
 int mdx_nlist_enqueue_clear(mdx_ctx* c);
 int mdx_enqueue_request_rebuild(mdx_ctx* c);
 // forces
 int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy);
+// Rebuild the nonbonded kernel's Ewald-correction table when beta or the cutoff changed.
+#define MDX_CTAB_N 256
+int mdx_coulomb_table_update(mdx_ctx* c);
 int mdx_enqueue_coulomb_plain(mdx_ctx* c);
 int mdx_pme_enqueue(mdx_ctx* c, bool energy);
 int mdx_bonded_enqueue(mdx_ctx* c, unsigned terms, const FillArgs* fill = nullptr);

@@ -745,13 +745,25 @@ int mdx_nlist_enqueue_pack(mdx_ctx* c) {
   return 0;
 }
 
-// Enqueue: displacement check, conditional rebuild, pack. Leaves the rebuild flag set for the
-// consumer (k_clear_rebuild is enqueued by the caller after the last reader).
+// squared displacement that triggers a rebuild; skin == 0 (reference pair set or no buffer):
+// rebuild every evaluation
+static double rebuild_thr2(const mdx_ctx* c) {
+  const double half = 0.5 * c->skin;
+  return c->skin > 0 ? half * half : -1.0;
+}
+
+// Enqueue: displacement check, conditional rebuild of the list, pack. Leaves the rebuild flag set
+// for the consumer (k_clear_rebuild is enqueued by the caller after the last reader).
 int mdx_nlist_update(mdx_ctx* c, bool force_rebuild) {
+  int rc = mdx_nlist_update_order(c, force_rebuild);
+  return rc ? rc : mdx_nlist_update_list(c);
+}
+
+// First part: displacement check and, on a rebuild, binning, per-cell sort and fill of the
+// sorted-order arrays; then the pack. Everything the PME plane kernels read is final after it.
+int mdx_nlist_update_order(mdx_ctx* c, bool force_rebuild) {
   MDX_CHECK(c->nb_configured, MDX_ERR_INVALID, "nonbonded not configured");
-  double half = 0.5 * c->skin;
-  // skin == 0 (reference pair set or no buffer): rebuild every evaluation
-  double thr2 = c->skin > 0 ? half * half : -1.0;
+  const double thr2 = rebuild_thr2(c);
   if (force_rebuild) {
     int rc0 = mdx_enqueue_request_rebuild(c);
     if (rc0) return rc0;
@@ -793,10 +805,16 @@ int mdx_nlist_update(mdx_ctx* c, bool force_rebuild) {
     k_grid_sort_fill<<<c->ncell, 256, 0, c->stream>>>(fa);
     MDX_LAUNCH_CHECK();
   }
-  int rc = pro ? 0 : mdx_nlist_enqueue_pack(c);
-  if (rc) return rc;
+  return pro ? 0 : mdx_nlist_enqueue_pack(c);
+}
+
+// Second part: the neighbour-list build (its workgroups return at once unless rebuilding).
+int mdx_nlist_update_list(mdx_ctx* c) {
+  const double thr2 = rebuild_thr2(c);
+  DevState* st = c->dstate.p;
+  const bool ref = c->pair_mode == MDX_PAIRS_REFERENCE;
   ListArgs a;
-  a.g = g;
+  make_grid(c, &a.g);
   a.n = c->n;
   a.cap = c->nbr_cap;
   double rl = c->rc + c->skin;

@@ -25,8 +25,68 @@ struct NbArgs {
   float Lx, Ly, Lz, ix, iy, iz;
   float rc2, guard2;
   float beta, twob_sqrtpi;
+  float ct_scale;  // Ewald-correction table pieces per A^2
   double Lxd, Lyd, Lzd, rcd;
+  const float4* ctab;
 };
+
+// Forces-only launches evaluate the direct-space Coulomb force as
+//   F_ij / (q_i q_j) = d (1/r^3 - G(r^2)),  G(z) = erf(beta r)/r^3 - (2 beta/sqrt(pi)) exp(-beta^2 z)/z,
+// which equals erfc(beta r)/r^3 + (2 beta/sqrt(pi)) exp(-beta^2 r^2)/r^2. G is smooth and finite
+// on [0, rc^2] (G(0) = 4 beta^3/(3 sqrt(pi))), so MDX_CTAB_N cubic Hermite pieces in z = r^2,
+// built in float64 from G and G' at the knots, reproduce it to ~1e-9 relative; each pair then
+// needs one LDS read and three FMAs instead of exp, a reciprocal and the erfc polynomial. The
+// subtraction leaves an absolute error of ~1 ulp of 1/r^3 per pair (relative error grows
+// towards the cutoff, where the force itself is small). Energy launches keep the erfc form.
+static void coulomb_correction(double beta, double z, double* G, double* dG) {
+  const double k = 2.0 * beta * beta * beta / sqrt(M_PI);
+  const double x2 = beta * beta * z;
+  if (x2 < 2.0) {
+    // G = k sum_{n>=1} (-1)^(n+1) 2n / (n! (2n+1)) x^(2n-2): the series of erf(x)/x^3 minus
+    // that of exp(-x^2)/x^2, without their cancellation at small z
+    double s = 0, sp = 0, fact = 1, xp = 1, xpm = 0;  // xp = x2^(n-1), xpm = x2^(n-2)
+    for (int n = 1; n <= 40; ++n) {
+      fact *= n;
+      const double cn = ((n & 1) ? 2.0 : -2.0) * n / (fact * (2 * n + 1));
+      s += cn * xp;
+      if (n >= 2) sp += cn * (n - 1) * beta * beta * xpm;
+      xpm = xp;
+      xp *= x2;
+    }
+    *G = k * s;
+    *dG = k * sp;
+    return;
+  }
+  const double r = sqrt(z), e = exp(-x2), er = erf(beta * r), c = 2.0 * beta / sqrt(M_PI);
+  *G = er / (z * r) - c * e / z;
+  *dG = beta * e / (sqrt(M_PI) * z * z) - 1.5 * er / (z * z * r) + c * (beta * beta * e / z + e / (z * z));
+}
+
+int mdx_coulomb_table_update(mdx_ctx* c) {
+  if (!(c->beta > 0 && c->rc > 0) || (c->ctab.p && c->ctab_beta == c->beta && c->ctab_rc == c->rc)) return 0;
+  const int N = MDX_CTAB_N;
+  const double zmax = c->rc * c->rc * (1.0 + 1e-3);  // the float64 cutoff re-decision may accept r2 a hair above rc^2
+  const double h = zmax / N;
+  std::vector<float4> tab(N);
+  double g0, d0;
+  coulomb_correction(c->beta, 0.0, &g0, &d0);
+  for (int k = 0; k < N; ++k) {
+    double g1, d1;
+    coulomb_correction(c->beta, (k + 1) * h, &g1, &d1);
+    const double a0 = g0, a1 = d0 * h, a2 = 3.0 * (g1 - g0) - (2.0 * d0 + d1) * h, a3 = 2.0 * (g0 - g1) + (d0 + d1) * h;
+    tab[k] = make_float4((float)a0, (float)a1, (float)a2, (float)a3);
+    g0 = g1;
+    d0 = d1;
+  }
+  int rc = c->ctab.ensure(N);
+  if (rc) return rc;
+  MDX_HIP(hipMemcpy(c->ctab.p, tab.data(), N * sizeof(float4), hipMemcpyHostToDevice));
+  c->ctab_beta = c->beta;
+  c->ctab_rc = c->rc;
+  c->ctab_scale = (float)(N / zmax);
+  mdx_invalidate_graph(c);
+  return 0;
+}
 
 // erfc(x) = exp(-x^2) t P(t), t = 1/(1 + p x): the form of Abramowitz & Stegun 7.1.26 refit
 // (mdpy_amd/tools/fit_erfc.py) to a 7-term polynomial with max relative error 4.5e-8 on
@@ -72,6 +132,16 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, const float4* 
                                                       const int* __restrict__ sorted_orig,
                                                       const double* __restrict__ pos, long long* __restrict__ ffix,
                                                       long long* __restrict__ efix, DevState* st) {
+  constexpr bool TAB = COUL && !ENERGY;
+#if NB_TAB_GLOBAL
+  const float4* __restrict__ ctab = a.ctab;
+#else
+  __shared__ float4 ctab[TAB ? MDX_CTAB_N : 1];
+  if (TAB) {
+    for (int k = threadIdx.x; k < MDX_CTAB_N; k += blockDim.x) ctab[k] = a.ctab[k];
+    __syncthreads();
+  }
+#endif
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int s = gt / G;
   const int g = gt & (G - 1);
@@ -135,7 +205,14 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, const float4* 
           fr -= 24.f * eps * (2.f * s12 - s6) * r2inv;
           if (ENERGY) elj += in ? (double)(4.f * eps * (s12 - s6)) : 0.0;
         }
-        if (COUL) {
+        if (TAB) {
+          const float zt = r2 * a.ct_scale;
+          const float fl = floorf(zt);
+          const float t = zt - fl;
+          const float4 cf = ctab[min((int)fl, MDX_CTAB_N - 1)];
+          const float gz = fmaf(t, fmaf(t, fmaf(t, cf.w, cf.z), cf.y), cf.x);
+          fr -= xi.w * xj[u].w * (rinv * r2inv - gz);
+        } else if (COUL) {
           const float qq = xi.w * xj[u].w;
           const float x = a.beta * (r2 * rinv);
           const float ex = __expf(-x * x);
@@ -206,6 +283,10 @@ int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy) {
   a.guard2 = (float)(c->rc * 2e-4);
   a.beta = (float)c->beta;
   a.twob_sqrtpi = (float)(2.0 * c->beta / sqrt(M_PI));
+  a.ct_scale = c->ctab_scale;
+  a.ctab = c->ctab.p;
+  MDX_CHECK(!coul || (a.ctab && c->ctab_beta == c->beta && c->ctab_rc == c->rc), MDX_ERR_INVALID,
+            "Ewald-correction table not built for the current beta and cutoff");
   a.Lxd = c->box[0];
   a.Lyd = c->box[1];
   a.Lzd = c->box[2];

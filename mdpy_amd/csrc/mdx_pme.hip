@@ -144,12 +144,6 @@ __device__ float2* lds_fft(float2* src, float2* dst, const FftPlan& pl, int B, c
 // so that both rows and columns are read across lanes without LDS bank conflicts.
 #define PLANE64_LS 65
 
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // one radix-4 pass (p = 4^S) of the 64-point line at X/Y + off, elements es apart; lane i of
 // the line's 16 computes butterfly i
 template <int DIR, int S>
@@ -938,6 +932,7 @@ extern "C" int mdx_pme_config(mdx_ctx* c, const int* h_grid, int order, double b
     c->pme_fused_lds = std::max(lds_x, std::max(plane_fwd_lds(c->K[1], c->K[2], c->pme_wf),
                                                 plane_gather_lds(c->K[1], c->K[2])));
   }
+  if ((rc = mdx_coulomb_table_update(c))) return rc;
   c->pme_configured = true;
   return 0;
 }

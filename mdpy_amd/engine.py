@@ -206,6 +206,12 @@ class Engine:
         L.call("mdx_compute_forces", self.ctx, int(terms), ctypes.c_void_p(buf.data_ptr()))
         return buf, self.energies()
 
+    def compute_forces_only(self, terms, out=None):
+        """Forces of `terms` with the kernels of integrator steps that record no energies."""
+        buf = self.force_buf if out is None else out
+        L.call("mdx_compute_forces_only", self.ctx, int(terms), ctypes.c_void_p(buf.data_ptr()))
+        return buf
+
     def compute_async(self, terms, out=None):
         buf = self.force_buf if out is None else out
         L.call("mdx_compute_forces", self.ctx, int(terms), ctypes.c_void_p(buf.data_ptr() if buf is not None else 0))

@@ -224,6 +224,47 @@ def test_forces_deterministic(water23328):
 
 
 @pytest.mark.parametrize("name", ["water648", "water23328", "chain"])
+def test_nonbonded_variants_match_oracle(name, request):
+    """Every nonbonded kernel variant (16/32/64 lanes per atom) against the oracle's LJ and
+    direct-space Coulomb, forces-only launches (tabulated Ewald correction) and energy launches
+    (erfc form) alike, with the tolerances of the full force test; each bitwise repeatable."""
+    from mdpy_amd import _lib as L
+    from mdpy_amd.engine import Engine
+    from mdpy_amd.unit import ONE_OVER_4PI_EPS0
+    from oracle import ewald as EW, mdref as R
+    s = request.getfixturevalue(name)
+    e = Engine(s, skin=1.5)
+    e.set_positions(s.positions)
+    xw = wrapped(s)
+    ep, ei = s.exclusions()
+    sp, si = s.scaling_pairs()
+    pairs = R.exact_pairs(xw, s.box, s.cutoff, ep, ei)
+    Flj, Elj = R.lj_pair_terms(xw, s.lj, pairs, s.box, sp, si)
+    Fd, Ed = EW.direct_terms(xw, s.charges, pairs, s.box, e.beta, ONE_OVER_4PI_EPS0)
+    _, Ed_abs = EW.direct_terms(xw, np.abs(s.charges), pairs, s.box, e.beta, ONE_OVER_4PI_EPS0)
+    for lanes in (16, 32, 64):
+        say("case: nonbonded variant", lanes, name)
+        assert e.tune_nonbonded(L.TERM_LJ | L.TERM_COUL_DIRECT, lanes) == lanes
+        for bit, key, Fref, Eref, scale in ((L.TERM_LJ, "lj", Flj, Elj, abs(Elj)),
+                                           (L.TERM_COUL_DIRECT, "coulomb_direct", Fd, Ed,
+                                            max(abs(Ed), 1e-2 * Ed_abs))):
+            F1, en1 = e.compute(bit)
+            F1 = F1.cpu().numpy().copy()
+            F2, en2 = e.compute(bit)
+            assert np.array_equal(F1, F2.cpu().numpy()) and en1[key] == en2[key], (lanes, key)
+            err = rel_rms(F1, Fref)
+            F3 = e.compute_forces_only(bit).cpu().numpy().copy()
+            assert np.array_equal(F3, e.compute_forces_only(bit).cpu().numpy()), (lanes, key)
+            err3 = rel_rms(F3, Fref)
+            say("  %-16s rel force RMS %.3e (forces-only launch %.3e)  E %.10e ref %.10e"
+                % (key, err, err3, en1[key], Eref))
+            assert err < 3 * FORCE_RTOL and err3 < 3 * FORCE_RTOL, (lanes, key, err, err3)
+            assert abs(en1[key] - Eref) <= max(ENERGY_RTOL * scale, 1e-9), (lanes, key, en1[key], Eref)
+        assert e.status() == 0
+    e.close()
+
+
+@pytest.mark.parametrize("name", ["water648", "water23328", "chain"])
 def test_pme_one_launch_equals_three_launches(name, request):
     """The one-launch reciprocal kernel (grid barrier between its phases) and the three-kernel
     path run the same arithmetic per plane and per pencil: forces bitwise equal, energies equal
