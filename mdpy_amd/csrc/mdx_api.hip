@@ -800,14 +800,17 @@ static int set_window_masks(mdx_ctx* c, const int* h_ptr, const int* h_idx, int 
   if (c->h_win_bad.size() != (size_t)n) c->h_win_bad.assign(n, 0);
   if (bit == 0 && c->h_s4_win.size() != (size_t)n) c->h_s4_win.assign(n, 0ull);
   if (bit == 1 && c->h_ex_win.size() != (size_t)n) c->h_ex_win.assign(n, 0ull);
+  // window: partners i-32 .. i+30 (bits 0..62); bit 63 stays clear, so a clamped out-of-window
+  // offset reads a zero bit. Bit 32 of the exclusion mask excludes the atom itself.
   for (int i = 0; i < n; ++i) {
     c->h_win_bad[i] &= (unsigned char)~(1u << bit);
     for (int k = h_ptr[i]; k < h_ptr[i + 1]; ++k) {
       const int d = h_idx[k] - i + 32;
-      if (d >= 0 && d < 64) m[i] |= 1ull << d;
+      if (d >= 0 && d < 63) m[i] |= 1ull << d;
       else c->h_win_bad[i] |= (unsigned char)(1u << bit);
     }
   }
+  for (int i = 0; i < n; ++i) c->h_ex_win[i] |= 1ull << 32;
   int rc;
   if ((rc = upload(c->ex_win, c->h_ex_win.data(), (size_t)n)) || (rc = upload(c->s4_win, c->h_s4_win.data(), (size_t)n)) ||
       (rc = upload(c->win_bad, c->h_win_bad.data(), (size_t)n)))

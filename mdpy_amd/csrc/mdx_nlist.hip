@@ -386,27 +386,26 @@ __device__ __forceinline__ int cyc_dist(int a, int b, int n) {
 // One workgroup per home cell: the candidates of every stencil cell (identical for all atoms of
 // the home cell) are staged in LDS in chunks, then each NB_G-lane group scans them for one home
 // atom at a time and compacts the accepted ones with a wave ballot. Same candidate order, hence
-// the same lists, as a per-atom walk of the stencil; global memory is read once per cell.
+// the same lists, as a per-atom walk of the stencil; global memory is read once per cell. The
+// next chunk's candidates are fetched into registers while the current one is scanned.
 // Staged candidates are shifted into the home cell's periodic frame (when the stencil holds each
 // periodic cell once, on no more than one side), so the scan needs no minimum-image arithmetic.
 // Exclusions and 1-4 partners within 32 atom indices of the home atom are bits of two 64-bit
 // masks; an atom with a partner outside that window scans its CSR lists (staged in LDS) instead.
-// Each lane tests BL_U candidates per pass, all LDS reads issued first; the float64 decision of
-// skin-0 lists is a separate instantiation.
+// Each lane tests BL_U candidates per pass, all LDS reads issued first. FAST instantiations
+// (shifted candidates, exact pair mode, masks present) compile out the reference-mode filter and
+// the minimum image; the float64 decision of skin-0 lists is the EXACT64 instantiation.
 #define BL_CH 768
 #define BL_MAXST 512
 #define BL_MAXEX 32  // exclusions (and 1-4 partners) per atom
 #define BL_HMAX 192  // home atoms staged in LDS (more: their data is read from global memory)
 #define BL_POOL 1024 // exclusion / 1-4 entries of a cell's home atoms staged in LDS
 #define BL_THREADS 512
+#define BL_SLOTS ((BL_CH + BL_THREADS - 1) / BL_THREADS)  // staged candidates per lane per chunk
 #define BL_U 4       // candidates per lane per pass
 #define BL_WIN 32    // index window of the exclusion masks: partners i-32 .. i+31
-struct CandTest {
-  float4 xi;
-  float Lx, Ly, Lz, ix, iy, iz, rl2;
-};
 
-template <bool EXACT64>
+template <bool EXACT64, bool FAST>
 __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
     ListArgs a, const float4* __restrict__ xq, const int* __restrict__ sorted_orig,
     const int* __restrict__ cell_start, const int* __restrict__ cell_count, const int* __restrict__ stencil,
@@ -418,14 +417,16 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
   if (!need_rebuild(st, thr2)) return;
   __shared__ float4 cxq[BL_CH];
   __shared__ int2 cto[BL_CH];      // (sorted slot, original atom id)
-  __shared__ int cr[BL_CH];
+  __shared__ int cr[FAST ? 1 : BL_CH];
   __shared__ int sb[BL_MAXST], sp[BL_MAXST + 1];
   __shared__ unsigned char shc[BL_MAXST];  // periodic shift of each stencil cell: 2 bits per axis
   __shared__ int acnt[SORT_MAX];
   __shared__ int exl[BL_THREADS / NB_G][BL_MAXEX], s4l[BL_THREADS / NB_G][BL_MAXEX];
-  // home atoms staged once per block: position, atom id and their exclusion / 1-4 lists
+  // home atoms staged once per block: position, atom id, window masks, exclusion / 1-4 lists
   __shared__ float4 hxi[BL_HMAX];
   __shared__ int ho[BL_HMAX], hex0[BL_HMAX + 1], hs40[BL_HMAX + 1];
+  __shared__ unsigned long long hexm[BL_HMAX], hs4m[BL_HMAX];
+  __shared__ unsigned char hbad[BL_HMAX];
   __shared__ int pool_ex[BL_POOL], pool_s4[BL_POOL];
   __shared__ int fast;
   const GridArgs& gr = a.g;
@@ -434,6 +435,7 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
   if (hn == 0 || hn > SORT_MAX) return;  // oversize cells are flagged by k_cell_scan
   const int tid = threadIdx.x;
   const int hz = cell % gr.nz, hy = (cell / gr.nz) % gr.ny, hx = cell / (gr.nz * gr.ny);
+  const bool shifted = FAST || a.shifted;
   for (int k = tid; k < a.nstencil; k += blockDim.x) {
     int p = stencil[k];
     int nx = hx + ((p >> 16) & 0xff) - 128, ny = hy + ((p >> 8) & 0xff) - 128, nz = hz + (p & 0xff) - 128;
@@ -459,6 +461,9 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
       hxi[h] = xq[hb + h];
       hex0[h + 1] = excl_ptr ? excl_ptr[o + 1] - excl_ptr[o] : 0;
       hs40[h + 1] = s14_ptr ? s14_ptr[o + 1] - s14_ptr[o] : 0;
+      hexm[h] = ex_win ? ex_win[o] : 0ull;
+      hs4m[h] = s4_win ? s4_win[o] : 0ull;
+      hbad[h] = win_bad ? win_bad[o] : (unsigned char)1;
     }
   }
   __syncthreads();
@@ -485,87 +490,112 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
         for (int r = 0; r < hs40[h + 1] - hs40[h]; ++r) pool_s4[hs40[h] + r] = s14_idx[s14_ptr[o] + r];
     }
   }
-  __syncthreads();
   const int total = sp[a.nstencil];
+  const float Lx = (float)gr.Lx, Ly = (float)gr.Ly, Lz = (float)gr.Lz;
+  // a lane's share of one chunk's candidates, fetched from global memory into registers
+  float4 pv[BL_SLOTS];
+  int2 pt[BL_SLOTS];
+  int pr[BL_SLOTS];
+  auto fetch = [&](int base) {
+    const int nch = min(BL_CH, total - base);
+#pragma unroll
+    for (int r = 0; r < BL_SLOTS; ++r) {
+      const int c = tid + r * BL_THREADS;
+      if (c < nch) {
+        const int idx = base + c;
+        int lo = 0, hi = a.nstencil - 1;  // last stencil cell whose prefix <= idx
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (sp[mid] <= idx) lo = mid;
+          else hi = mid - 1;
+        }
+        const int t = sb[lo] + (idx - sp[lo]);
+        float4 v = xq[t];
+        if (shifted) {
+          const int code = shc[lo];
+          v.x += (float)((code & 3) - 1) * Lx;
+          v.y += (float)(((code >> 2) & 3) - 1) * Ly;
+          v.z += (float)(((code >> 4) & 3) - 1) * Lz;
+        }
+        pv[r] = v;
+        pt[r] = make_int2(t, sorted_orig[t]);
+        if (!FAST && a.ref_mode) pr[r] = ref_cell_sorted[t];
+      }
+    }
+  };
+  auto store = [&](int base) {
+    const int nch = min(BL_CH, total - base);
+#pragma unroll
+    for (int r = 0; r < BL_SLOTS; ++r) {
+      const int c = tid + r * BL_THREADS;
+      if (c < nch) {
+        cxq[c] = pv[r];
+        cto[c] = pt[r];
+        if (!FAST && a.ref_mode) cr[c] = pr[r];
+      }
+    }
+  };
+  __syncthreads();  // sp, pools
+  if (total > 0) fetch(0);
+  store(0);
+  __syncthreads();
   const int lane = tid & 63;
   const int q = tid / NB_G, g = tid & (NB_G - 1);
   const int ngroups = blockDim.x / NB_G;
   const unsigned long long gmask = (NB_G == 64 ? ~0ull : ((1ull << NB_G) - 1ull)) << (lane & ~(NB_G - 1));
   const unsigned long long below = (1ull << lane) - 1ull;
-  CandTest T;
-  T.Lx = (float)gr.Lx;
-  T.Ly = (float)gr.Ly;
-  T.Lz = (float)gr.Lz;
-  T.ix = 1.0f / T.Lx;
-  T.iy = 1.0f / T.Ly;
-  T.iz = 1.0f / T.Lz;
-  T.rl2 = a.rl2;
+  const float ix = 1.0f / Lx, iy = 1.0f / Ly, iz = 1.0f / Lz;
   int* my_ex = exl[q];
   int* my_s4 = s4l[q];
   for (int base = 0; base < total; base += BL_CH) {
     const int nch = min(BL_CH, total - base);
-    for (int c = tid; c < nch; c += blockDim.x) {
-      const int idx = base + c;
-      int lo = 0, hi = a.nstencil - 1;  // last stencil cell whose prefix <= idx
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (sp[mid] <= idx) lo = mid;
-        else hi = mid - 1;
-      }
-      const int t = sb[lo] + (idx - sp[lo]);
-      float4 v = xq[t];
-      if (a.shifted) {
-        const int code = shc[lo];
-        v.x += (float)((code & 3) - 1) * T.Lx;
-        v.y += (float)(((code >> 2) & 3) - 1) * T.Ly;
-        v.z += (float)(((code >> 4) & 3) - 1) * T.Lz;
-      }
-      cxq[c] = v;
-      cto[c] = make_int2(t, sorted_orig[t]);
-      if (a.ref_mode) cr[c] = ref_cell_sorted[t];
-    }
-    __syncthreads();
+    const bool more = base + BL_CH < total;
+    if (more) fetch(base + BL_CH);  // in flight during this chunk's scan
     for (int h = q; h < hn; h += ngroups) {  // uniform per group
       const int s = hb + h;
       int o, nex_all, ns4_all;
       const int* src_ex;
       const int* src_s4;
+      float4 xi;
+      bool masked;
+      unsigned long long exm, s4m;
       if (fast) {  // everything from LDS
         o = ho[h];
-        T.xi = hxi[h];
+        xi = hxi[h];
         nex_all = hex0[h + 1] - hex0[h];
         ns4_all = hs40[h + 1] - hs40[h];
         src_ex = pool_ex + hex0[h];
         src_s4 = pool_s4 + hs40[h];
+        masked = hbad[h] == 0;
+        exm = hexm[h];
+        s4m = hs4m[h];
       } else {     // crowded cell: read the home atom's data from global memory
         o = sorted_orig[s];
-        T.xi = xq[s];
+        xi = xq[s];
         const int e0 = excl_ptr ? excl_ptr[o] : 0, t0s = s14_ptr ? s14_ptr[o] : 0;
         nex_all = excl_ptr ? excl_ptr[o + 1] - e0 : 0;
         ns4_all = s14_ptr ? s14_ptr[o + 1] - t0s : 0;
         src_ex = excl_idx + e0;
         src_s4 = s14_idx + t0s;
+        masked = win_bad && win_bad[o] == 0;
+        exm = ex_win ? ex_win[o] : 0ull;
+        s4m = s4_win ? s4_win[o] : 0ull;
       }
       // index-window masks, or the CSR lists for an atom with a partner outside the window
-      const bool masked = !(win_bad && win_bad[o]);
-      const unsigned long long exm = ex_win ? ex_win[o] : 0ull, s4m = s4_win ? s4_win[o] : 0ull;
       const int nex = masked ? 0 : min(nex_all, BL_MAXEX), ns4 = masked ? 0 : min(ns4_all, BL_MAXEX);
       if (!masked) {
         if (g < nex) my_ex[g] = src_ex[g];
         if (g < ns4) my_s4[g] = src_s4[g];
         if (g == 0 && (nex_all > BL_MAXEX || ns4_all > BL_MAXEX)) atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
       }
-      const int rci = a.ref_mode ? ref_cell_sorted[s] : 0;
+      const int rci = (!FAST && a.ref_mode) ? ref_cell_sorted[s] : 0;
       double pi[3] = {0.0, 0.0, 0.0};
       if (EXACT64) {
         pi[0] = pos[3 * o];
         pi[1] = pos[3 * o + 1];
         pi[2] = pos[3 * o + 2];
       }
-      // the group's lanes are in one wave: LDS writes above are visible after this wave-level fence
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      wave_lds_sync();  // the group's lanes are in one wave
       unsigned* row = nbr_list + (long long)s * a.cap;
       int cnt = acnt[h];
       for (int cb = 0; cb < nch; cb += BL_U * NB_G) {
@@ -577,18 +607,18 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
           const int ci = min(cb + u * NB_G + g, nch - 1);
           xj[u] = cxq[ci];
           tj[u] = cto[ci];
-          rcj[u] = a.ref_mode ? cr[ci] : 0;
+          rcj[u] = (!FAST && a.ref_mode) ? cr[ci] : 0;
         }
 #pragma unroll
         for (int u = 0; u < BL_U; ++u) {
           const int jo = tj[u].y;
-          float dx = xj[u].x - T.xi.x, dy = xj[u].y - T.xi.y, dz = xj[u].z - T.xi.z;
-          if (!a.shifted) {
-            dx -= T.Lx * rintf(dx * T.ix);
-            dy -= T.Ly * rintf(dy * T.iy);
-            dz -= T.Lz * rintf(dz * T.iz);
+          float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
+          if (!shifted) {
+            dx -= Lx * rintf(dx * ix);
+            dy -= Ly * rintf(dy * iy);
+            dz -= Lz * rintf(dz * iz);
           }
-          bool acc = (cb + u * NB_G + g < nch) & (tj[u].x != s) & (dx * dx + dy * dy + dz * dz < T.rl2);
+          bool acc = (cb + u * NB_G + g < nch) & (tj[u].x != s) & (dx * dx + dy * dy + dz * dz < a.rl2);
           bool s14;
           if (masked) {
             const int dw = jo - o + BL_WIN;
@@ -601,7 +631,7 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
             for (int r = 0; r < nex; ++r) acc &= (my_ex[r] != jo);
             for (int r = 0; r < ns4; ++r) s14 |= (my_s4[r] == jo);
           }
-          if (a.ref_mode) {
+          if (!FAST && a.ref_mode) {
             const int ax = cyc_dist(rci >> 20, rcj[u] >> 20, a.rnx);
             const int ay = cyc_dist((rci >> 10) & 1023, (rcj[u] >> 10) & 1023, a.rny);
             const int az = cyc_dist(rci & 1023, rcj[u] & 1023, a.rnz);
@@ -625,12 +655,214 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
         }
       }
       if (g == 0) acnt[h] = cnt;
-      // the next home atom of this group overwrites my_ex / my_s4
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      wave_lds_sync();  // the next home atom of this group overwrites my_ex / my_s4
     }
     __syncthreads();
+    if (more) {
+      store(base + BL_CH);
+      __syncthreads();
+    }
+  }
+  for (int h = tid; h < hn; h += blockDim.x) {
+    const int cnt = acnt[h];
+    nbr_count[hb + h] = min(cnt, a.cap);
+    if (cnt > a.cap) atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
+  }
+}
+
+// The common case of the build above as its own kernel: shifted candidates, exact pair mode,
+// float32 list radius, window masks present. One wave per home atom, so every ballot, count and
+// branch is wave-uniform; exclusions, the atom itself (bit 32) and 1-4 partners are mask bits, the
+// window offset is clamped to bit 63 (always clear) instead of range-checked, and the staged
+// chunk is padded with far-away candidates up to a multiple of the pass width, so a candidate
+// test is distance, two mask bits and the ballot. Home atoms with a partner outside the window
+// scan their CSR lists. Produces the same lists, in the same order, as k_build_list_cell.
+#define BLF_PASS (64 * BL_U)
+__global__ __launch_bounds__(BL_THREADS) void k_build_list_fast(
+    ListArgs a, const float4* __restrict__ xq, const int* __restrict__ sorted_orig,
+    const int* __restrict__ cell_start, const int* __restrict__ cell_count, const int* __restrict__ stencil,
+    const int* __restrict__ excl_ptr, const int* __restrict__ excl_idx, const int* __restrict__ s14_ptr,
+    const int* __restrict__ s14_idx, const unsigned long long* __restrict__ ex_win,
+    const unsigned long long* __restrict__ s4_win, const unsigned char* __restrict__ win_bad,
+    int* __restrict__ nbr_count, unsigned* __restrict__ nbr_list, DevState* st, double thr2) {
+  static_assert(BL_CH % BLF_PASS == 0, "chunks must hold whole passes");
+  if (!need_rebuild(st, thr2)) return;
+  __shared__ float4 cxq[BL_CH];
+  __shared__ int2 cto[BL_CH];  // (sorted slot, original atom id)
+  __shared__ int sb[BL_MAXST], sp[BL_MAXST + 1];
+  __shared__ unsigned char shc[BL_MAXST];
+  __shared__ int acnt[SORT_MAX];
+  __shared__ int exl[BL_THREADS / 64][BL_MAXEX], s4l[BL_THREADS / 64][BL_MAXEX];
+  __shared__ float4 hxi[BL_HMAX];
+  __shared__ int ho[BL_HMAX];
+  __shared__ unsigned long long hexm[BL_HMAX], hs4m[BL_HMAX];
+  __shared__ unsigned char hbad[BL_HMAX];
+  const GridArgs& gr = a.g;
+  const int cell = blockIdx.x;
+  const int hb = cell_start[cell], hn = cell_count[cell];
+  if (hn == 0 || hn > SORT_MAX) return;  // oversize cells are flagged by k_cell_scan
+  const int tid = threadIdx.x;
+  const int hz = cell % gr.nz, hy = (cell / gr.nz) % gr.ny, hx = cell / (gr.nz * gr.ny);
+  for (int k = tid; k < a.nstencil; k += blockDim.x) {
+    int p = stencil[k];
+    int nx = hx + ((p >> 16) & 0xff) - 128, ny = hy + ((p >> 8) & 0xff) - 128, nz = hz + (p & 0xff) - 128;
+    const int cx = nx < 0 ? 0 : (nx >= gr.nx ? 2 : 1);
+    const int cy = ny < 0 ? 0 : (ny >= gr.ny ? 2 : 1);
+    const int cz = nz < 0 ? 0 : (nz >= gr.nz ? 2 : 1);
+    shc[k] = (unsigned char)(cx | (cy << 2) | (cz << 4));
+    nx += nx < 0 ? gr.nx : (nx >= gr.nx ? -gr.nx : 0);
+    ny += ny < 0 ? gr.ny : (ny >= gr.ny ? -gr.ny : 0);
+    nz += nz < 0 ? gr.nz : (nz >= gr.nz ? -gr.nz : 0);
+    const int nc = (nx * gr.ny + ny) * gr.nz + nz;
+    sb[k] = cell_start[nc];
+    sp[k + 1] = cell_count[nc];
+  }
+  for (int j = tid; j < hn; j += blockDim.x) acnt[j] = 0;
+  const bool fit = hn <= BL_HMAX;
+  if (fit) {
+    for (int h = tid; h < hn; h += blockDim.x) {
+      const int o = sorted_orig[hb + h];
+      ho[h] = o;
+      hxi[h] = xq[hb + h];
+      hexm[h] = ex_win[o];
+      hs4m[h] = s4_win[o];
+      hbad[h] = win_bad[o];
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    sp[0] = 0;
+    for (int k = 0; k < a.nstencil; ++k) sp[k + 1] += sp[k];
+  }
+  __syncthreads();
+  const int total = sp[a.nstencil];
+  const float Lx = (float)gr.Lx, Ly = (float)gr.Ly, Lz = (float)gr.Lz;
+  float4 pv[BL_SLOTS];
+  int2 pt[BL_SLOTS];
+  auto fetch = [&](int base) {
+    const int nch = min(BL_CH, total - base);
+#pragma unroll
+    for (int r = 0; r < BL_SLOTS; ++r) {
+      const int c = tid + r * BL_THREADS;
+      if (c < nch) {
+        const int idx = base + c;
+        int lo = 0, hi = a.nstencil - 1;  // last stencil cell whose prefix <= idx
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (sp[mid] <= idx) lo = mid;
+          else hi = mid - 1;
+        }
+        const int t = sb[lo] + (idx - sp[lo]);
+        float4 v = xq[t];
+        const int code = shc[lo];
+        v.x += (float)((code & 3) - 1) * Lx;
+        v.y += (float)(((code >> 2) & 3) - 1) * Ly;
+        v.z += (float)(((code >> 4) & 3) - 1) * Lz;
+        pv[r] = v;
+        pt[r] = make_int2(t, sorted_orig[t]);
+      }
+    }
+  };
+  auto store = [&](int base) {
+    const int nch = min(BL_CH, total - base);
+    const int npad = min(BL_CH, (nch + BLF_PASS - 1) / BLF_PASS * BLF_PASS);
+#pragma unroll
+    for (int r = 0; r < BL_SLOTS; ++r) {
+      const int c = tid + r * BL_THREADS;
+      if (c < nch) {
+        cxq[c] = pv[r];
+        cto[c] = pt[r];
+      } else if (c < npad) {  // far away: fails the distance test
+        cxq[c] = make_float4(1e30f, 1e30f, 1e30f, 0.f);
+        cto[c] = make_int2(0, 0);
+      }
+    }
+  };
+  if (total > 0) fetch(0);
+  store(0);
+  __syncthreads();
+  const int lane = tid & 63, w = tid >> 6;
+  constexpr int nw = BL_THREADS / 64;
+  int* my_ex = exl[w];
+  int* my_s4 = s4l[w];
+  for (int base = 0; base < total; base += BL_CH) {
+    const int nch = min(BL_CH, total - base);
+    const bool more = base + BL_CH < total;
+    if (more) fetch(base + BL_CH);  // in flight during this chunk's scan
+    for (int h = w; h < hn; h += nw) {  // wave-uniform
+      const int s = hb + h;
+      int o;
+      float4 xi;
+      unsigned long long exm, s4m;
+      bool masked;
+      if (fit) {
+        o = ho[h];
+        xi = hxi[h];
+        exm = hexm[h];
+        s4m = hs4m[h];
+        masked = hbad[h] == 0;
+      } else {
+        o = sorted_orig[s];
+        xi = xq[s];
+        exm = ex_win[o];
+        s4m = s4_win[o];
+        masked = win_bad[o] == 0;
+      }
+      unsigned* row = nbr_list + (long long)s * a.cap;
+      int cnt = acnt[h];
+      if (masked) {
+        const unsigned o32 = (unsigned)(o - 32);
+        for (int cb = 0; cb < nch; cb += BLF_PASS) {
+          float4 xj[BL_U];
+          int2 tj[BL_U];
+#pragma unroll
+          for (int u = 0; u < BL_U; ++u) {
+            xj[u] = cxq[cb + u * 64 + lane];
+            tj[u] = cto[cb + u * 64 + lane];
+          }
+#pragma unroll
+          for (int u = 0; u < BL_U; ++u) {
+            const float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
+            const unsigned dw = min((unsigned)tj[u].y - o32, 63u);
+            const bool acc = dx * dx + dy * dy + dz * dz < a.rl2 && !((exm >> dw) & 1ull);
+            const unsigned long long m = __ballot(acc);
+            const int slot = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            if (acc && slot < a.cap) row[slot] = (unsigned)tj[u].x | ((unsigned)((s4m >> dw) & 1ull) << 31);
+            cnt += __popcll(m);
+          }
+        }
+      } else {  // a partner outside the index window: the CSR lists
+        const int e0 = excl_ptr ? excl_ptr[o] : 0, t0s = s14_ptr ? s14_ptr[o] : 0;
+        const int nex_all = excl_ptr ? excl_ptr[o + 1] - e0 : 0, ns4_all = s14_ptr ? s14_ptr[o + 1] - t0s : 0;
+        const int nex = min(nex_all, BL_MAXEX), ns4 = min(ns4_all, BL_MAXEX);
+        if (lane < nex) my_ex[lane] = excl_idx[e0 + lane];
+        if (lane < ns4) my_s4[lane] = s14_idx[t0s + lane];
+        if (lane == 0 && (nex_all > BL_MAXEX || ns4_all > BL_MAXEX)) atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
+        wave_lds_sync();
+        for (int cb = 0; cb < nch; cb += 64) {
+          const float4 xj = cxq[cb + lane];
+          const int2 tj = cto[cb + lane];
+          const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+          bool acc = dx * dx + dy * dy + dz * dz < a.rl2 && tj.x != s;
+          bool s14 = false;
+          for (int r = 0; r < nex; ++r) acc &= (my_ex[r] != tj.y);
+          for (int r = 0; r < ns4; ++r) s14 |= (my_s4[r] == tj.y);
+          const unsigned long long m = __ballot(acc);
+          const int slot = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+          if (acc && slot < a.cap) row[slot] = (unsigned)tj.x | (s14 ? 0x80000000u : 0u);
+          cnt += __popcll(m);
+        }
+        wave_lds_sync();  // the wave's next home atom overwrites my_ex / my_s4
+      }
+      if (lane == 0) acnt[h] = cnt;
+    }
+    __syncthreads();
+    if (more) {
+      store(base + BL_CH);
+      __syncthreads();
+    }
   }
   for (int h = tid; h < hn; h += blockDim.x) {
     const int cnt = acnt[h];
@@ -833,7 +1065,17 @@ int mdx_nlist_update_list(mdx_ctx* c) {
   for (int d = 0; d < 3; ++d)
     if (2 * c->reach[d] + 1 >= c->gn[d]) a.shifted = 0;
   const bool win = c->win_ready && c->h_win_bad.size() == (size_t)c->n;
-  auto build = a.exact64 ? k_build_list_cell<true> : k_build_list_cell<false>;
+  const bool fastk = a.shifted && !a.ref_mode && win;
+  if (fastk && !a.exact64) {
+    k_build_list_fast<<<c->ncell, BL_THREADS, 0, c->stream>>>(
+        a, c->xq.p, c->sorted_orig.p, c->cell_start.p, c->cell_count.p, c->d_stencil.p,
+        c->has_excl ? c->excl_ptr.p : nullptr, c->excl_idx.p, c->has_s14 ? c->s14_ptr.p : nullptr, c->s14_idx.p,
+        c->ex_win.p, c->s4_win.p, c->win_bad.p, c->nbr_count.p, c->nbr_list.p, st, thr2);
+    MDX_LAUNCH_CHECK();
+    return 0;
+  }
+  auto build = a.exact64 ? (fastk ? k_build_list_cell<true, true> : k_build_list_cell<true, false>)
+                         : (fastk ? k_build_list_cell<false, true> : k_build_list_cell<false, false>);
   build<<<c->ncell, BL_THREADS, 0, c->stream>>>(
       a, c->xq.p, c->sorted_orig.p, c->cell_start.p, c->cell_count.p, c->d_stencil.p,
       c->has_excl ? c->excl_ptr.p : nullptr, c->excl_idx.p, c->has_s14 ? c->s14_ptr.p : nullptr, c->s14_idx.p,
