@@ -915,12 +915,7 @@ int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy) {
     MDX_CHECK(!lj || c->has_lj, MDX_ERR_INVALID, "LJ requested but no LJ parameters set");
     MDX_CHECK(!cd || (c->has_charges && c->pme_configured), MDX_ERR_INVALID,
               "PME direct space requested but charges/PME not configured");
-    // inside an integrator step the bonded terms share the list-fill launch
-    c->fill_bonded_terms = c->prologue_done ? terms : 0u;
-    c->bonded_done = false;
-    rc = mdx_nlist_update_order(c, false);
-    c->fill_bonded_terms = 0;
-    if (rc) return rc;
+    if ((rc = mdx_nlist_update_order(c, false))) return rc;
   }
   // the PME plane kernels read only the cell-sorted order and packed positions, final after the
   // first part of the list update: run them on the side stream, concurrently with the list build
@@ -937,8 +932,12 @@ int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy) {
     if (rc) return rc;
     MDX_HIP(hipEventRecord(c->ev_join, c->side_stream));
   }
+  // the bonded terms ride in the nonbonded launch as extra workgroups (off the critical path:
+  // they need only positions and add into the fixed-point accumulator)
+  const unsigned bonded = terms & (MDX_TERM_BOND | MDX_TERM_ANGLE | MDX_TERM_DIHEDRAL | MDX_TERM_IMPROPER |
+                                   MDX_TERM_COUL_RECIP);
   if (lj || cd) {
-    if ((rc = mdx_nlist_update_list(c)) || (rc = mdx_enqueue_nonbonded(c, lj, cd, energy))) return rc;
+    if ((rc = mdx_nlist_update_list(c)) || (rc = mdx_enqueue_nonbonded(c, lj, cd, energy, bonded))) return rc;
   }
   if (terms & MDX_TERM_COUL_RECIP) {
     if (!fork_pme) {
@@ -959,8 +958,7 @@ int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy) {
   if (terms & MDX_TERM_MLP) {
     if ((rc = mdx_mlp_enqueue(c, energy))) return rc;
   }
-  if (!c->bonded_done && (rc = mdx_bonded_enqueue(c, terms))) return rc;
-  c->bonded_done = false;
+  if (!(lj || cd) && (rc = mdx_bonded_enqueue(c, terms))) return rc;
   if (fork_pme) MDX_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   return 0;
 }

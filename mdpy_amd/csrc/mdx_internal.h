@@ -181,8 +181,6 @@ struct mdx_ctx {
   bool list_valid = false;
   bool prologue_done = false;    // the integrator's fused kernel already checked/zeroed/packed this step
   bool prologue_binned = false;  // ... and (on rebuild steps) counted, scanned and scattered the atoms
-  unsigned fill_bonded_terms = 0;  // bonded terms to launch together with the list fill (0: none)
-  bool bonded_done = false;        // this evaluation's bonded terms were launched with the fill
   // exclusion / 1-4 partners as 64-bit masks over the index window i-32 .. i+31 (list build);
   // win_bad[i] bit 0 / 1: atom i has an exclusion / 1-4 partner outside the window
   DevBuf<unsigned long long> ex_win, s4_win;
@@ -375,13 +373,15 @@ int mdx_nlist_enqueue_pack(mdx_ctx* c);
 int mdx_nlist_enqueue_clear(mdx_ctx* c);
 int mdx_enqueue_request_rebuild(mdx_ctx* c);
 // forces
-int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy);
+// nonbonded launch; the bonded terms in `bonded_terms` (bond/angle/dihedral/improper bits, and the
+// PME exclusion correction with MDX_TERM_COUL_RECIP) ride along as extra workgroups
+int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy, unsigned bonded_terms = 0u);
 // Rebuild the nonbonded kernel's Ewald-correction table when beta or the cutoff changed.
 #define MDX_CTAB_N 256
 int mdx_coulomb_table_update(mdx_ctx* c);
 int mdx_enqueue_coulomb_plain(mdx_ctx* c);
 int mdx_pme_enqueue(mdx_ctx* c, bool energy);
-int mdx_bonded_enqueue(mdx_ctx* c, unsigned terms, const FillArgs* fill = nullptr);
+int mdx_bonded_enqueue(mdx_ctx* c, unsigned terms);
 int mdx_enqueue_zero_accumulators(mdx_ctx* c);
 int mdx_enqueue_forces_out(mdx_ctx* c, double* d_out);
 int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy);

@@ -1029,14 +1029,8 @@ int mdx_nlist_update_order(mdx_ctx* c, bool force_rebuild) {
   }
   bool ref = c->pair_mode == MDX_PAIRS_REFERENCE;
   const FillArgs fa = fill_args(c, thr2);
-  if (c->fill_bonded_terms) {  // the bonded terms of this evaluation share the fill launch
-    int rc0 = mdx_bonded_enqueue(c, c->fill_bonded_terms, &fa);
-    if (rc0) return rc0;
-    c->bonded_done = true;
-  } else {
-    k_grid_sort_fill<<<c->ncell, 256, 0, c->stream>>>(fa);
-    MDX_LAUNCH_CHECK();
-  }
+  k_grid_sort_fill<<<c->ncell, 256, 0, c->stream>>>(fa);
+  MDX_LAUNCH_CHECK();
   return pro ? 0 : mdx_nlist_enqueue_pack(c);
 }
 

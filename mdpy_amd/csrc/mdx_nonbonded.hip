@@ -15,7 +15,7 @@
 // evaluated from both ends (full list), so there are no atomics in the pair loop; the group
 // reduces with xor shuffles and adds each atom's force once to the int64 fixed-point
 // accumulator (deterministic).
-#include "mdx_internal.h"
+#include "mdx_bonded_terms.h"
 
 #define NB_BLOCK 256
 #define NB_U 4  // list entries per lane per pass
@@ -131,7 +131,12 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, const float4* 
                                                       const unsigned* __restrict__ nbr_list,
                                                       const int* __restrict__ sorted_orig,
                                                       const double* __restrict__ pos, long long* __restrict__ ffix,
-                                                      long long* __restrict__ efix, DevState* st) {
+                                                      long long* __restrict__ efix, DevState* st,
+                                                      BondedArgs ba, int nb_blocks) {
+  if ((int)blockIdx.x >= nb_blocks) {  // trailing workgroups: bonded terms
+    bonded_block(ba, blockIdx.x - nb_blocks);
+    return;
+  }
   constexpr bool TAB = COUL && !ENERGY;
 #if NB_TAB_GLOBAL
   const float4* __restrict__ ctab = a.ctab;
@@ -267,7 +272,7 @@ __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, const float4* 
   }
 }
 
-int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy) {
+int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy, unsigned bonded_terms) {
   NbArgs a;
   a.n = c->n;
   a.cap = c->nbr_cap;
@@ -292,7 +297,11 @@ int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy) {
   a.Lzd = c->box[2];
   a.rcd = c->rc;
   const int lanes = c->nb_lanes;
-  dim3 grid(div_up((long long)c->n * lanes, NB_BLOCK));
+  static_assert(NB_BLOCK == BD_BLOCK, "bonded workgroups share the nonbonded launch");
+  BondedArgs ba{};
+  const int nbb = (int)div_up((long long)c->n * lanes, NB_BLOCK);
+  const int nbo = bonded_terms ? mdx_bonded_args(c, bonded_terms, &ba) : 0;
+  dim3 grid(nbb + nbo);
 #define NB_LAUNCH(L_, C_, E_)                                                                          \
   do {                                                                                                 \
     if (lanes == 16)                                                                                   \
@@ -303,7 +312,7 @@ int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy) {
       k_nonbonded<L_, C_, E_, 32><<<grid, NB_BLOCK, 0, c->stream>>>(NB_ARGS);                          \
   } while (0)
 #define NB_ARGS a, c->xq.p, c->ljs.p, c->ljs14.p, c->nbr_count.p, c->nbr_list.p, c->sorted_orig.p, c->pos, \
-                c->ffix.p, c->efix.p, c->dstate.p
+                c->ffix.p, c->efix.p, c->dstate.p, ba, nbb
   if (lj && coul) {
     if (energy) NB_LAUNCH(true, true, true); else NB_LAUNCH(true, true, false);
   } else if (lj) {
