@@ -126,6 +126,14 @@ int mdx_pme_config(mdx_ctx* ctx, const int* h_grid, int order, double beta);
  * MI355X its barriers (cache write-back and invalidation across the XCDs) cost more than the
  * launch gaps they remove, so it stays off unless selected. */
 int mdx_pme_set_fused(mdx_ctx* ctx, int enable);
+/* Phase launches (enable != 0; no reference counterpart): force evaluations of LJ + PME run the
+ * three reciprocal launches on the context's stream, each also running a share of the
+ * nonbonded workgroups (frac0 of them with the spread/2D FFT launch, frac1 with the x-axis
+ * launch, the rest with the gather launch, which also carries the bonded terms) instead of the
+ * nonbonded kernel on the main stream and the reciprocal chain on a side stream joined after.
+ * Same forces as the two-stream path up to float32 summation order within an atom's partial
+ * sums (the nonbonded workgroups are wider). Ignored where the plane PME path is not used. */
+int mdx_force_phases_config(mdx_ctx* ctx, int enable, double frac0, double frac1);
 
 /* ---- force evaluation (Constraint.update / Ensemble.update, ensemble.py:53-61) ------------ */
 /* Evaluate the terms in `terms` at the current positions. If d_force_out != NULL, write the

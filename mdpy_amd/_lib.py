@@ -79,6 +79,7 @@ SIGNATURES = {
     "mdx_neighbor_pairs": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, _c_i64_p]),
     "mdx_pme_config": (ctypes.c_int, [_vp, _c_int_p, ctypes.c_int, ctypes.c_double]),
     "mdx_pme_set_fused": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "mdx_force_phases_config": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     "mdx_compute_forces": (ctypes.c_int, [_vp, ctypes.c_uint, _vp]),
     "mdx_compute_forces_only": (ctypes.c_int, [_vp, ctypes.c_uint, _vp]),
     "mdx_energies_fetch": (ctypes.c_int, [_vp, _c_dbl_p]),

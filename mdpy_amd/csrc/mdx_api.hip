@@ -917,6 +917,20 @@ int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy) {
               "PME direct space requested but charges/PME not configured");
     if ((rc = mdx_nlist_update_order(c, false))) return rc;
   }
+  const unsigned bonded = terms & (MDX_TERM_BOND | MDX_TERM_ANGLE | MDX_TERM_DIHEDRAL | MDX_TERM_IMPROPER |
+                                   MDX_TERM_COUL_RECIP);
+  if (lj && cd && (terms & MDX_TERM_COUL_RECIP) && mdx_force_phases_ok(c)) {
+    // one stream: list build, then the PME phases each carrying a share of the nonbonded work
+    if ((rc = mdx_nlist_update_list(c)) || (rc = mdx_force_phases_enqueue(c, energy, bonded))) return rc;
+    if (terms & MDX_TERM_COUL_PLAIN) {
+      MDX_CHECK(c->has_charges, MDX_ERR_INVALID, "Coulomb requested but no charges set");
+      if ((rc = mdx_enqueue_coulomb_plain(c))) return rc;
+    }
+    if (terms & MDX_TERM_MLP) {
+      if ((rc = mdx_mlp_enqueue(c, energy))) return rc;
+    }
+    return 0;
+  }
   // the PME plane kernels read only the cell-sorted order and packed positions, final after the
   // first part of the list update: run them on the side stream, concurrently with the list build
   // (all of a rebuild step's ~160 us), the nonbonded and the bonded kernels; join before returning
@@ -934,8 +948,6 @@ int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy) {
   }
   // the bonded terms ride in the nonbonded launch as extra workgroups (off the critical path:
   // they need only positions and add into the fixed-point accumulator)
-  const unsigned bonded = terms & (MDX_TERM_BOND | MDX_TERM_ANGLE | MDX_TERM_DIHEDRAL | MDX_TERM_IMPROPER |
-                                   MDX_TERM_COUL_RECIP);
   if (lj || cd) {
     if ((rc = mdx_nlist_update_list(c)) || (rc = mdx_enqueue_nonbonded(c, lj, cd, energy, bonded))) return rc;
   }

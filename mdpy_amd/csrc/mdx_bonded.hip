@@ -6,7 +6,7 @@ __global__ __launch_bounds__(BD_BLOCK) void k_bonded(BondedArgs a) { bonded_bloc
 
 // arguments of the bonded terms of `terms` plus, with MDX_TERM_COUL_RECIP, the PME
 // exclusion/self/background part; returns the number of workgroups they take
-int mdx_bonded_args(const mdx_ctx* c, unsigned terms, BondedArgs* out) {
+int mdx_bonded_args(const mdx_ctx* c, unsigned terms, BondedArgs* out, int block) {
   BondedArgs& a = *out;
   a.n = c->n;
   for (int q = 0; q < 3; ++q) {
@@ -24,7 +24,7 @@ int mdx_bonded_args(const mdx_ctx* c, unsigned terms, BondedArgs* out) {
   int total = 0;
   for (int k = 0; k < 4; ++k) {
     a.cnt[k] = on[k] ? counts[k] : 0;
-    a.blocks[k] = div_up(a.cnt[k], BD_BLOCK);
+    a.blocks[k] = div_up(a.cnt[k], block);
     a.idx[k] = idx[k];
     a.par[k] = par[k];
     total += a.blocks[k];
@@ -32,7 +32,7 @@ int mdx_bonded_args(const mdx_ctx* c, unsigned terms, BondedArgs* out) {
   a.refcompat = c->dih_refcompat;
   const bool recip = (terms & MDX_TERM_COUL_RECIP) != 0;
   a.cnt[4] = recip ? c->n_excl_pairs : 0;
-  a.blocks[4] = recip ? std::max(1, div_up(c->n_excl_pairs, BD_BLOCK)) : 0;
+  a.blocks[4] = recip ? std::max(1, div_up(c->n_excl_pairs, block)) : 0;
   a.idx[4] = c->excl_pairs.p;
   a.par[4] = nullptr;
   total += a.blocks[4];

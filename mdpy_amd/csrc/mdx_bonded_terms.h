@@ -49,13 +49,13 @@ __device__ __forceinline__ void add_force(long long* __restrict__ ffix, int n, i
 }
 
 static __device__ void block_energy(double e, long long* __restrict__ slot) {
-  __shared__ double sm[BD_BLOCK / 64];
+  __shared__ double sm[16];  // up to 1024 lanes
   for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = e;
   __syncthreads();
   if (threadIdx.x == 0) {
     double s = 0;
-    for (int w = 0; w < BD_BLOCK / 64; ++w) s += sm[w];
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sm[w];
     atomicAdd((unsigned long long*)slot, (unsigned long long)to_fix(s));
   }
 }
@@ -226,7 +226,7 @@ struct BondedArgs {
 __device__ __forceinline__ void bonded_block(const BondedArgs& a, int blk) {
   int seg = 0;
   while (seg < 4 && blk >= a.blocks[seg]) blk -= a.blocks[seg++];
-  const int t = blk * BD_BLOCK + threadIdx.x;
+  const int t = blk * (int)blockDim.x + threadIdx.x;  // workgroups of `block` lanes (mdx_bonded_args)
   double e = 0.0;
   if (t < a.cnt[seg]) {
     switch (seg) {
@@ -242,5 +242,5 @@ __device__ __forceinline__ void bonded_block(const BondedArgs& a, int blk) {
 }
 
 // arguments of the bonded terms of `terms` (and the PME exclusion part with MDX_TERM_COUL_RECIP);
-// returns the number of BD_BLOCK workgroups bonded_block expects
-int mdx_bonded_args(const mdx_ctx* c, unsigned terms, BondedArgs* out);
+// returns the number of workgroups of `block` lanes that bonded_block expects
+int mdx_bonded_args(const mdx_ctx* c, unsigned terms, BondedArgs* out, int block = BD_BLOCK);

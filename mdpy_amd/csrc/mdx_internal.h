@@ -211,6 +211,11 @@ struct mdx_ctx {
   int pme_fused_B = 0;           // x-axis lines per chunk of the one-launch PME kernel (0: not usable)
   size_t pme_fused_lds = 0;      // its dynamic LDS
   bool pme_fused_on = false;     // one-launch PME kernel selected (never inside a batch)
+  // phase launches: the three PME reciprocal kernels each carry a share of the nonbonded
+  // workgroups (and the last one the bonded terms), all on one stream (no fork and join)
+  bool phases_on = false;
+  double phase_frac[2] = {0.26, 0.13};  // nonbonded shares of the first two launches
+  int pme_wf_phase = 0;                 // forward-plane candidate window under the phase kernels' LDS
   bool in_batch = false;         // advanced by an mdx_batch
   bool fpart_live = false;       // fpart belongs to the current evaluation
   // ---- integrator
@@ -369,6 +374,10 @@ int mdx_nlist_prepare(mdx_ctx* c);                    // allocate grid/list buff
 int mdx_nlist_update(mdx_ctx* c, bool force_rebuild);  // enqueue displacement check + conditional rebuild + pack
 int mdx_nlist_update_order(mdx_ctx* c, bool force_rebuild);  // its first part: up to the sorted arrays and pack
 int mdx_nlist_update_list(mdx_ctx* c);                       // its second part: the list build
+// the PME reciprocal chain as three launches that also run the nonbonded kernel (LJ + direct
+// Coulomb, in shares) and the bonded terms of `bonded_terms`; true if the context can
+bool mdx_force_phases_ok(const mdx_ctx* c);
+int mdx_force_phases_enqueue(mdx_ctx* c, bool energy, unsigned bonded_terms);
 int mdx_nlist_enqueue_pack(mdx_ctx* c);
 int mdx_nlist_enqueue_clear(mdx_ctx* c);
 int mdx_enqueue_request_rebuild(mdx_ctx* c);

@@ -1,43 +1,12 @@
-// Nonbonded direct-space kernel (CHARMM LJ + erfc-screened Coulomb) and the plain all-pairs
-// Coulomb kernel, gfx950.
-//
-// LJ physics follows mdpy/constraint/charmm_nonbonded_constraint.py:88-107:
-//   eps = sqrt(eps_i eps_j), sigma = (sigma_i + sigma_j)/2 ((eps14, sigma14) for 1-4 partners),
-//   F_i = -24 eps (2 (s/r)^12 - (s/r)^6) / r^2 * d,  E = 4 eps ((s/r)^12 - (s/r)^6),  r <= rc,
-//   with d = minimum-image x_j - x_i.
-// The Coulomb part is the PME direct space q_i q_j erfc(beta r)/r over the same pair set.
-// Plain Coulomb follows mdpy/constraint/electrostatic_constraint.py:52-79.
-//
-// Layout: a group of G lanes per atom (16, 32 or 64, chosen by mdx_nonbonded_tune) in
-// cell-sorted order walks the atom's contiguous
-// neighbour list (coalesced index loads), gathers each neighbour's float4 (x, y, z, q*sqrt(ke))
-// and float2 (sigma/2, sqrt(eps)), and keeps partial forces in registers. Every pair is
-// evaluated from both ends (full list), so there are no atomics in the pair loop; the group
-// reduces with xor shuffles and adds each atom's force once to the int64 fixed-point
-// accumulator (deterministic).
+// Nonbonded launch (the pair evaluation of mdx_nonbonded_body.h, with the bonded terms as
+// trailing workgroups), the Ewald-correction table, and the plain all-pairs Coulomb kernel
+// (mdpy/constraint/electrostatic_constraint.py:52-79), gfx950.
 #include "mdx_bonded_terms.h"
+#include "mdx_nonbonded_body.h"
 
 #define NB_BLOCK 256
-#define NB_U 4  // list entries per lane per pass
 
-struct NbArgs {
-  int n, cap;
-  float Lx, Ly, Lz, ix, iy, iz;
-  float rc2, guard2;
-  float beta, twob_sqrtpi;
-  float ct_scale;  // Ewald-correction table pieces per A^2
-  double Lxd, Lyd, Lzd, rcd;
-  const float4* ctab;
-};
-
-// Forces-only launches evaluate the direct-space Coulomb force as
-//   F_ij / (q_i q_j) = d (1/r^3 - G(r^2)),  G(z) = erf(beta r)/r^3 - (2 beta/sqrt(pi)) exp(-beta^2 z)/z,
-// which equals erfc(beta r)/r^3 + (2 beta/sqrt(pi)) exp(-beta^2 r^2)/r^2. G is smooth and finite
-// on [0, rc^2] (G(0) = 4 beta^3/(3 sqrt(pi))), so MDX_CTAB_N cubic Hermite pieces in z = r^2,
-// built in float64 from G and G' at the knots, reproduce it to ~1e-9 relative; each pair then
-// needs one LDS read and three FMAs instead of exp, a reciprocal and the erfc polynomial. The
-// subtraction leaves an absolute error of ~1 ulp of 1/r^3 per pair (relative error grows
-// towards the cutoff, where the force itself is small). Energy launches keep the erfc form.
+// G(z) of the forces-only Coulomb path and its derivative, float64, for the table
 static void coulomb_correction(double beta, double z, double* G, double* dG) {
   const double k = 2.0 * beta * beta * beta / sqrt(M_PI);
   const double x2 = beta * beta * z;
@@ -88,191 +57,16 @@ int mdx_coulomb_table_update(mdx_ctx* c) {
   return 0;
 }
 
-// erfc(x) = exp(-x^2) t P(t), t = 1/(1 + p x): the form of Abramowitz & Stegun 7.1.26 refit
-// (mdpy_amd/tools/fit_erfc.py) to a 7-term polynomial with max relative error 4.5e-8 on
-// [0, 3.6] (beta * rc = 3.29 for the 1e-5 Ewald tolerance), so pairs near the cutoff, where
-// erfc ~ 1e-5, keep float32 relative accuracy.
-#define ERFC_P 0.355f
-#define ERFC_C1 1.9758294902e-01f
-#define ERFC_C2 2.3220149337e-01f
-#define ERFC_C3 2.8235364196e-02f
-#define ERFC_C4 5.9699421939e-01f
-#define ERFC_C5 -5.6119238634e-01f
-#define ERFC_C6 6.9343106043e-01f
-#define ERFC_C7 -1.8725274521e-01f
-
-// float64 cutoff decision with the reference's minimum-image sequence (pbc.py:40-44)
-__device__ __noinline__ bool within_cutoff64(const double* __restrict__ pos, int oi, int oj, double Lx, double Ly,
-                                              double Lz, double rc) {
-  double L[3] = {Lx, Ly, Lz};
-  double d[3];
-  for (int q = 0; q < 3; ++q) {
-    double v = pos[3 * oj + q] - pos[3 * oi + q];
-    double sv = v * (1.0 / L[q]);
-    sv -= rint(sv);
-    d[q] = sv * L[q];
-  }
-  return sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) <= rc;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// G lanes share one atom: lane g walks list entries g, g + G, ... of the atom's
-// contiguous (row-major) list, so a group's index loads and its neighbours' position loads
-// are contiguous; the group reduces its partial forces with xor shuffles at the end.
 template <bool LJ, bool COUL, bool ENERGY, int G>
-__global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, const float4* __restrict__ xq,
-                                                      const float2* __restrict__ ljs,
-                                                      const float2* __restrict__ ljs14,
-                                                      const int* __restrict__ nbr_count,
-                                                      const unsigned* __restrict__ nbr_list,
-                                                      const int* __restrict__ sorted_orig,
-                                                      const double* __restrict__ pos, long long* __restrict__ ffix,
-                                                      long long* __restrict__ efix, DevState* st,
-                                                      BondedArgs ba, int nb_blocks) {
+__global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, NbPtrs P, BondedArgs ba, int nb_blocks) {
   if ((int)blockIdx.x >= nb_blocks) {  // trailing workgroups: bonded terms
     bonded_block(ba, blockIdx.x - nb_blocks);
     return;
   }
-  constexpr bool TAB = COUL && !ENERGY;
-#if NB_TAB_GLOBAL
-  const float4* __restrict__ ctab = a.ctab;
-#else
-  __shared__ float4 ctab[TAB ? MDX_CTAB_N : 1];
-  if (TAB) {
-    for (int k = threadIdx.x; k < MDX_CTAB_N; k += blockDim.x) ctab[k] = a.ctab[k];
-    __syncthreads();
-  }
-#endif
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = gt / G;
-  const int g = gt & (G - 1);
-  if (gt == 0) {  // every rebuild kernel has finished reading these
-    st->rebuild = 0;
-    st->max_disp2_bits = 0;
-  }
-  float fx = 0.f, fy = 0.f, fz = 0.f;
-  double elj = 0.0, ec = 0.0;
-  if (s < a.n) {
-    const float4 xi = xq[s];
-    float2 pi = make_float2(0.f, 0.f), pi14 = pi;
-    if (LJ) {
-      pi = ljs[s];
-      pi14 = ljs14[s];
-    }
-    const int cnt = nbr_count[s];
-    const unsigned* lp = nbr_list + (long long)s * a.cap;
-    int oi = -1;
-    // NB_U list entries per lane per pass: all index loads, then all neighbour loads, are
-    // issued before any of them is consumed, so each pass waits for memory once, not 3*NB_U times
-    for (int kb = g; kb < cnt; kb += G * NB_U) {
-      unsigned e[NB_U];
-#pragma unroll
-      for (int u = 0; u < NB_U; ++u) {
-        const int k = kb + u * G;
-        e[u] = k < cnt ? lp[k] : (unsigned)s | 0x40000000u;  // bit 30: padding slot
-      }
-      float4 xj[NB_U];
-      float2 pj[NB_U];
-#pragma unroll
-      for (int u = 0; u < NB_U; ++u) {
-        const int t = (int)(e[u] & 0x3fffffffu);
-        xj[u] = xq[t];
-        if (LJ) pj[u] = ((e[u] >> 31) ? ljs14 : ljs)[t];
-      }
-#pragma unroll
-      for (int u = 0; u < NB_U; ++u) {
-        float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
-        dx -= a.Lx * rintf(dx * a.ix);
-        dy -= a.Ly * rintf(dy * a.iy);
-        dz -= a.Lz * rintf(dz * a.iz);
-        const float r2raw = dx * dx + dy * dy + dz * dz;
-        bool in = r2raw <= a.rc2 && !(e[u] & 0x40000000u);
-        if (fabsf(r2raw - a.rc2) < a.guard2 && !(e[u] & 0x40000000u)) {
-          // near the cutoff: decide in float64 like the reference (rare)
-          if (oi < 0) oi = sorted_orig[s];
-          in = within_cutoff64(pos, oi, sorted_orig[e[u] & 0x3fffffffu], a.Lxd, a.Lyd, a.Lzd, a.rcd);
-        }
-        const float r2 = in ? r2raw : 1.0f;
-        const float rinv = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32: r2 >= ~0.5 A^2, never denormal
-        const float r2inv = rinv * rinv;
-        float fr = 0.f;
-        if (LJ) {
-          const float2 pp = (e[u] >> 31) ? pi14 : pi;
-          const float sig = pp.x + pj[u].x;
-          const float eps = pp.y * pj[u].y;
-          const float s2 = sig * sig * r2inv;
-          const float s6 = s2 * s2 * s2;
-          const float s12 = s6 * s6;
-          fr -= 24.f * eps * (2.f * s12 - s6) * r2inv;
-          if (ENERGY) elj += in ? (double)(4.f * eps * (s12 - s6)) : 0.0;
-        }
-        if (TAB) {
-          const float zt = r2 * a.ct_scale;
-          const float fl = floorf(zt);
-          const float t = zt - fl;
-          const float4 cf = ctab[min((int)fl, MDX_CTAB_N - 1)];
-          const float gz = fmaf(t, fmaf(t, fmaf(t, cf.w, cf.z), cf.y), cf.x);
-          fr -= xi.w * xj[u].w * (rinv * r2inv - gz);
-        } else if (COUL) {
-          const float qq = xi.w * xj[u].w;
-          const float x = a.beta * (r2 * rinv);
-          const float ex = __expf(-x * x);
-          // 1/(1 + p x): v_rcp_f32 plus one Newton step (the raw 1-ulp reciprocal is biased, and the
-          // direct-space energy sum cancels strongly enough to show it)
-          const float den = 1.f + ERFC_P * x;
-          const float r0 = __builtin_amdgcn_rcpf(den);
-          const float tt = fmaf(r0, fmaf(-den, r0, 1.f), r0);
-          const float poly =
-              ERFC_C1 + tt * (ERFC_C2 + tt * (ERFC_C3 + tt * (ERFC_C4 + tt * (ERFC_C5 + tt * (ERFC_C6 + tt * ERFC_C7)))));
-          const float erfcx = tt * poly * ex;
-          fr -= qq * (erfcx * rinv + a.twob_sqrtpi * ex) * r2inv;
-          if (ENERGY) ec += in ? (double)(qq * erfcx * rinv) : 0.0;
-        }
-        fr = in ? fr : 0.f;
-        fx += fr * dx;
-        fy += fr * dy;
-        fz += fr * dz;
-      }
-    }
-  }
-  for (int off = G / 2; off > 0; off >>= 1) {
-    fx += __shfl_xor(fx, off, 64);
-    fy += __shfl_xor(fy, off, 64);
-    fz += __shfl_xor(fz, off, 64);
-  }
-  if (s < a.n && g == 0) {
-    const int o = sorted_orig[s];
-    atomicAdd((unsigned long long*)&ffix[o], (unsigned long long)to_fix_f(fx));
-    atomicAdd((unsigned long long*)&ffix[a.n + o], (unsigned long long)to_fix_f(fy));
-    atomicAdd((unsigned long long*)&ffix[2 * a.n + o], (unsigned long long)to_fix_f(fz));
-  }
-  if (ENERGY) {
-    __shared__ double sm[2][NB_BLOCK / 64];
-    elj = wave_sum(elj);
-    ec = wave_sum(ec);
-    if ((threadIdx.x & 63) == 0) {
-      sm[0][threadIdx.x >> 6] = elj;
-      sm[1][threadIdx.x >> 6] = ec;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double a0 = 0, a1 = 0;
-      for (int w = 0; w < NB_BLOCK / 64; ++w) {
-        a0 += sm[0][w];
-        a1 += sm[1][w];
-      }
-      // every pair was summed from both ends
-      if (LJ) atomicAdd((unsigned long long*)&efix[0], (unsigned long long)to_fix(0.5 * a0));
-      if (COUL) atomicAdd((unsigned long long*)&efix[1], (unsigned long long)to_fix(0.5 * a1));
-    }
-  }
+  nb_block<LJ, COUL, ENERGY, G, NB_BLOCK>(a, P, blockIdx.x);
 }
 
-int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy, unsigned bonded_terms) {
+NbArgs nb_args(const mdx_ctx* c) {
   NbArgs a;
   a.n = c->n;
   a.cap = c->nbr_cap;
@@ -290,12 +84,33 @@ int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy, unsigned 
   a.twob_sqrtpi = (float)(2.0 * c->beta / sqrt(M_PI));
   a.ct_scale = c->ctab_scale;
   a.ctab = c->ctab.p;
-  MDX_CHECK(!coul || (a.ctab && c->ctab_beta == c->beta && c->ctab_rc == c->rc), MDX_ERR_INVALID,
-            "Ewald-correction table not built for the current beta and cutoff");
   a.Lxd = c->box[0];
   a.Lyd = c->box[1];
   a.Lzd = c->box[2];
   a.rcd = c->rc;
+  return a;
+}
+
+NbPtrs nb_ptrs(const mdx_ctx* c) {
+  NbPtrs P;
+  P.xq = c->xq.p;
+  P.ljs = c->ljs.p;
+  P.ljs14 = c->ljs14.p;
+  P.nbr_count = c->nbr_count.p;
+  P.nbr_list = c->nbr_list.p;
+  P.sorted_orig = c->sorted_orig.p;
+  P.pos = c->pos;
+  P.ffix = c->ffix.p;
+  P.efix = c->efix.p;
+  P.st = c->dstate.p;
+  return P;
+}
+
+int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy, unsigned bonded_terms) {
+  const NbArgs a = nb_args(c);
+  const NbPtrs P = nb_ptrs(c);
+  MDX_CHECK(!coul || (a.ctab && c->ctab_beta == c->beta && c->ctab_rc == c->rc), MDX_ERR_INVALID,
+            "Ewald-correction table not built for the current beta and cutoff");
   const int lanes = c->nb_lanes;
   static_assert(NB_BLOCK == BD_BLOCK, "bonded workgroups share the nonbonded launch");
   BondedArgs ba{};
@@ -311,8 +126,7 @@ int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy, unsigned 
     else                                                                                               \
       k_nonbonded<L_, C_, E_, 32><<<grid, NB_BLOCK, 0, c->stream>>>(NB_ARGS);                          \
   } while (0)
-#define NB_ARGS a, c->xq.p, c->ljs.p, c->ljs14.p, c->nbr_count.p, c->nbr_list.p, c->sorted_orig.p, c->pos, \
-                c->ffix.p, c->efix.p, c->dstate.p, ba, nbb
+#define NB_ARGS a, P, ba, nbb
   if (lj && coul) {
     if (energy) NB_LAUNCH(true, true, true); else NB_LAUNCH(true, true, false);
   } else if (lj) {

@@ -11,6 +11,8 @@
 //   corrections  float64 excluded-pair erf correction, self energy and neutralising background,
 //                launched with the bonded terms (mdx_bonded.hip)
 #include "mdx_internal.h"
+#include "mdx_bonded_terms.h"
+#include "mdx_nonbonded_body.h"
 #include <cmath>
 
 #define PME_BLOCK 256
@@ -722,7 +724,7 @@ __device__ void plane_gather_body(const PlaneArgs& a, const FftPlan& plz, const 
     atomicAdd(&st->pme_slots, bcount);
     __threadfence();
     const unsigned t = atomicAdd(&st->pme_ticket, 1u);
-    if (t == gridDim.x - 1) {
+    if (t == (unsigned)a.Kx - 1) {  // the last of the Kx plane workgroups (launches may hold others)
       __threadfence();
       const int tot = atomicAdd(&st->pme_slots, 0);
       if (tot != 4 * a.n) atomicOr(&st->flags, MDX_FLAG_PME_COVERAGE);
@@ -795,6 +797,62 @@ __global__ __launch_bounds__(PLANE_THREADS) void k_pme_fused(
   grid_barrier(st);
   plane_gather_body(a, plz, ply, twz_g, twy_g, sorted_orig, cell_start, pos_sorted, xq, cg, grec, grec_n, fpart, st,
                     blockIdx.x, lds);
+}
+
+// Phase launches of one force evaluation, on one stream: workgroups [0, npme) run a PME
+// reciprocal phase (0: spread + 2D forward FFT per x-plane, 1: x-axis pass with the influence
+// function, 2: 2D inverse FFT + gather per x-plane), the next nbn run nonbonded workgroups
+// nb0 .. nb0 + nbn - 1 (PLANE_THREADS lanes each), the rest the bonded terms. The PME
+// workgroups come first, so they take compute units before the nonbonded ones; the nonbonded
+// shares are sized to end with their phase. This replaces the side stream of the three-kernel
+// path and its fork and join.
+struct PhaseArgs {
+  PlaneArgs a;
+  FftPlan plz, ply, plx;
+  FftLines lnx;
+  const float2 *twz, *twy, *twx;
+  const int* sorted_orig;
+  const int* cell_start;
+  const double* pos_sorted;
+  const float4* xq;
+  float2* cg;
+  const float* bc;
+  long long* eslot;
+  GatherRec* grec;
+  int* grec_n;
+  double* fpart;
+  DevState* st;
+};
+
+template <bool ENERGY, int G>
+__global__ __launch_bounds__(PLANE_THREADS) void k_force_phase(int phase, int npme, int nb0, int nbn, PhaseArgs p,
+                                                               NbArgs na, NbPtrs P, BondedArgs ba) {
+  extern __shared__ float2 lds[];
+  const int b = blockIdx.x;
+  if (b < npme) {
+    if (phase == 0)
+      plane_fwd_body(p.a, p.plz, p.ply, p.twz, p.twy, p.sorted_orig, p.cell_start, p.pos_sorted, p.xq, p.cg, p.grec,
+                     p.grec_n, p.st, b, lds);
+    else if (phase == 1)
+      fft_axis_body<-1, 3>(p.plx, p.lnx, p.twx, p.cg, nullptr, nullptr, p.bc, p.eslot, b * p.lnx.B, lds);
+    else
+      plane_gather_body(p.a, p.plz, p.ply, p.twz, p.twy, p.sorted_orig, p.cell_start, p.pos_sorted, p.xq, p.cg,
+                        p.grec, p.grec_n, p.fpart, p.st, b, lds);
+    return;
+  }
+  const int vb = b - npme;
+  if (vb < nbn) {
+    nb_block<true, true, ENERGY, G, PLANE_THREADS>(na, P, nb0 + vb);
+    return;
+  }
+  bonded_block(ba, vb - nbn);
+}
+
+typedef void (*PhaseKernel)(int, int, int, int, PhaseArgs, NbArgs, NbPtrs, BondedArgs);
+static PhaseKernel phase_kernel(bool energy, int G) {
+  if (G == 16) return energy ? k_force_phase<true, 16> : k_force_phase<false, 16>;
+  if (G == 64) return energy ? k_force_phase<true, 64> : k_force_phase<false, 64>;
+  return energy ? k_force_phase<true, 32> : k_force_phase<false, 32>;
 }
 
 // LDS bytes of the two plane kernels (the forward one for a record window of W candidates)
@@ -932,6 +990,27 @@ extern "C" int mdx_pme_config(mdx_ctx* c, const int* h_grid, int order, double b
     c->pme_fused_lds = std::max(lds_x, std::max(plane_fwd_lds(c->K[1], c->K[2], c->pme_wf),
                                                 plane_gather_lds(c->K[1], c->K[2])));
   }
+  // phase launches: their static LDS (the nonbonded workgroups' table) comes off the budget
+  c->pme_wf_phase = 0;
+  if (c->pme_planes && c->pme_fused_B > 0) {
+    size_t st_max = 0;
+    for (int e = 0; e < 2; ++e)
+      for (int G = 16; G <= 64; G *= 2) {
+        hipFuncAttributes fa;
+        MDX_HIP(hipFuncGetAttributes(&fa, (const void*)phase_kernel(e != 0, G)));
+        st_max = std::max(st_max, (size_t)fa.sharedSizeBytes);
+      }
+    const size_t pb = (size_t)lds_max > st_max + 256 ? (size_t)lds_max - st_max - 256 : 0;
+    for (int W = 8192; W >= PLANE_THREADS; W -= PLANE_THREADS)
+      if (!c->pme_wf_phase && plane_fwd_lds(c->K[1], c->K[2], W) <= pb) c->pme_wf_phase = W;
+    const size_t lx = (2 * (size_t)c->pme_fused_B * (c->K[0] + 1) + c->K[0]) * sizeof(float2);
+    if (plane_gather_lds(c->K[1], c->K[2]) > pb || lx > pb) c->pme_wf_phase = 0;
+    if (c->pme_wf_phase)
+      for (int e = 0; e < 2; ++e)
+        for (int G = 16; G <= 64; G *= 2)
+          MDX_HIP(hipFuncSetAttribute((const void*)phase_kernel(e != 0, G),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)pb));
+  }
   if ((rc = mdx_coulomb_table_update(c))) return rc;
   c->pme_configured = true;
   return 0;
@@ -1049,6 +1128,95 @@ static int pme_enqueue_planes(mdx_ctx* c) {
       a, plz, ply, twz, twy, c->sorted_orig.p, c->cell_start.p, c->pos_sorted.p, c->xq.p, c->cgrid.p, grec,
       c->pme_grec_n.p, c->fpart.p, c->dstate.p);
   MDX_LAUNCH_CHECK();
+  return 0;
+}
+
+bool mdx_force_phases_ok(const mdx_ctx* c) {
+  return c->phases_on && c->pme_configured && c->pme_planes && c->pme_wf_phase > 0 && c->pme_fused_B > 0 &&
+         c->nb_configured && c->has_lj && c->has_charges &&
+         (c->nb_lanes == 16 || c->nb_lanes == 32 || c->nb_lanes == 64);
+}
+
+int mdx_force_phases_enqueue(mdx_ctx* c, bool energy, unsigned bonded_terms) {
+  MDX_CHECK(mdx_force_phases_ok(c), MDX_ERR_INVALID, "phase launches not available for this context");
+  MDX_CHECK(c->fpart.p && c->fpart.n >= 12 * (size_t)c->n, MDX_ERR_INVALID, "PME configured for another system size");
+  const int Kx = c->K[0], Ky = c->K[1], Kz = c->K[2];
+  PhaseArgs p;
+  PlaneArgs& a = p.a;
+  a.n = c->n;
+  a.Kx = Kx;
+  a.Ky = Ky;
+  a.Kz = Kz;
+  a.Lx = c->box[0];
+  a.Ly = c->box[1];
+  a.Lz = c->box[2];
+  a.iLx = 1.0 / c->box[0];
+  a.iLy = 1.0 / c->box[1];
+  a.iLz = 1.0 / c->box[2];
+  a.gnx = c->gn[0];
+  a.gny = c->gn[1];
+  a.gnz = c->gn[2];
+  a.wx = c->box[0] / c->gn[0];
+  a.margin = 0.5 * c->skin + 1e-6 * c->box[0] + 1e-9;
+  a.rec_cap = c->pme_rec_cap;
+  a.W = c->pme_wf_phase;
+  MDX_CHECK(c->pme_grec.n >= (size_t)Kx * a.rec_cap * 6, MDX_ERR_INVALID, "PME configured for another system size");
+  factorize(Kz, &p.plz);
+  factorize(Ky, &p.ply);
+  factorize(Kx, &p.plx);
+  p.lnx.nlines = Ky * Kz;
+  p.lnx.inner = p.lnx.nlines;
+  p.lnx.inner_stride = 1;
+  p.lnx.outer_stride = 0;
+  p.lnx.estride = Ky * Kz;
+  p.lnx.B = c->pme_fused_B;
+  const int Kmax = std::max(Kx, std::max(Ky, Kz));
+  p.twx = c->twiddle.p;
+  p.twy = c->twiddle.p + (size_t)Kmax;
+  p.twz = c->twiddle.p + 2 * (size_t)Kmax;
+  p.sorted_orig = c->sorted_orig.p;
+  p.cell_start = c->cell_start.p;
+  p.pos_sorted = c->pos_sorted.p;
+  p.xq = c->xq.p;
+  p.cg = c->cgrid.p;
+  p.bc = c->bc.p;
+  p.eslot = c->efix.p + 2;
+  p.grec = (GatherRec*)c->pme_grec.p;
+  p.grec_n = c->pme_grec_n.p;
+  p.fpart = c->fpart.p;
+  p.st = c->dstate.p;
+  const NbArgs na = nb_args(c);
+  const NbPtrs P = nb_ptrs(c);
+  MDX_CHECK(na.ctab && c->ctab_beta == c->beta && c->ctab_rc == c->rc, MDX_ERR_INVALID,
+            "Ewald-correction table not built for the current beta and cutoff");
+  BondedArgs ba{};
+  const int nbo = bonded_terms ? mdx_bonded_args(c, bonded_terms, &ba, PLANE_THREADS) : 0;
+  const int G = c->nb_lanes;
+  const int nbt = (int)div_up((long long)c->n * G, PLANE_THREADS);
+  const int n0 = std::min(nbt, (int)(c->phase_frac[0] * nbt + 0.5));
+  const int n1 = std::min(nbt - n0, (int)(c->phase_frac[1] * nbt + 0.5));
+  const int n2 = nbt - n0 - n1;
+  const int nx = (p.lnx.nlines + p.lnx.B - 1) / p.lnx.B;  // x-pass workgroups
+  const size_t lx = (2 * (size_t)p.lnx.B * (Kx + 1) + Kx) * sizeof(float2);
+  PhaseKernel k = phase_kernel(energy, G);
+  k<<<Kx + n0, PLANE_THREADS, plane_fwd_lds(Ky, Kz, a.W), c->stream>>>(0, Kx, 0, n0, p, na, P, ba);
+  MDX_LAUNCH_CHECK();
+  k<<<nx + n1, PLANE_THREADS, lx, c->stream>>>(1, nx, n0, n1, p, na, P, ba);
+  MDX_LAUNCH_CHECK();
+  k<<<Kx + n2 + nbo, PLANE_THREADS, plane_gather_lds(Ky, Kz), c->stream>>>(2, Kx, n0 + n1, n2, p, na, P, ba);
+  MDX_LAUNCH_CHECK();
+  c->fpart_live = true;
+  return 0;
+}
+
+extern "C" int mdx_force_phases_config(mdx_ctx* c, int enable, double frac0, double frac1) {
+  MDX_CHECK(c, MDX_ERR_INVALID, "mdx_force_phases_config: no context");
+  MDX_CHECK(frac0 >= 0 && frac1 >= 0 && frac0 + frac1 <= 1, MDX_ERR_INVALID,
+            "nonbonded shares must be >= 0 and sum to at most 1");
+  c->phases_on = enable != 0;
+  c->phase_frac[0] = frac0;
+  c->phase_frac[1] = frac1;
+  mdx_invalidate_graph(c);
   return 0;
 }
 

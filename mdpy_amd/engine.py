@@ -118,6 +118,11 @@ class Engine:
         """Reciprocal space in one launch or in three (the default); the forces are the same."""
         L.call("mdx_pme_set_fused", self.ctx, 1 if enable else 0)
 
+    def set_force_phases(self, enable=True, frac0=0.26, frac1=0.13):
+        """LJ + PME evaluations as three launches on one stream, each PME phase carrying a share
+        of the nonbonded work (frac0, frac1, rest), instead of two streams joined per step."""
+        L.call("mdx_force_phases_config", self.ctx, 1 if enable else 0, float(frac0), float(frac1))
+
     def _bind(self):
         box = (ctypes.c_double * 3)(*self.box.tolist())
         L.call("mdx_system_set", self.ctx, self.n, ctypes.c_void_p(self.pos.data_ptr()),

@@ -265,6 +265,48 @@ def test_nonbonded_variants_match_oracle(name, request):
 
 
 @pytest.mark.parametrize("name", ["water648", "water23328", "chain"])
+def test_force_phases_equal_two_stream_path(name, request):
+    """Phase launches (the PME kernels carrying the nonbonded and bonded workgroups on one stream)
+    against the two-stream path: each atom's nonbonded partial sums run in the same lanes and
+    order, the PME spread is fixed-point and the gather per plane slot, so forces are bitwise
+    equal; energies differ only in the per-workgroup rounding of the block sums."""
+    from mdpy_amd import _lib as L
+    from mdpy_amd.engine import Engine
+    s = request.getfixturevalue(name)
+    say("case: force phases vs two streams,", name)
+    terms = sum(TERM_BITS.values())
+    e = Engine(s, skin=1.5)
+    e.set_positions(s.positions)
+    for lanes in (16, 64):
+        e.tune_nonbonded(L.TERM_LJ | L.TERM_COUL_DIRECT, lanes)
+        out = {}
+        for on in (False, True, True):
+            e.set_force_phases(on, 0.3, 0.1)
+            F, en = e.compute(terms)
+            Fo = e.compute_forces_only(terms).cpu().numpy().copy()
+            out.setdefault(on, []).append((F.cpu().numpy().copy(), dict(en), Fo))
+            assert e.status() == 0, (lanes, on)
+        (Fa, Ea, Foa), (Fb, Eb, Fob) = out[False][0], out[True][0]
+        assert np.array_equal(Fa, Fb) and np.array_equal(Foa, Fob), lanes
+        assert np.array_equal(out[True][0][0], out[True][1][0])  # repeatable
+        for k in ("lj", "coulomb_direct", "coulomb_reciprocal", "bond", "angle", "dihedral", "improper"):
+            assert abs(Ea[k] - Eb[k]) <= 4096 * 2.0 ** -40 + 1e-12 * abs(Ea[k]), (lanes, k, Ea[k], Eb[k])
+    # inside the integrator's graph: same trajectory as the two-stream path
+    e.tune_nonbonded(L.TERM_LJ | L.TERM_COUL_DIRECT, 16)
+    xs = []
+    for on in (False, True):
+        e.set_positions(s.positions)
+        e.set_velocities(np.zeros((e.n, 3)))
+        e.set_force_phases(on, 0.3, 0.1)
+        e.configure_integrator(L.INT_VELOCITY_VERLET, dt=0.5)
+        e.integrate(terms, 60, True, 20)
+        xs.append(e.positions().copy())
+        assert e.status() == 0, on
+    assert np.array_equal(xs[0], xs[1])
+    e.close()
+
+
+@pytest.mark.parametrize("name", ["water648", "water23328", "chain"])
 def test_pme_one_launch_equals_three_launches(name, request):
     """The one-launch reciprocal kernel (grid barrier between its phases) and the three-kernel
     path run the same arithmetic per plane and per pencil: forces bitwise equal, energies equal
