@@ -220,6 +220,8 @@ def main():
     ap.add_argument("--energy-every", type=int, default=50,
                     help="potential energies every this many steps (forces every step)")
     ap.add_argument("--lanes", type=int, default=0, help="nonbonded lanes per atom (0 = autotune)")
+    ap.add_argument("--persist", type=int, default=3,
+                    help="nonbonded workgroups per compute unit looping over the work (0: one per unit of work)")
     ap.add_argument("--phases", default="off",
                     help="'off': nonbonded and PME on two streams joined per step; 'f0,f1': the PME "
                          "launches carry the nonbonded work in shares f0, f1 and the rest, on one stream")
@@ -252,6 +254,7 @@ def main():
     terms = L.TERM_LJ | L.TERM_COUL_DIRECT | L.TERM_COUL_RECIP | L.TERM_BOND | L.TERM_ANGLE
     use_graph = not args.no_graph
 
+    e.set_nonbonded_persist(args.persist)
     phases = None
     if args.phases != "off":
         phases = [float(v) for v in args.phases.split(",")]
@@ -303,7 +306,7 @@ def main():
         "config": {"workload": "water23328: 7,776 TIP3P-like waters, 61.9 A cube, r_c 12 A, PME 64^3 "
                                "order 4, G-JF Langevin 300 K, dt 1 fs, plain FP32 path",
                    "atoms": s.num_atoms, "list_skin_A": args.skin, "hipgraph": use_graph, "nb_lanes": lanes,
-                   "energy_every": args.energy_every, "force_phases": phases,
+                   "energy_every": args.energy_every, "force_phases": phases, "nb_resident_per_cu": args.persist,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "kernel": "k_nonbonded (LJ + erfc Coulomb)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

@@ -228,6 +228,11 @@ int mdx_time_kernel(mdx_ctx* ctx, int which, unsigned terms, int reps, double* h
  * on the current neighbour list (`reps` forces-only launches of `terms`' LJ/direct parts) and
  * keep the fastest. The choice goes to *h_lanes (may be NULL). Synchronises. */
 int mdx_nonbonded_tune(mdx_ctx* ctx, unsigned terms, int lanes, int reps, int* h_lanes);
+/* Resident nonbonded workgroups (no reference counterpart): per_cu > 0 launches per_cu
+ * workgroups per compute unit that loop over the nonbonded work, so the PME kernels running
+ * beside it find free registers on every compute unit (default 3); 0 launches one workgroup per
+ * unit of work. Same forces either way. */
+int mdx_nonbonded_persist(mdx_ctx* ctx, int per_cu);
 /* Candidate (list) pair slots and in-cutoff unique pairs of the last nonbonded evaluation
  * (counts the list on the device; for roofline accounting; synchronises). */
 int mdx_pair_counts(mdx_ctx* ctx, int64_t* h_list_slots, int64_t* h_unique_in_cutoff);

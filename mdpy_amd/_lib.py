@@ -95,6 +95,7 @@ SIGNATURES = {
     "mdx_pair_counts": (ctypes.c_int, [_vp, _c_i64_p, _c_i64_p]),
     "mdx_time_kernel": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint, ctypes.c_int, _c_dbl_p]),
     "mdx_nonbonded_tune": (ctypes.c_int, [_vp, ctypes.c_uint, ctypes.c_int, ctypes.c_int, _c_int_p]),
+    "mdx_nonbonded_persist": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mdx_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _vp, ctypes.POINTER(_vp)]),
     "mdx_batch_integrate": (ctypes.c_int, [_vp, ctypes.c_uint, ctypes.c_int, ctypes.c_int]),
     "mdx_batch_destroy": (ctypes.c_int, [_vp]),

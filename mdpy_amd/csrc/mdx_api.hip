@@ -612,6 +612,7 @@ extern "C" int mdx_ctx_create(int device, void* hip_stream, mdx_ctx** out) {
   MDX_HIP(hipSetDevice(device));
   mdx_ctx* c = new mdx_ctx();
   c->device = device;
+  if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) c->ncu = 0;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
@@ -1568,6 +1569,13 @@ extern "C" int mdx_nonbonded_tune(mdx_ctx* c, unsigned terms, int lanes, int rep
   c->nb_lanes = best;
   mdx_invalidate_graph(c);
   if (h_lanes) *h_lanes = best;
+  return 0;
+}
+
+extern "C" int mdx_nonbonded_persist(mdx_ctx* c, int per_cu) {
+  MDX_CHECK(c && per_cu >= 0, MDX_ERR_INVALID, "mdx_nonbonded_persist: bad arguments");
+  c->nb_persist = c->ncu > 0 ? per_cu : 0;
+  mdx_invalidate_graph(c);
   return 0;
 }
 

@@ -177,6 +177,10 @@ struct mdx_ctx {
   DevBuf<unsigned> nbr_list;     // column major: [k * stride + s]
   int nbr_cap = 0;               // capacity per atom
   int nb_lanes = 32;             // lanes per atom in the nonbonded kernel (16, 32 or 64)
+  // nonbonded workgroups per compute unit looping over the work (0: one per unit of work); 3 (12
+  // waves of 123 VGPRs) leaves a quarter of every SIMD's registers to the PME kernels beside it
+  int nb_persist = 3;
+  int ncu = 0;                   // compute units of the device
   int list_stride = 0;
   bool list_valid = false;
   bool prologue_done = false;    // the integrator's fused kernel already checked/zeroed/packed this step

@@ -57,13 +57,15 @@ int mdx_coulomb_table_update(mdx_ctx* c) {
   return 0;
 }
 
+// Workgroups [0, nres) loop over the nb_blocks nonbonded workgroup indices (nres == nb_blocks:
+// one each); the trailing ones evaluate the bonded terms.
 template <bool LJ, bool COUL, bool ENERGY, int G>
-__global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, NbPtrs P, BondedArgs ba, int nb_blocks) {
-  if ((int)blockIdx.x >= nb_blocks) {  // trailing workgroups: bonded terms
-    bonded_block(ba, blockIdx.x - nb_blocks);
+__global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, NbPtrs P, BondedArgs ba, int nb_blocks, int nres) {
+  if ((int)blockIdx.x >= nres) {  // trailing workgroups: bonded terms
+    bonded_block(ba, blockIdx.x - nres);
     return;
   }
-  nb_block<LJ, COUL, ENERGY, G, NB_BLOCK>(a, P, blockIdx.x);
+  for (int vb = blockIdx.x; vb < nb_blocks; vb += nres) nb_block<LJ, COUL, ENERGY, G, NB_BLOCK>(a, P, vb);
 }
 
 NbArgs nb_args(const mdx_ctx* c) {
@@ -116,7 +118,10 @@ int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy, unsigned 
   BondedArgs ba{};
   const int nbb = (int)div_up((long long)c->n * lanes, NB_BLOCK);
   const int nbo = bonded_terms ? mdx_bonded_args(c, bonded_terms, &ba) : 0;
-  dim3 grid(nbb + nbo);
+  // resident workgroups: all of them, or nb_persist per compute unit looping over the rest (so
+  // that the concurrent PME kernels find room on every compute unit)
+  const int nres = c->nb_persist > 0 ? std::min(nbb, c->nb_persist * c->ncu) : nbb;
+  dim3 grid(nres + nbo);
 #define NB_LAUNCH(L_, C_, E_)                                                                          \
   do {                                                                                                 \
     if (lanes == 16)                                                                                   \
@@ -126,7 +131,7 @@ int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy, unsigned 
     else                                                                                               \
       k_nonbonded<L_, C_, E_, 32><<<grid, NB_BLOCK, 0, c->stream>>>(NB_ARGS);                          \
   } while (0)
-#define NB_ARGS a, P, ba, nbb
+#define NB_ARGS a, P, ba, nbb, nres
   if (lj && coul) {
     if (energy) NB_LAUNCH(true, true, true); else NB_LAUNCH(true, true, false);
   } else if (lj) {

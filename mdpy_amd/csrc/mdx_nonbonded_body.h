@@ -18,7 +18,9 @@
 #pragma once
 #include "mdx_internal.h"
 
+#ifndef NB_U
 #define NB_U 4  // list entries per lane per pass
+#endif
 
 struct NbArgs {
   int n, cap;

@@ -268,6 +268,10 @@ class Engine:
         L.call("mdx_nonbonded_tune", self.ctx, int(terms), int(lanes), int(reps), ctypes.byref(out))
         return out.value
 
+    def set_nonbonded_persist(self, per_cu):
+        """Nonbonded workgroups per compute unit looping over the work (0: one per unit of work)."""
+        L.call("mdx_nonbonded_persist", self.ctx, int(per_cu))
+
     def time_kernel(self, which, terms, reps=20):
         """Average ms per launch (HIP events on the library stream): which 0 nonbonded,
         1 force evaluation, 2 PME reciprocal chain, 3 integrator step."""

@@ -210,6 +210,8 @@ def test_forces_match_oracle(name, request):
 
 
 def test_forces_deterministic(water23328):
+    """Repeated evaluations are bitwise equal, and so are the resident-workgroup nonbonded
+    launches (workgroups looping over the work) and one workgroup per unit of work."""
     from mdpy_amd.engine import Engine
     say("case: determinism water23328")
     e = Engine(water23328, skin=1.5)
@@ -219,6 +221,12 @@ def test_forces_deterministic(water23328):
     F2, E2 = e.compute(127)
     assert np.array_equal(F1, F2.cpu().numpy())
     assert E1 == E2
+    Fo = e.compute_forces_only(127).cpu().numpy().copy()
+    for per_cu in (0, 1, 3):
+        e.set_nonbonded_persist(per_cu)
+        F3, E3 = e.compute(127)
+        assert np.array_equal(F1, F3.cpu().numpy()) and E1 == E3, per_cu
+        assert np.array_equal(Fo, e.compute_forces_only(127).cpu().numpy()), per_cu
     assert e.status() == 0
     e.close()
 
