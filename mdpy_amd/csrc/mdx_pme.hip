@@ -407,7 +407,7 @@ __global__ __launch_bounds__(PME_BLOCK) void k_gather(PmeArgs a, const double* _
 // contiguous ranges of sorted slots, and no atom is more than skin/2 away from its slice.
 // Phase 1 (one lane per candidate) compacts the touching atoms into LDS records; phase 2 gives
 // each record 16 lanes, one per (ky, kz) point of its 4x4 stencil in the plane.
-#define PLANE_THREADS 1024
+#define PLANE_THREADS 1024  // the 64 x 64 wave-level transforms need all 16 waves (4 lines each)
 
 struct PlaneArgs {
   int n;
