@@ -191,6 +191,7 @@ struct IntArgs {
   double L[3], inv[3];
   NoiseSrc ns;
   const double* fpart;  // per-plane reciprocal forces of the last evaluation, or null
+  double* gsave;        // G-JF: the position half's xi_{n+1} (3 per atom), reused by the velocity half
 };
 
 __device__ __forceinline__ double fix_force(const long long* ffix, const IntArgs& A, int i, int d) {
@@ -232,6 +233,7 @@ __global__ void k_gjf_a(IntArgs A, double* __restrict__ x, const double* __restr
   if (i >= A.n) return;
   double g[3];
   noise3(A.ns, st->step + 1, i, A.n, g);
+  for (int d = 0; d < 3; ++d) A.gsave[3 * i + d] = g[d];
   double sm = sqrt(im[i]);
   for (int d = 0; d < 3; ++d) {
     double acc = fix_force(ffix, A, i, d) * im[i];
@@ -418,8 +420,8 @@ __global__ __launch_bounds__(PRO_THREADS) void k_step_pro(IntArgs A, StepPro P, 
     }
     if (FUSE_B) {
       if (GJF) {
-        double g[3];
-        noise3(A.ns, st->step, i, A.n, g);
+        double g[3];  // xi_{n+1}: drawn by the previous step's position half
+        for (int d = 0; d < 3; ++d) g[d] = A.gsave[3 * i + d];
         const double sm = sqrt(im[i]);
         for (int d = 0; d < 3; ++d) {
           const double acc = f[d] * im[i];
@@ -437,6 +439,7 @@ __global__ __launch_bounds__(PRO_THREADS) void k_step_pro(IntArgs A, StepPro P, 
     if (GJF) {
       double g[3];
       noise3(A.ns, st->step + 1, i, A.n, g);
+      for (int d = 0; d < 3; ++d) A.gsave[3 * i + d] = g[d];
       const double sm = sqrt(im[i]);
       for (int d = 0; d < 3; ++d) {
         const double acc = f[d] * im[i];
@@ -467,8 +470,8 @@ __global__ void k_gjf_b(IntArgs A, double* __restrict__ v, const double* __restr
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double e = 0;
   if (i < A.n) {
-    double g[3];
-    noise3(A.ns, st->step, i, A.n, g);
+    double g[3];  // xi_{n+1}: drawn by the position half
+    for (int d = 0; d < 3; ++d) g[d] = A.gsave[3 * i + d];
     double sm = sqrt(im[i]);
     for (int d = 0; d < 3; ++d) {
       double acc = fix_force(ffix, A, i, d) * im[i];
@@ -668,7 +671,7 @@ extern "C" int mdx_ctx_destroy(mdx_ctx* c) {
   c->twiddle.release(); c->excl_pairs.release(); c->xa.release(); c->xb.release(); c->va.release();
   c->vb.release(); c->acc_prev.release(); c->acc_cur.release(); c->ehist.release(); c->tmp64.release();
   c->pos_sorted.release(); c->fpart.release(); c->pme_grec.release(); c->pme_grec_n.release();
-  c->ex_win.release(); c->s4_win.release(); c->win_bad.release(); c->ctab.release();
+  c->ex_win.release(); c->s4_win.release(); c->win_bad.release(); c->ctab.release(); c->gsave.release();
   if (c->owns_stream) hipStreamDestroy(c->stream);
   if (c->side_stream) hipStreamDestroy(c->side_stream);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
@@ -1042,6 +1045,9 @@ extern "C" int mdx_integrator_config(mdx_ctx* c, int kind, double dt, double gam
   c->gamma = gamma;
   c->kT = kT;
   c->seed = seed;
+  const size_t n3 = 3 * (size_t)c->n;  // step kernels write these from the first step on
+  int rc;
+  if ((rc = c->acc_prev.ensure(n3)) || (rc = c->acc_cur.ensure(n3)) || (rc = c->gsave.ensure(n3))) return rc;
   c->int_configured = true;
   mdx_invalidate_graph(c);
   return 0;
@@ -1086,6 +1092,7 @@ static IntArgs int_args(mdx_ctx* c) {
   A.ns.ext_base = c->ext_noise_base;
   A.ns.ext_steps = c->ext_noise_steps;
   A.fpart = live_fpart(c);
+  A.gsave = c->gsave.p;
   return A;
 }
 
@@ -1142,6 +1149,7 @@ static bool fused_steps(const mdx_ctx* c, unsigned terms) {
 // velocity half (the previous call to this function had emit_b = false); emit_b: finish this
 // step with the standalone velocity half.
 int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy, bool fuse_prev_b, bool emit_b) {
+  MDX_CHECK(c->int_configured && c->acc_prev.p && c->gsave.p, MDX_ERR_INVALID, "integrator not configured");
   IntArgs A = int_args(c);
   dim3 g(div_up(c->n, 256)), b(256);
   int rc;
@@ -1222,7 +1230,7 @@ static int integ_begin(mdx_ctx* c, unsigned terms, int nsteps) {
   MDX_CHECK(nsteps >= 0, MDX_ERR_INVALID, "nsteps must be >= 0");
   int rc = 0;
   size_t n3 = 3 * (size_t)c->n;
-  if ((rc = c->acc_prev.ensure(n3)) || (rc = c->acc_cur.ensure(n3))) return rc;
+  if ((rc = c->acc_prev.ensure(n3)) || (rc = c->acc_cur.ensure(n3)) || (rc = c->gsave.ensure(n3))) return rc;
   if (c->int_kind >= MDX_INT_VERLET_REF &&
       ((rc = c->xa.ensure(n3)) || (rc = c->xb.ensure(n3)) || (rc = c->va.ensure(n3)) || (rc = c->vb.ensure(n3))))
     return rc;

@@ -229,6 +229,7 @@ struct mdx_ctx {
   uint64_t seed = 0;
   bool int_primed = false;       // history arrays valid
   DevBuf<double> xa, xb, va, vb, acc_prev, acc_cur;   // integrator history (3N each)
+  DevBuf<double> gsave;          // G-JF: xi_{n+1} of the last position half (3N), read by the velocity half
   const double* ext_noise = nullptr;
   int ext_noise_steps = 0;
   long long ext_noise_base = 0;
