@@ -59,13 +59,13 @@ int mdx_coulomb_table_update(mdx_ctx* c) {
 
 // Workgroups [0, nres) loop over the nb_blocks nonbonded workgroup indices (nres == nb_blocks:
 // one each); the trailing ones evaluate the bonded terms.
-template <bool LJ, bool COUL, bool ENERGY, int G>
+template <bool LJ, bool COUL, bool ENERGY, int G, bool S14>
 __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, NbPtrs P, BondedArgs ba, int nb_blocks, int nres) {
   if ((int)blockIdx.x >= nres) {  // trailing workgroups: bonded terms
     bonded_block(ba, blockIdx.x - nres);
     return;
   }
-  for (int vb = blockIdx.x; vb < nb_blocks; vb += nres) nb_block<LJ, COUL, ENERGY, G, NB_BLOCK>(a, P, vb);
+  for (int vb = blockIdx.x; vb < nb_blocks; vb += nres) nb_block<LJ, COUL, ENERGY, G, NB_BLOCK, S14>(a, P, vb);
 }
 
 NbArgs nb_args(const mdx_ctx* c) {
@@ -122,14 +122,19 @@ int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy, unsigned 
   // that the concurrent PME kernels find room on every compute unit)
   const int nres = c->nb_persist > 0 ? std::min(nbb, c->nb_persist * c->ncu) : nbb;
   dim3 grid(nres + nbo);
-#define NB_LAUNCH(L_, C_, E_)                                                                          \
+#define NB_LAUNCH4(L_, C_, E_, S_)                                                                     \
   do {                                                                                                 \
     if (lanes == 16)                                                                                   \
-      k_nonbonded<L_, C_, E_, 16><<<grid, NB_BLOCK, 0, c->stream>>>(NB_ARGS);                          \
+      k_nonbonded<L_, C_, E_, 16, S_><<<grid, NB_BLOCK, 0, c->stream>>>(NB_ARGS);                      \
     else if (lanes == 64)                                                                              \
-      k_nonbonded<L_, C_, E_, 64><<<grid, NB_BLOCK, 0, c->stream>>>(NB_ARGS);                          \
+      k_nonbonded<L_, C_, E_, 64, S_><<<grid, NB_BLOCK, 0, c->stream>>>(NB_ARGS);                      \
     else                                                                                               \
-      k_nonbonded<L_, C_, E_, 32><<<grid, NB_BLOCK, 0, c->stream>>>(NB_ARGS);                          \
+      k_nonbonded<L_, C_, E_, 32, S_><<<grid, NB_BLOCK, 0, c->stream>>>(NB_ARGS);                      \
+  } while (0)
+  // the list holds 1-4 pairs only if the system has scaled pairs
+#define NB_LAUNCH(L_, C_, E_)                                                                          \
+  do {                                                                                                 \
+    if (c->has_s14) NB_LAUNCH4(L_, C_, E_, true); else NB_LAUNCH4(L_, C_, E_, false);                  \
   } while (0)
 #define NB_ARGS a, P, ba, nbb, nres
   if (lj && coul) {
@@ -140,6 +145,7 @@ int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy, unsigned 
     if (energy) NB_LAUNCH(false, true, true); else NB_LAUNCH(false, true, false);
   }
 #undef NB_LAUNCH
+#undef NB_LAUNCH4
 #undef NB_ARGS
   MDX_LAUNCH_CHECK();
   return 0;

@@ -90,7 +90,9 @@ struct NbPtrs {
   DevState* st;
 };
 
-template <bool LJ, bool COUL, bool ENERGY, int G, int BS>
+// S14: the list may hold 1-4 pairs (bit 31), which take the (sigma14, eps14) columns; without
+// any, the per-slot column choice is compiled out.
+template <bool LJ, bool COUL, bool ENERGY, int G, int BS, bool S14>
 __device__ __forceinline__ void nb_block(const NbArgs& a, const NbPtrs& P, int vb) {
   const float4* __restrict__ xq = P.xq;
   const float2* __restrict__ ljs = P.ljs;
@@ -126,7 +128,7 @@ __device__ __forceinline__ void nb_block(const NbArgs& a, const NbPtrs& P, int v
     float2 pi = make_float2(0.f, 0.f), pi14 = pi;
     if (LJ) {
       pi = ljs[s];
-      pi14 = ljs14[s];
+      if (S14) pi14 = ljs14[s];
     }
     const int cnt = nbr_count[s];
     const unsigned* lp = nbr_list + (long long)s * a.cap;
@@ -146,7 +148,7 @@ __device__ __forceinline__ void nb_block(const NbArgs& a, const NbPtrs& P, int v
       for (int u = 0; u < NB_U; ++u) {
         const int t = (int)(e[u] & 0x3fffffffu);
         xj[u] = xq[t];
-        if (LJ) pj[u] = ((e[u] >> 31) ? ljs14 : ljs)[t];
+        if (LJ) pj[u] = ((S14 && (e[u] >> 31)) ? ljs14 : ljs)[t];
       }
 #pragma unroll
       for (int u = 0; u < NB_U; ++u) {
@@ -166,7 +168,7 @@ __device__ __forceinline__ void nb_block(const NbArgs& a, const NbPtrs& P, int v
         const float r2inv = rinv * rinv;
         float fr = 0.f;
         if (LJ) {
-          const float2 pp = (e[u] >> 31) ? pi14 : pi;
+          const float2 pp = (S14 && (e[u] >> 31)) ? pi14 : pi;
           const float sig = pp.x + pj[u].x;
           const float eps = pp.y * pj[u].y;
           const float s2 = sig * sig * r2inv;

@@ -842,7 +842,7 @@ __global__ __launch_bounds__(PLANE_THREADS) void k_force_phase(int phase, int np
   }
   const int vb = b - npme;
   if (vb < nbn) {
-    nb_block<true, true, ENERGY, G, PLANE_THREADS>(na, P, nb0 + vb);
+    nb_block<true, true, ENERGY, G, PLANE_THREADS, true>(na, P, nb0 + vb);
     return;
   }
   bonded_block(ba, vb - nbn);
