@@ -147,7 +147,7 @@ int mdx_compute_forces_only(mdx_ctx* ctx, unsigned terms, double* d_force_out);
  * h_out[MDX_NTERMS + 1]. Synchronises. */
 int mdx_energies_fetch(mdx_ctx* ctx, double* h_out);
 
-/* ---- integration (Integrator.integrate, integrator/*.py) --------------------------------- */
+/* ---- integration (Integrator.integrate, the modules of mdpy/integrator) ------------------ */
 int mdx_integrator_config(mdx_ctx* ctx, int kind, double dt, double gamma, double kT, uint64_t seed);
 /* Forget integrator history (Integrator.erase_cache, integrator.py:17-22). */
 int mdx_integrator_reset(mdx_ctx* ctx);
