@@ -339,8 +339,9 @@ __device__ inline void grid_sort_fill_block(const FillArgs& f, int cell) {
               &f.pos_sorted[3 * s]);
     if (f.lj64) {
       double e = f.lj64[4 * o + 0], sg = f.lj64[4 * o + 1], e14 = f.lj64[4 * o + 2], s14 = f.lj64[4 * o + 3];
-      f.ljs[s] = make_float2((float)(0.5 * sg), (float)sqrt(e));
-      f.ljs14[s] = make_float2((float)(0.5 * s14), (float)sqrt(e14));
+      // (sigma/2, sqrt(24 eps)): a pair's 24 eps_ij is one product
+      f.ljs[s] = make_float2((float)(0.5 * sg), (float)sqrt(24.0 * e));
+      f.ljs14[s] = make_float2((float)(0.5 * s14), (float)sqrt(24.0 * e14));
     }
     if (f.ref_lin) {
       int lin = f.ref_lin[o];

@@ -11,7 +11,7 @@
 // Layout: a group of G lanes per atom (16, 32 or 64, chosen by mdx_nonbonded_tune) in
 // cell-sorted order walks the atom's contiguous
 // neighbour list (coalesced index loads), gathers each neighbour's float4 (x, y, z, q*sqrt(ke))
-// and float2 (sigma/2, sqrt(eps)), and keeps partial forces in registers. Every pair is
+// and float2 (sigma/2, sqrt(24 eps)), and keeps partial forces in registers. Every pair is
 // evaluated from both ends (full list), so there are no atomics in the pair loop; the group
 // reduces with xor shuffles and adds each atom's force once to the int64 fixed-point
 // accumulator (deterministic).
@@ -174,8 +174,8 @@ __device__ __forceinline__ void nb_block(const NbArgs& a, const NbPtrs& P, int v
           const float s2 = sig * sig * r2inv;
           const float s6 = s2 * s2 * s2;
           const float s12 = s6 * s6;
-          fr -= 24.f * eps * (2.f * s12 - s6) * r2inv;
-          if (ENERGY) elj += in ? (double)(4.f * eps * (s12 - s6)) : 0.0;
+          fr -= eps * (2.f * s12 - s6) * r2inv;  // eps holds 24 eps_ij
+          if (ENERGY) elj += in ? (double)(eps * (1.f / 6.f) * (s12 - s6)) : 0.0;
         }
         if (TAB) {
           const float zt = r2 * a.ct_scale;
