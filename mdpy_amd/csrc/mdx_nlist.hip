@@ -781,7 +781,7 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_fast(
   if (total > 0) fetch(0);
   store(0);
   __syncthreads();
-  const int lane = tid & 63, w = tid >> 6;
+  const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar registers
   constexpr int nw = BL_THREADS / 64;
   int* my_ex = exl[w];
   int* my_s4 = s4l[w];
@@ -811,7 +811,7 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_fast(
       unsigned* row = nbr_list + (long long)s * a.cap;
       int cnt = acnt[h];
       if (masked) {
-        const unsigned o32 = (unsigned)(o - 32);
+        const unsigned o32 = __builtin_amdgcn_readfirstlane((unsigned)(o - 32));
         for (int cb = 0; cb < nch; cb += BLF_PASS) {
           float4 xj[BL_U];
           int2 tj[BL_U];
@@ -824,8 +824,9 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_fast(
           for (int u = 0; u < BL_U; ++u) {
             const float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
             const unsigned dw = min((unsigned)tj[u].y - o32, 63u);
-            const bool acc = dx * dx + dy * dy + dz * dz < a.rl2 && !((exm >> dw) & 1ull);
-            const unsigned long long m = __ballot(acc);
+            const bool near = dx * dx + dy * dy + dz * dz < a.rl2, kept = !((exm >> dw) & 1ull);
+            const bool acc = near && kept;
+            const unsigned long long m = __ballot(near) & __ballot(kept);  // two compares, no re-materialised mask
             const int slot = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
             if (acc && slot < a.cap) row[slot] = (unsigned)tj[u].x | ((unsigned)((s4m >> dw) & 1ull) << 31);
