@@ -676,7 +676,8 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
 // window offset is clamped to bit 63 (always clear) instead of range-checked, and the staged
 // chunk is padded with far-away candidates up to a multiple of the pass width, so a candidate
 // test is distance, two mask bits and the ballot. Home atoms with a partner outside the window
-// scan their CSR lists. Produces the same lists, in the same order, as k_build_list_cell.
+// scan their CSR lists. A group of 64 candidates with none inside the list radius skips the mask
+// and store work. Produces the same lists, in the same order, as k_build_list_cell.
 #define BLF_PASS (64 * BL_U)
 __global__ __launch_bounds__(BL_THREADS) void k_build_list_fast(
     ListArgs a, const float4* __restrict__ xq, const int* __restrict__ sorted_orig,
@@ -823,10 +824,13 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_fast(
 #pragma unroll
           for (int u = 0; u < BL_U; ++u) {
             const float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
+            const bool near = dx * dx + dy * dy + dz * dz < a.rl2;
+            const unsigned long long mn = __ballot(near);
+            if (mn == 0) continue;  // wave-uniform: no candidate of these 64 inside the list radius
             const unsigned dw = min((unsigned)tj[u].y - o32, 63u);
-            const bool near = dx * dx + dy * dy + dz * dz < a.rl2, kept = !((exm >> dw) & 1ull);
+            const bool kept = !((exm >> dw) & 1ull);
             const bool acc = near && kept;
-            const unsigned long long m = __ballot(near) & __ballot(kept);  // two compares, no re-materialised mask
+            const unsigned long long m = mn & __ballot(kept);  // two compares, no re-materialised mask
             const int slot = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
             if (acc && slot < a.cap) row[slot] = (unsigned)tj[u].x | ((unsigned)((s4m >> dw) & 1ull) << 31);
