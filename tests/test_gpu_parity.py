@@ -118,16 +118,21 @@ def test_reference_neighbor_set_bit_exact(name, request):
     e.close()
 
 
-def test_exact_neighbor_set_chain(chain):
+@pytest.mark.parametrize("name", ["chain", "water23328"])
+def test_exact_neighbor_set(name, request):
+    """The skinned list (wave-per-atom build, far candidate groups skipped) cut at r_c in float64
+    holds exactly the oracle's pair set."""
     from mdpy_amd.engine import Engine
     from oracle import mdref as R
-    say("case: exact neighbour set chain31200")
-    e = Engine(chain, skin=1.5)
-    e.set_positions(chain.positions)
-    xw = wrapped(chain)
-    ep, ei = chain.exclusions()
-    ref = R.exact_pairs(xw, chain.box, chain.cutoff, ep, ei)
+    s = request.getfixturevalue(name)
+    say("case: exact neighbour set", name)
+    e = Engine(s, skin=1.5)
+    e.set_positions(s.positions)
+    xw = wrapped(s)
+    ep, ei = s.exclusions()
+    ref = R.exact_pairs(xw, s.box, s.cutoff, ep, ei)
     got = e.neighbor_pairs()
+    assert got.shape == ref.shape
     assert np.array_equal(got, ref)
     e.close()
 
