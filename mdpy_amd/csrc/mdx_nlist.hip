@@ -395,7 +395,9 @@ __device__ __forceinline__ int cyc_dist(int a, int b, int n) {
 // Each lane tests BL_U candidates per pass, all LDS reads issued first. FAST instantiations
 // (shifted candidates, exact pair mode, masks present) compile out the reference-mode filter and
 // the minimum image; the float64 decision of skin-0 lists is the EXACT64 instantiation.
-#define BL_CH 768
+#ifndef BL_CH
+#define BL_CH 1024  // candidates staged per chunk (a multiple of the fast kernel's pass)
+#endif
 #define BL_MAXST 512
 #define BL_MAXEX 32  // exclusions (and 1-4 partners) per atom
 #define BL_HMAX 192  // home atoms staged in LDS (more: their data is read from global memory)
