@@ -58,11 +58,21 @@ int mdx_coulomb_table_update(mdx_ctx* c) {
 }
 
 // Workgroups [0, nres) loop over the nb_blocks nonbonded workgroup indices (nres == nb_blocks:
-// one each); the trailing ones evaluate the bonded terms.
+// one each); the trailing ones evaluate the bonded terms. Blocks are numbered in cell order, so
+// a contiguous range of them is a compact region of the box.
 template <bool LJ, bool COUL, bool ENERGY, int G, bool S14>
 __global__ __launch_bounds__(NB_BLOCK) void k_nonbonded(NbArgs a, NbPtrs P, BondedArgs ba, int nb_blocks, int nres) {
   if ((int)blockIdx.x >= nres) {  // trailing workgroups: bonded terms
     bonded_block(ba, blockIdx.x - nres);
+    return;
+  }
+  if (nres % 8 == 0 && nres < nb_blocks) {
+    // workgroups are dealt to the 8 XCDs round-robin: give each XCD a contiguous (spatially
+    // compact) range of the blocks, its nres/8 workgroups striding through it
+    const int x = blockIdx.x & 7, per = nres >> 3;
+    const int b1 = (int)((long long)nb_blocks * (x + 1) / 8);
+    for (int vb = (int)((long long)nb_blocks * x / 8) + (blockIdx.x >> 3); vb < b1; vb += per)
+      nb_block<LJ, COUL, ENERGY, G, NB_BLOCK, S14>(a, P, vb);
     return;
   }
   for (int vb = blockIdx.x; vb < nb_blocks; vb += nres) nb_block<LJ, COUL, ENERGY, G, NB_BLOCK, S14>(a, P, vb);
