@@ -34,8 +34,7 @@ class Constraint:
     __hash__ = object.__hash__
 
     def bind_ensemble(self, ensemble):
-        raise NotImplementedError("The subclass of mdpy_amd.constraint.Constraint class should overload "
-                                  "bind_ensemble method")
+        raise NotImplementedError("%s must implement bind_ensemble()" % type(self).__name__)
 
     def _bind_common(self, ensemble):
         self._parent_ensemble = ensemble
@@ -43,7 +42,7 @@ class Constraint:
 
     def _check_bound_state(self):
         if self._parent_ensemble is None:
-            raise NonBoundedError("%s has not been bounded to any Ensemble instance" % self)
+            raise NonBoundedError("%s is not bound to an Ensemble yet (add it with Ensemble.add_constraints)" % self)
 
     def update(self):
         """Evaluate this constraint's terms at the ensemble's current positions (on the GPU)."""

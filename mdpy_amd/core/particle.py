@@ -35,10 +35,10 @@ class Particle:
     def add_bonded_particle(self, particle_matrix_id):
         """particle.py:53-64: duplicates and self references are errors."""
         if particle_matrix_id in self._bonded_particles:
-            raise ParticleConflictError("Particle %d has been added twice to the bonded_particles of Particle %d"
+            raise ParticleConflictError("bonded partner %d is already listed for atom %d"
                                         % (particle_matrix_id, self._matrix_id))
         if particle_matrix_id == self._matrix_id:
-            raise ParticleConflictError("Particle itself can not be added to the bonded_particle list.")
+            raise ParticleConflictError("an atom cannot be bonded to itself")
         self._bonded_particles.append(particle_matrix_id)
 
     def del_bonded_particle(self, particle_matrix_id):
@@ -48,7 +48,7 @@ class Particle:
     def add_scaling_particle(self, particle_matrix_id: int, factor):
         """particle.py:71-80: the same 1-4 partner twice is ignored (benzene-like rings)."""
         if particle_matrix_id == self._matrix_id:
-            raise ParticleConflictError("Particle itself can not be added to the scaling_particle list.")
+            raise ParticleConflictError("an atom cannot be its own 1-4 (scaled) partner")
         if particle_matrix_id not in self._scaling_particles:
             self._scaling_particles.append(particle_matrix_id)
             self._scaling_factors.append(factor)

@@ -25,33 +25,32 @@ class Topology:
         return "<mdpy_amd.core.Topology object: %d particles at %x>" % (self.num_particles, id(self))
 
     def __str__(self) -> str:
-        return "Toplogy with %d particles, %d bonds, %d angles, %d dihedrals, %d impropers" % (
+        return "Topology: %d particles, %d bonds, %d angles, %d dihedrals, %d impropers" % (
             self.num_particles, self.num_bonds, self.num_angles, self.num_dihedrals, self.num_impropers)
 
     def _check_matrix_ids(self, *ids):
         for k, i in enumerate(ids):
             if i >= self.num_particles:
-                raise ParticleConflictError("Matrix id %d beyonds the range of particles contain in toplogy, "
-                                            "can not be added as part of topology connection" % i)
+                raise ParticleConflictError("atom index %d is outside this topology's %d particles" % (i, self.num_particles))
             if i in ids[k + 1:]:
-                raise ParticleConflictError("Particle appears twice in a topology connection")
+                raise ParticleConflictError("the same atom is listed more than once in one bonded term")
 
     def _check_joined(self):
         if self._is_joined:
-            raise ModifyJoinedTopologyError("%s has been joined. No change can be made." % self)
+            raise ModifyJoinedTopologyError("the topology is already joined and can no longer be modified (%s)" % self)
 
     def add_particles(self, particles):
         self._check_joined()
         for p in particles:
             if not isinstance(p, Particle):
-                raise TypeError("mdpy_amd.core.Particle type is excepted, while %s provided" % type(p))
+                raise TypeError("expected a mdpy_amd.core.Particle, got %s" % type(p))
             p.change_matrix_id(len(self._particles))
             self._particles.append(p)
 
     def add_bond(self, bond):
         self._check_joined()
         if len(bond) != 2:
-            raise GeomtryDimError("Bond should be a matrix id list of 2 Particles, instead of %d" % len(bond))
+            raise GeomtryDimError("a bond joins 2 atoms; %d indices were given" % len(bond))
         p1, p2 = bond
         self._check_matrix_ids(p1, p2)
         self._bonds.append(list(bond))
@@ -61,7 +60,7 @@ class Topology:
     def add_angle(self, angle):
         self._check_joined()
         if len(angle) != 3:
-            raise GeomtryDimError("Angle should be a matrix id list of 3 Particles, instead of %d" % len(angle))
+            raise GeomtryDimError("an angle spans 3 atoms; %d indices were given" % len(angle))
         p1, p2, p3 = angle
         self._check_matrix_ids(p1, p2, p3)
         self._angles.append(list(angle))
@@ -71,7 +70,7 @@ class Topology:
     def add_dihedral(self, dihedral, scaling_factor=1):
         self._check_joined()
         if len(dihedral) != 4:
-            raise GeomtryDimError("Dihedral should be a matrix id list of 4 Particles, instead of %d" % len(dihedral))
+            raise GeomtryDimError("a dihedral spans 4 atoms; %d indices were given" % len(dihedral))
         p1, p2, p3, p4 = dihedral
         self._check_matrix_ids(p1, p2, p3, p4)
         self._dihedrals.append(list(dihedral))
@@ -81,7 +80,7 @@ class Topology:
     def add_improper(self, improper):
         self._check_joined()
         if len(improper) != 4:
-            raise GeomtryDimError("Improper should be a matrix id list of 4 Particles, instead of %d" % len(improper))
+            raise GeomtryDimError("an improper spans 4 atoms; %d indices were given" % len(improper))
         self._check_matrix_ids(*improper)
         self._impropers.append(list(improper))
 

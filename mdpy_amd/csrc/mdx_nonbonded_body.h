@@ -42,9 +42,10 @@ struct NbArgs {
 // towards the cutoff, where the force itself is small). Energy launches keep the erfc form.
 
 // erfc(x) = exp(-x^2) t P(t), t = 1/(1 + p x): the form of Abramowitz & Stegun 7.1.26 refit
-// (mdpy_amd/tools/fit_erfc.py) to a 7-term polynomial with max relative error 4.5e-8 on
-// [0, 3.6] (beta * rc = 3.29 for the 1e-5 Ewald tolerance), so pairs near the cutoff, where
-// erfc ~ 1e-5, keep float32 relative accuracy.
+// to a 7-term polynomial (least squares at Chebyshev nodes, reweighted towards minimax relative
+// error, Lawson's algorithm) with max relative error 5.5e-8 on [0, 3.6] with the coefficients
+// rounded to float32 (beta * rc = 3.29 for the 1e-5 Ewald tolerance), so pairs near the cutoff,
+// where erfc ~ 1e-5, keep float32 relative accuracy (tests/test_erfc_fit.py).
 #define ERFC_P 0.355f
 #define ERFC_C1 1.9758294902e-01f
 #define ERFC_C2 2.3220149337e-01f

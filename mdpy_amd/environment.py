@@ -24,8 +24,7 @@ class MDPYEnvironment:
         precision = precision.upper()
         if precision not in self._supported_precisions:
             raise EnvironmentVariableError(
-                "Precision %s is not supported. Check supported precision with "
-                "`mdpy_amd.env.supported_precisions`" % precision)
+                "unknown precision %r; the choices are listed in mdpy_amd.env.supported_precisions" % precision)
         self._precision = precision
         if precision == "SINGLE":
             self.NUMPY_FLOAT, self.NUMPY_INT = np.float32, np.int32
@@ -36,8 +35,7 @@ class MDPYEnvironment:
         platform = platform.upper()
         if platform not in self._supported_platforms:
             raise EnvironmentVariableError(
-                "Platform %s is not supported. Check supported platform with "
-                "`mdpy_amd.env.supported_platforms`" % platform)
+                "unknown platform %r; the choices are listed in mdpy_amd.env.supported_platforms" % platform)
         self._platform = platform
 
     def set_default(self):

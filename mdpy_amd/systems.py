@@ -252,7 +252,7 @@ def _chain_molecule(rng, n_chain, L):
     every tenth quad (moved forward to the next quad whose angle is within 90 deg of 0) carries a
     harmonic improper k U(5, 20) near its generated angle. The improper energy k (psi - psi0)^2 is
     not periodic (mdpy/constraint/charmm_improper_constraint.py:70,90), so psi0 is kept away from
-    +-180 deg, where a crossing would jump the energy. Charges U(-0.5, 0.5) e shifted to a
+    +-180 deg, where a crossing would jump the energy. Charges U(-0.6, 0.6) e shifted to a
     neutral chain; LJ eps U(0.05, 0.2) kcal/mol, Rmin/2 U(1.7, 2.1) A, 1-4 eps halved.
     """
     pos = np.zeros((n_chain, 3))
@@ -320,7 +320,7 @@ def _chain_molecule(rng, n_chain, L):
     impropers = quads[imp_rows].reshape(-1, 4)
     improper_params = np.stack([rng.uniform(5, 20, len(imp_rows)) * KCAL_PER_MOL,
                                 psi_all[imp_rows] + rng.uniform(-0.1, 0.1, len(imp_rows))], 1).reshape(-1, 2)
-    q = rng.uniform(-0.5, 0.5, n_chain)
+    q = rng.uniform(-0.6, 0.6, n_chain)
     q -= q.mean()
     eps = rng.uniform(0.05, 0.2, n_chain) * KCAL_PER_MOL
     sig = 2.0 * rng.uniform(1.7, 2.1, n_chain) * RMIN_TO_SIGMA_FACTOR

@@ -407,6 +407,45 @@ def test_langevin_reference_free_particle_closed_form():
     assert xs[0, 0] == pytest.approx(cx) and vs[0, 0] == pytest.approx(cv)
 
 
+def _verlet_ref_run(x0, v0, acc, dt, calls):
+    """VerletReference on one atom of unit mass under a constant acceleration, in a box too large
+    for the wrap to act; `calls` = steps per integrate call (the cache carries x_{n-1} across)."""
+    integ = R.VerletReference(dt)
+    pbc = np.diag([1e9] * 3)
+    inv = np.linalg.inv(pbc)
+    x, v = x0.copy(), v0.copy()
+    for n in calls:
+        x, v = integ.integrate(x, v, [1.0], lambda y: np.array([acc]), pbc, inv, n)
+    return x, v
+
+
+def test_verlet_reference_free_particle_ends_at_half_velocity():
+    # verlet_integrator.py:28-50 with F = 0: x_{-1} = x0 - v0 dt, then x_n = x0 + n v0 dt and
+    # the output velocity mi(x_n - x_{n-1}) / (2 dt) is v0 / 2 (the reference's central
+    # difference over one step divided by 2 dt)
+    x0, v0, dt = np.array([[1.0, -2.0, 0.5]]), np.array([[0.3, -0.1, 0.02]]), 0.5
+    x, v = _verlet_ref_run(x0, v0, [0.0, 0.0, 0.0], dt, [7])
+    assert np.allclose(x, x0 + 7 * v0 * dt, rtol=0, atol=1e-12)
+    assert np.allclose(v, v0 / 2, rtol=0, atol=1e-12)
+
+
+def test_verlet_reference_constant_force_closed_form():
+    # x_{-1} = x0 - v0 dt + a dt^2 (start-up with +a dt^2, verlet_integrator.py:31-34) and
+    # x_{n+1} = 2 x_n - x_{n-1} + a dt^2 solve to x_n = x0 + (v0 dt - a dt^2/2) n + a dt^2 n^2 / 2,
+    # so the output velocity (x_n - x_{n-1}) / (2 dt) is (v0 + a dt (n - 1)) / 2
+    a = np.array([0.2, -0.05, 0.01])
+    x0, v0, dt = np.array([[0.0, 1.0, -1.0]]), np.array([[0.1, 0.0, -0.3]]), 0.25
+    for n in (1, 2, 9):
+        x, v = _verlet_ref_run(x0, v0, a, dt, [n])
+        xn = x0 + (v0 * dt - a * dt ** 2 / 2) * n + a * dt ** 2 * n ** 2 / 2
+        assert np.allclose(x, xn, rtol=0, atol=1e-12)
+        assert np.allclose(v, (v0 + a * dt * (n - 1)) / 2, rtol=0, atol=1e-12)
+    # the cache (integrator.py:17-22) makes 3 + 6 steps in two calls the same trajectory as 9 in one
+    x2, v2 = _verlet_ref_run(x0, v0, a, dt, [3, 6])
+    x1, v1 = _verlet_ref_run(x0, v0, a, dt, [9])
+    assert np.allclose(x2, x1, rtol=0, atol=1e-12) and np.allclose(v2, v1, rtol=0, atol=1e-12)
+
+
 def test_gjf_langevin_temperature():
     # harmonic wells, 300 K: <m v^2> -> kT per component (G-JF gives exact configurational T)
     from mdpy_amd.unit import KB
