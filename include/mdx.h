@@ -104,6 +104,13 @@ int mdx_wrap_positions(mdx_ctx* ctx);
  * Writes d_cell_index (N*3 int32) and returns grid dims and the largest cell population in
  * h_n[3], h_maxpc. Synchronises. */
 int mdx_celllist_reference(mdx_ctx* ctx, double cell_cutoff, int* d_cell_index, int* h_n, int* h_maxpc);
+/* The caller wrote the bound position array directly (not through this library): the next
+ * mdx_integrate evaluates forces at the new positions before its first step. Without it, G-JF and
+ * velocity-Verlet calls start from the forces and energies of the previous call's last step (the
+ * integrator's cache, as integrator.py:17-22 keeps positions between calls); every other entry
+ * point of this library that changes positions, parameters or the force buffers drops that cache
+ * by itself. No counterpart in the reference, whose integrate() always re-evaluates. */
+int mdx_positions_changed(mdx_ctx* ctx);
 /* Dense cell table [nx][ny][nz][maxpc], atoms ascending within a cell, -1 padded (cell_list.py:105-116),
  * for the binning of the last mdx_celllist_reference call. */
 int mdx_celllist_table(mdx_ctx* ctx, int* d_table, int maxpc);
@@ -154,9 +161,12 @@ int mdx_integrator_reset(mdx_ctx* ctx);
 /* Use caller-provided standard normals d_noise[nsteps][N][3] (f64) instead of the built-in
  * Philox4x32-10 stream, step by step from the next step on (NULL switches back). */
 int mdx_set_external_noise(mdx_ctx* ctx, const double* d_noise, int nsteps);
-/* Advance `nsteps` steps with force terms `terms`. use_graph != 0 captures the step into a
- * hipGraph and replays it. Positions/velocities in the bound arrays are updated (positions
- * wrapped as State.set_positions does); per-step energies go to the energy history. */
+/* Advance `nsteps` steps with force terms `terms`. use_graph != 0 replays the steps after the
+ * first from hipGraphs of fixed sizes (forces-only runs of 32/16/4/1 steps and a single energy
+ * step), captured once per (terms, integrator kind) and reused by calls of any length.
+ * Positions/velocities in the bound arrays are updated (positions wrapped as State.set_positions
+ * does); per-step energies go to the energy history. The call's last step always evaluates
+ * energies. */
 int mdx_integrate(mdx_ctx* ctx, unsigned terms, int nsteps, int use_graph);
 /* Evaluate potential energies every `every` steps during mdx_integrate (default 1; forces are
  * evaluated every step). The energy history then holds rows for steps 0, every, 2*every, ... */

@@ -90,6 +90,13 @@ __global__ void k_zero_acc(long long* __restrict__ ffix, long long n3, long long
   }
 }
 
+// start of an integrate call: its energy-history rows count from the current step, and the
+// kinetic-energy slot is recomputed from the current velocities
+__global__ void k_begin_call(DevState* st, long long* __restrict__ ke_slot) {
+  st->hist_base = st->step;
+  *ke_slot = 0;
+}
+
 __global__ void k_record(const long long* __restrict__ efix, int ne, double* __restrict__ ehist, int ehist_cap,
                          DevState* st, int every) {
   long long k = st->step - st->hist_base;
@@ -294,8 +301,7 @@ __device__ void prologue_atom(const IntArgs& A, const StepPro& P, DevState* st, 
     for (int d = 0; d < 3; ++d) {
       double x = xn[d];
       x -= A.L[d] * floor(x / A.L[d]);
-      const int k = (int)(x / P.gw[d]);
-      c[d] = k < 0 ? 0 : (k >= P.gn[d] ? P.gn[d] - 1 : k);
+      c[d] = clamped_floor_index(x / P.gw[d], P.gn[d]);
     }
     P.atom_cell[i] = (c[0] * P.gn[1] + c[1]) * P.gn[2] + c[2];
   }
@@ -381,7 +387,7 @@ __device__ void prologue_block_end(const StepPro& P, DevState* st, double d2, bo
   if (threadIdx.x == 0) __threadfence();  // acquire for the block (device-scope fences act on whole caches)
   __syncthreads();
   const bool rebuild = atomicAdd(&st->rebuild, 0) != 0 ||
-                       dec_double_order(atomicAdd(&st->max_disp2_bits, 0ull)) > P.thr2;
+                       !(dec_double_order(atomicAdd(&st->max_disp2_bits, 0ull)) <= P.thr2);
   if (rebuild) {
     if (P.bin) last_block_bin(P, st);
     else
@@ -626,11 +632,19 @@ extern "C" int mdx_ctx_create(int device, void* hip_stream, mdx_ctx** out) {
   }
   ExtSync s;
   s.ext = (hipStream_t)hip_stream;
-  hipEventCreateWithFlags(&s.in, hipEventDisableTiming);
-  hipEventCreateWithFlags(&s.out, hipEventDisableTiming);
+  if (hipEventCreateWithFlags(&s.in, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s.out, hipEventDisableTiming) != hipSuccess) {
+    if (s.in) hipEventDestroy(s.in);
+    mdx_ctx_destroy(c);
+    mdx_set_error("event creation failed");
+    return MDX_ERR_HIP;
+  }
   g_sync.push_back({c, s});
   int rc;
-  if ((rc = c->dstate.ensure(1)) || (rc = c->efix.ensure(MDX_NTERMS + 1))) return rc;
+  if ((rc = c->dstate.ensure(1)) || (rc = c->efix.ensure(MDX_NTERMS + 1))) {
+    mdx_ctx_destroy(c);
+    return rc;
+  }
   MDX_HIP(hipMemset(c->dstate.p, 0, sizeof(DevState)));
   MDX_HIP(hipMemset(c->efix.p, 0, sizeof(long long) * (MDX_NTERMS + 1)));
   int one = 1;
@@ -649,8 +663,7 @@ extern "C" int mdx_ctx_set_stream(mdx_ctx* c, void* hip_stream) {
 extern "C" int mdx_ctx_destroy(mdx_ctx* c) {
   if (!c) return 0;
   hipSetDevice(c->device);
-  hipStreamSynchronize(c->stream);
-  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+  mdx_invalidate_graph(c);  // synchronises the context's streams first
   for (size_t k = 0; k < g_sync.size(); ++k)
     if (g_sync[k].first == c) {
       hipEventDestroy(g_sync[k].second.in);
@@ -682,9 +695,28 @@ extern "C" int mdx_ctx_destroy(mdx_ctx* c) {
 
 void mdx_invalidate_graph(mdx_ctx* c) {
   c->graph_gen++;
-  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
-  c->graph_exec = nullptr;
+  c->forces_current = false;
+  bool any = c->graph_e != nullptr;
+  for (hipGraphExec_t g : c->graph_f) any |= g != nullptr;
+  if (any) {
+    // a replay of these graphs may still be in flight on the context's streams (launches return
+    // before the work ends): let it finish before the executable graphs are destroyed
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side_stream) (void)hipStreamSynchronize(c->side_stream);
+    for (hipGraphExec_t& g : c->graph_f) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+    if (c->graph_e) (void)hipGraphExecDestroy(c->graph_e);
+    c->graph_e = nullptr;
+  }
   c->graph_kind = -1;
+}
+
+extern "C" int mdx_positions_changed(mdx_ctx* c) {
+  MDX_CHECK(c, MDX_ERR_INVALID, "null context");
+  c->forces_current = false;
+  return 0;
 }
 
 extern "C" int mdx_system_set(mdx_ctx* c, int n, double* d_pos, double* d_vel, const double* h_box) {
@@ -982,6 +1014,7 @@ int mdx_enqueue_forces(mdx_ctx* c, unsigned terms, bool energy) {
 static int compute_forces(mdx_ctx* c, unsigned terms, double* d_force_out, bool energy) {
   MDX_CHECK(c && c->n > 0, MDX_ERR_INVALID, "mdx_compute_forces: no system");
   ENTER();
+  c->forces_current = false;
   int rc = mdx_enqueue_zero_accumulators(c);
   if (!rc) rc = mdx_enqueue_forces(c, terms, energy);
   if (!rc && energy && c->vel && c->has_mass) {
@@ -1224,7 +1257,9 @@ int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy, bool fu
 }
 
 // ---- integrate = prologue (prime forces, energy-history base) + steps + epilogue
-static int integ_begin(mdx_ctx* c, unsigned terms, int nsteps) {
+// buffers of an integrate call (before any graph is captured: growing the energy history
+// invalidates captured steps, which hold its pointer)
+static int integ_alloc(mdx_ctx* c, int nsteps) {
   MDX_CHECK(c && c->n > 0 && c->vel, MDX_ERR_INVALID, "mdx_integrate: system with velocities required");
   MDX_CHECK(c->int_configured && c->has_mass, MDX_ERR_INVALID, "integrator or masses not configured");
   MDX_CHECK(nsteps >= 0, MDX_ERR_INVALID, "nsteps must be >= 0");
@@ -1237,20 +1272,29 @@ static int integ_begin(mdx_ctx* c, unsigned terms, int nsteps) {
   const int ne = MDX_NTERMS + 1;
   const int nrows = nsteps / c->energy_every + 1;
   if (nrows > c->ehist_cap) {
-    if ((rc = c->ehist.ensure((size_t)nrows * ne))) return rc;
-    c->ehist_cap = nrows;
+    const int cap = std::max(nrows, 4096);
+    if ((rc = c->ehist.ensure((size_t)cap * ne))) return rc;
+    c->ehist_cap = cap;
     mdx_invalidate_graph(c);  // captured steps hold the old history pointer
   }
+  return 0;
+}
+
+static int integ_begin(mdx_ctx* c, unsigned terms, int nsteps) {
+  int rc = integ_alloc(c, nsteps);
+  if (rc) return rc;
   dim3 g(div_up(c->n, 256)), b(256);
-  // energy-history rows are (device step counter - hist_base); set the base for this call
-  long long s0 = 0;
-  MDX_HIP(hipMemcpyAsync(&s0, &c->dstate.p->step, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
-  MDX_HIP(hipStreamSynchronize(c->stream));
-  MDX_HIP(hipMemcpyAsync(&c->dstate.p->hist_base, &s0, sizeof(long long), hipMemcpyHostToDevice, c->stream));
-  MDX_HIP(hipStreamSynchronize(c->stream));
+  // energy-history rows are (device step counter - hist_base); the base of this call is set on
+  // the device (no host round trip)
+  k_begin_call<<<1, 1, 0, c->stream>>>(c->dstate.p, c->efix.p + MDX_E_KINETIC);
+  MDX_LAUNCH_CHECK();
   c->ehist_count = 0;
-  // prime: forces at the current positions (every mdpy integrate call starts with ensemble.update)
-  if ((rc = mdx_enqueue_zero_accumulators(c)) || (rc = mdx_enqueue_forces(c, terms, true))) return rc;
+  // prime: forces at the current positions (every mdpy integrate call starts with ensemble.update,
+  // integrator.py:26-28). The step kernels of G-JF and velocity Verlet keep the last step's forces
+  // and potential energies (the last step of a call always evaluates energies); when nothing has
+  // changed since, they are the forces at the current positions and the prime is skipped.
+  const bool reuse = c->forces_current && c->forces_terms == terms && fused_steps(c, terms);
+  if (!reuse && ((rc = mdx_enqueue_zero_accumulators(c)) || (rc = mdx_enqueue_forces(c, terms, true)))) return rc;
   IntArgs A = int_args(c);
   k_kinetic<<<g, b, 0, c->stream>>>(c->n, c->vel, c->mass.p, c->efix.p + MDX_E_KINETIC);
   if (c->int_kind == MDX_INT_VERLET_REF) {
@@ -1282,24 +1326,28 @@ static int integ_end(mdx_ctx* c, int nsteps) {
   }
   MDX_LAUNCH_CHECK();
   c->evals += nsteps + 1;
-  // positions were wrapped: the neighbour list must not trust its displacement reference
-  return mdx_enqueue_request_rebuild(c);
+  // The wrap moves atoms by whole box edges: the list's displacement test is a minimum-image
+  // distance and every kernel re-wraps positions, so the list stays valid. The reference-
+  // compatible integrators wrap every step and keep their own history; they always prime.
+  if (c->int_kind >= MDX_INT_VERLET_REF) return mdx_enqueue_request_rebuild(c);
+  return 0;
 }
 
-// steps per graph: a whole number of energy intervals, at least 10 steps when possible
-static int graph_steps_for(const mdx_ctx* c, int nsteps) {
-  const int every = c->energy_every;
-  return std::min(nsteps, every >= 10 ? every : every * ((10 + every - 1) / every));
+// Potential energies are evaluated on the call's steps whose index + 1 is a multiple of the
+// energy interval (the prime counts as step 0 of the history) and always on the call's last step,
+// so the next call can start from its forces and energies.
+static bool energy_step(const mdx_ctx* c, int local, int nsteps) {
+  return (local + 1) % c->energy_every == 0 || local == nsteps - 1;
 }
 
 // steps local0 .. local0+count-1 of the current call. With the step prologue every step but
 // the call's first folds the previous step's velocity half into its prologue (fuse_first for the
 // first of these), and the call ends with one standalone velocity half (integ_final_b), so graph
 // replays and direct launches run the same kernels in the same order.
-static int integ_steps(mdx_ctx* c, unsigned terms, int local0, int count, bool fuse_first) {
+static int integ_steps(mdx_ctx* c, unsigned terms, int local0, int count, bool fuse_first, int nsteps) {
   int rc = 0;
   for (int k = 0; k < count && !rc; ++k)
-    rc = mdx_integrator_enqueue_step(c, terms, (local0 + k + 1) % c->energy_every == 0, k > 0 || fuse_first, false);
+    rc = mdx_integrator_enqueue_step(c, terms, energy_step(c, local0 + k, nsteps), k > 0 || fuse_first, false);
   return rc;
 }
 
@@ -1316,55 +1364,95 @@ static int integ_final_b(mdx_ctx* c, unsigned terms, int nsteps) {
   return 0;
 }
 
-// the context's own graph of `gsteps` steps following a call's first step (captured on its
-// stream; cached)
-static int ensure_graph(mdx_ctx* c, unsigned terms, int gsteps) {
-  const int every = c->energy_every;
-  if (c->graph_exec && c->graph_terms == terms && c->graph_steps == gsteps && c->graph_kind == c->int_kind &&
-      c->graph_every == every)
-    return 0;
-  mdx_invalidate_graph(c);
+// Graphs of kGraphSizes[k] forces-only steps and one of a single energy step, each following a
+// step of the same call (its prologue folds the previous velocity half). A call's steps after
+// its first are a sequence of forces-only runs separated by energy steps; each run is covered
+// by the largest graphs that fit, so the same executable graphs serve every call length and are
+// captured once per (terms, integrator kind).
+static const int kGraphSizes[4] = {32, 16, 4, 1};
+
+static int capture_steps(mdx_ctx* c, unsigned terms, int count, bool energy, hipGraphExec_t* out) {
   hipGraph_t graph;
   MDX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  int rc = integ_steps(c, terms, 1, gsteps, true);
+  int rc = 0;
+  for (int k = 0; k < count && !rc; ++k) rc = mdx_integrator_enqueue_step(c, terms, energy, true, false);
   hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
   if (rc) return rc;
   if (e2 != hipSuccess) { mdx_set_error("graph capture failed: %s", hipGetErrorString(e2)); return MDX_ERR_HIP; }
-  hipError_t e3 = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
+  hipError_t e3 = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
   hipGraphDestroy(graph);
-  if (e3 != hipSuccess) { mdx_set_error("graph instantiate failed: %s", hipGetErrorString(e3)); return MDX_ERR_HIP; }
-  c->graph_terms = terms;
-  c->graph_steps = gsteps;
-  c->graph_every = every;
-  c->graph_kind = c->int_kind;
+  if (e3 != hipSuccess) { *out = nullptr; mdx_set_error("graph instantiate failed: %s", hipGetErrorString(e3)); return MDX_ERR_HIP; }
   return 0;
 }
+
+static int ensure_graphs(mdx_ctx* c, unsigned terms) {
+  if (c->graph_e && c->graph_terms == terms && c->graph_kind == c->int_kind) return 0;
+  const bool cur = c->forces_current;  // capturing enqueues nothing that runs
+  const unsigned cur_terms = c->forces_terms;
+  mdx_invalidate_graph(c);
+  int rc = 0;
+  for (int k = 0; k < 4 && !rc; ++k) rc = capture_steps(c, terms, kGraphSizes[k], false, &c->graph_f[k]);
+  if (!rc) rc = capture_steps(c, terms, 1, true, &c->graph_e);
+  if (rc) {
+    mdx_invalidate_graph(c);
+    return rc;
+  }
+  c->graph_terms = terms;
+  c->graph_kind = c->int_kind;
+  c->graph_lanes = c->nb_lanes;
+  c->forces_current = cur;
+  c->forces_terms = cur_terms;
+  return 0;
+}
+
+// the launches covering steps 1 .. nsteps-1 of a call (graph index, 4 = the energy step)
+static void graph_schedule(const mdx_ctx* c, int nsteps, std::vector<int>& seq) {
+  seq.clear();
+  int s = 1;
+  while (s < nsteps) {
+    if (energy_step(c, s, nsteps)) {
+      seq.push_back(4);
+      ++s;
+      continue;
+    }
+    int run = 0;
+    while (s + run < nsteps && !energy_step(c, s + run, nsteps)) ++run;
+    s += run;
+    for (int k = 0; k < 4; ++k)
+      for (; run >= kGraphSizes[k]; run -= kGraphSizes[k]) seq.push_back(k);
+  }
+}
+
+static hipGraphExec_t graph_of(const mdx_ctx* c, int k) { return k == 4 ? c->graph_e : c->graph_f[k]; }
 
 extern "C" int mdx_integrate(mdx_ctx* c, unsigned terms, int nsteps, int use_graph) {
   MDX_CHECK(c, MDX_ERR_INVALID, "null context");
   ENTER();
-  int rc = integ_begin(c, terms, nsteps);
+  int rc = integ_alloc(c, nsteps);
+  if (!rc && use_graph && nsteps > 1) rc = ensure_graphs(c, terms);
+  if (!rc) rc = integ_begin(c, terms, nsteps);
   if (rc) LEAVE_RET(rc);
-  // potential energies are evaluated on steps that are multiples of `every` (the prime counts
-  // as step 0); step 0 of the call is launched directly, then the graph (a whole number of
-  // energy intervals) covers the following steps, then the remainder and the final velocity half
+  // step 0 of the call is launched directly (its prologue has no previous velocity half), the
+  // following steps are graph replays, then the final velocity half
   if (use_graph && nsteps > 1) {
-    const int rest = nsteps - 1;
-    const int gsteps = graph_steps_for(c, rest);
-    if ((rc = ensure_graph(c, terms, gsteps))) LEAVE_RET(rc);
-    if ((rc = integ_steps(c, terms, 0, 1, false))) LEAVE_RET(rc);
-    const int full = rest / gsteps, rem = rest % gsteps;
-    for (int k = 0; k < full; ++k) {
-      hipError_t e = hipGraphLaunch(c->graph_exec, c->stream);
+    if ((rc = integ_steps(c, terms, 0, 1, false, nsteps))) LEAVE_RET(rc);
+    std::vector<int> seq;
+    graph_schedule(c, nsteps, seq);
+    for (int k : seq) {
+      hipError_t e = hipGraphLaunch(graph_of(c, k), c->stream);
       if (e != hipSuccess) { mdx_set_error("hipGraphLaunch: %s", hipGetErrorString(e)); LEAVE_RET(MDX_ERR_HIP); }
     }
-    rc = integ_steps(c, terms, 1 + full * gsteps, rem, true);
   } else {
-    rc = integ_steps(c, terms, 0, nsteps, false);
+    rc = integ_steps(c, terms, 0, nsteps, false, nsteps);
   }
   if (!rc) rc = integ_final_b(c, terms, nsteps);
   if (rc) LEAVE_RET(rc);
-  LEAVE_RET(integ_end(c, nsteps));
+  rc = integ_end(c, nsteps);
+  if (!rc && fused_steps(c, terms)) {
+    c->forces_current = true;
+    c->forces_terms = terms;
+  }
+  LEAVE_RET(rc);
 }
 
 // ------------------------------------------------------------------ batched replicas
@@ -1451,41 +1539,43 @@ extern "C" int mdx_batch_integrate(mdx_batch* b, unsigned terms, int nsteps, int
   MDX_HIP(hipStreamWaitEvent(b->lead, b->ev_in, 0));
   for (mdx_ctx* c : b->ctx) MDX_HIP(hipStreamWaitEvent(c->stream, b->ev_in, 0));
   int rc = 0;
+  // same schedule as mdx_integrate: step 0 directly, fixed-size graphs for the following steps,
+  // final velocity half
+  for (mdx_ctx* c : b->ctx) {
+    if ((rc = integ_alloc(c, nsteps))) return rc;
+    if (nsteps > 1 && (rc = ensure_graphs(c, terms))) return rc;
+  }
   for (mdx_ctx* c : b->ctx)
     if ((rc = integ_begin(c, terms, nsteps))) return rc;
-  // same schedule as mdx_integrate: step 0 directly, graphs for the following steps, remainder,
-  // final velocity half
-  const int rest = nsteps > 0 ? nsteps - 1 : 0;
-  const int gsteps = graph_steps_for(b->ctx[0], rest);
-  const int full = gsteps > 0 ? rest / gsteps : 0, rem = gsteps > 0 ? rest % gsteps : 0;
-  if (full > 0)
-    for (mdx_ctx* c : b->ctx)
-      if ((rc = ensure_graph(c, terms, gsteps))) return rc;
   if (nsteps > 0)
     for (mdx_ctx* c : b->ctx)
-      if ((rc = integ_steps(c, terms, 0, 1, false))) return rc;
-  if (full > 0) {
-    if (mode == MDX_BATCH_STREAMS) {  // interleaved launches: the replicas' graphs run side by side
-      for (int k = 0; k < full; ++k)
-        for (mdx_ctx* c : b->ctx) {
-          hipError_t e = hipGraphLaunch(c->graph_exec, c->stream);
-          if (e != hipSuccess) { mdx_set_error("hipGraphLaunch: %s", hipGetErrorString(e)); return MDX_ERR_HIP; }
-        }
-    } else {  // sequential: one replica after another, on one stream order
-      for (size_t i = 0; i < b->ctx.size(); ++i) {
-        mdx_ctx* c = b->ctx[i];
-        if (i > 0 && (rc = stream_after(c->stream, b->ctx[i - 1]->stream, b->ev_out[i - 1]))) return rc;
-        for (int k = 0; k < full; ++k) {
-          hipError_t e = hipGraphLaunch(c->graph_exec, c->stream);
-          if (e != hipSuccess) { mdx_set_error("hipGraphLaunch: %s", hipGetErrorString(e)); return MDX_ERR_HIP; }
-        }
-      }
+      if ((rc = integ_steps(c, terms, 0, 1, false, nsteps))) return rc;
+  std::vector<int> seq;
+  graph_schedule(b->ctx[0], nsteps, seq);
+  auto launch = [](mdx_ctx* c, int k) -> int {
+    hipError_t e = hipGraphLaunch(graph_of(c, k), c->stream);
+    if (e != hipSuccess) { mdx_set_error("hipGraphLaunch: %s", hipGetErrorString(e)); return MDX_ERR_HIP; }
+    return 0;
+  };
+  if (mode == MDX_BATCH_STREAMS) {  // interleaved launches: the replicas' graphs run side by side
+    for (int k : seq)
+      for (mdx_ctx* c : b->ctx)
+        if ((rc = launch(c, k))) return rc;
+  } else {  // sequential: one replica after another, on one stream order
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+      mdx_ctx* c = b->ctx[i];
+      if (i > 0 && (rc = stream_after(c->stream, b->ctx[i - 1]->stream, b->ev_out[i - 1]))) return rc;
+      for (int k : seq)
+        if ((rc = launch(c, k))) return rc;
     }
   }
   for (mdx_ctx* c : b->ctx) {
-    if (nsteps > 0 && (rc = integ_steps(c, terms, 1 + full * gsteps, rem, true))) return rc;
     if ((rc = integ_final_b(c, terms, nsteps))) return rc;
     if ((rc = integ_end(c, nsteps))) return rc;
+    if (fused_steps(c, terms)) {
+      c->forces_current = true;
+      c->forces_terms = terms;
+    }
   }
   // leave: the caller's stream waits for every replica
   for (size_t i = 0; i < b->ctx.size(); ++i) {
@@ -1574,8 +1664,10 @@ extern "C" int mdx_nonbonded_tune(mdx_ctx* c, unsigned terms, int lanes, int rep
       }
     }
   }
+  // the timing runs ended with the last candidate: captured steps hold the launch shape
+  // they were captured with, so they are dropped only when that differs from the choice
+  if (best != c->graph_lanes) mdx_invalidate_graph(c);
   c->nb_lanes = best;
-  mdx_invalidate_graph(c);
   if (h_lanes) *h_lanes = best;
   return 0;
 }
@@ -1590,6 +1682,7 @@ extern "C" int mdx_nonbonded_persist(mdx_ctx* c, int per_cu) {
 extern "C" int mdx_time_kernel(mdx_ctx* c, int which, unsigned terms, int reps, double* h_avg_ms) {
   MDX_CHECK(c && h_avg_ms && reps > 0, MDX_ERR_INVALID, "mdx_time_kernel: bad arguments");
   MDX_ENTER();
+  c->forces_current = false;  // the timed launches accumulate into the force buffers
   int rc = 0;
   bool lj = (terms & MDX_TERM_LJ) != 0, cd = (terms & MDX_TERM_COUL_DIRECT) != 0;
   if (which == 0) {  // make sure the list is current

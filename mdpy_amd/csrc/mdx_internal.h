@@ -105,6 +105,15 @@ struct DevState {
   unsigned bar_gen;         // and its generation (advanced by the last arrival)
 };
 
+// floor(u) clamped to [0, n) for a cell or grid coordinate u >= 0. The range is decided in
+// floating point before the conversion: a float-to-int conversion of NaN, or of a value outside
+// int's range, has no defined result (LLVM treats it as poison, which a later integer clamp does
+// not repair), so a NaN or runaway coordinate must land in a valid cell, never at an arbitrary
+// index. NaN and negative u give 0, u >= n gives n - 1.
+__device__ __forceinline__ int clamped_floor_index(double u, int n) {
+  return u >= 1.0 ? (u < (double)n ? (int)u : n - 1) : 0;
+}
+
 // float32 sorted-order copy of one atom: wrapped, centred position and q*sqrt(ke) (the layout
 // the pair kernels read), plus its float64 position
 __device__ __forceinline__ void pack_atom(double x, double y, double z, float q, double Lx, double Ly, double Lz,
@@ -237,14 +246,19 @@ struct mdx_ctx {
   int ehist_cap = 0;
   int ehist_count = 0;
   long long evals = 0;
-  // graph
-  hipGraphExec_t graph_exec = nullptr;
+  // step graphs: a few fixed sizes of forces-only steps plus one energy step, captured once per
+  // (terms, integrator kind) and replayed whatever the call's length (mdx_api.hip kGraphSizes)
+  hipGraphExec_t graph_f[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipGraphExec_t graph_e = nullptr;
   unsigned graph_terms = 0;
-  int graph_steps = 0;
   int graph_kind = -1;
-  int graph_every = 0;
+  int graph_lanes = 0;           // nonbonded lanes per atom the graphs were captured with
   int energy_every = 1;          // potential energies every this many steps during integrate
   unsigned long long graph_gen = 0;  // bumped whenever captured graphs of this context go stale
+  // the forces in the accumulators (and the potential energies in efix) belong to the current
+  // positions: the last step of the last mdx_integrate call, with nothing changed since
+  bool forces_current = false;
+  unsigned forces_terms = 0;
   // scratch
   DevBuf<long long> tmp64;
   // learned surrogate (optional)
@@ -274,8 +288,9 @@ static inline double dec_double_order_host(unsigned long long e) {
 
 // ---- list rebuild, per-cell sort and fill (shared by the fill kernel and the fill + bonded launch)
 // decision shared by every rebuild kernel: rebuild requested, or an atom moved > skin/2
+// (a NaN displacement, from non-finite positions, also rebuilds: written as !(d <= thr2))
 __device__ __forceinline__ bool need_rebuild(const DevState* st, double thr2) {
-  return st->rebuild != 0 || dec_double_order(st->max_disp2_bits) > thr2;
+  return st->rebuild != 0 || !(dec_double_order(st->max_disp2_bits) <= thr2);
 }
 
 struct FillArgs {
@@ -303,13 +318,39 @@ struct FillArgs {
 #ifndef SORT_MAX
 #define SORT_MAX 1024
 #endif
+// slot s of the sorted order holds atom o: its position at the build, packed copies and LJ
+__device__ inline void fill_slot(const FillArgs& f, int s, int o) {
+  f.orig_sorted[o] = s;
+  for (int d = 0; d < 3; ++d) f.pos_build[3 * o + d] = f.pos[3 * o + d];
+  pack_atom(f.pos[3 * o], f.pos[3 * o + 1], f.pos[3 * o + 2], f.qs ? f.qs[o] : 0.f, f.L[0], f.L[1], f.L[2], &f.xq[s],
+            &f.pos_sorted[3 * s]);
+  if (f.lj64) {
+    double e = f.lj64[4 * o + 0], sg = f.lj64[4 * o + 1], e14 = f.lj64[4 * o + 2], s14 = f.lj64[4 * o + 3];
+    // (sigma/2, sqrt(24 eps)): a pair's 24 eps_ij is one product
+    f.ljs[s] = make_float2((float)(0.5 * sg), (float)sqrt(24.0 * e));
+    f.ljs14[s] = make_float2((float)(0.5 * s14), (float)sqrt(24.0 * e14));
+  }
+  if (f.ref_lin) {
+    int lin = f.ref_lin[o];
+    int cz = lin % f.rnz, cy = (lin / f.rnz) % f.rny, cx = lin / (f.rnz * f.rny);
+    f.ref_cell_sorted[s] = (cx << 20) | (cy << 10) | cz;
+  }
+}
+
 // per cell: sort members ascending (deterministic order), then fill the per-slot arrays
 __device__ inline void grid_sort_fill_block(const FillArgs& f, int cell) {
   __shared__ int buf[SORT_MAX];
   int* sorted_orig = f.sorted_orig;
   int start = f.cell_start[cell], cnt = f.cell_count[cell];
-  if (cnt > SORT_MAX) return;  // flagged in k_cell_scan
+  // A cell beyond the sort's capacity (flagged as an overflow where the cells were counted) keeps
+  // the scatter's order: every one of its slots is still filled, so the sorted order stays a
+  // permutation and no slot is left stale for the list build or the PME kernels.
+  const bool sortable = cnt <= SORT_MAX;
   int P = 1;
+  if (!sortable) {
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) fill_slot(f, start + t, sorted_orig[start + t]);
+    return;
+  }
   while (P < cnt) P <<= 1;
   for (int t = threadIdx.x; t < P; t += blockDim.x) buf[t] = t < cnt ? sorted_orig[start + t] : 0x7fffffff;
   __syncthreads();
@@ -333,25 +374,9 @@ __device__ inline void grid_sort_fill_block(const FillArgs& f, int cell) {
     int o = buf[t];
     int s = start + t;
     sorted_orig[s] = o;
-    f.orig_sorted[o] = s;
-    for (int d = 0; d < 3; ++d) f.pos_build[3 * o + d] = f.pos[3 * o + d];
-    pack_atom(f.pos[3 * o], f.pos[3 * o + 1], f.pos[3 * o + 2], f.qs ? f.qs[o] : 0.f, f.L[0], f.L[1], f.L[2], &f.xq[s],
-              &f.pos_sorted[3 * s]);
-    if (f.lj64) {
-      double e = f.lj64[4 * o + 0], sg = f.lj64[4 * o + 1], e14 = f.lj64[4 * o + 2], s14 = f.lj64[4 * o + 3];
-      // (sigma/2, sqrt(24 eps)): a pair's 24 eps_ij is one product
-      f.ljs[s] = make_float2((float)(0.5 * sg), (float)sqrt(24.0 * e));
-      f.ljs14[s] = make_float2((float)(0.5 * s14), (float)sqrt(24.0 * e14));
-    }
-    if (f.ref_lin) {
-      int lin = f.ref_lin[o];
-      int cz = lin % f.rnz, cy = (lin / f.rnz) % f.rny, cx = lin / (f.rnz * f.rny);
-      f.ref_cell_sorted[s] = (cx << 20) | (cy << 10) | cz;
-    }
+    fill_slot(f, s, o);
   }
 }
-
-
 
 // lanes cooperating on one atom's neighbour list (nonbonded kernel and list build)
 #define NB_G 32

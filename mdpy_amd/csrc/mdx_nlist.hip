@@ -71,11 +71,10 @@ __global__ void k_ref_bin(const double* __restrict__ pos, int n, int nx, int ny,
   int c[3];
   for (int d = 0; d < 3; ++d) {
     double s = (pos[3 * i + d] - mn[d]) * inv[d];  // (x - min x) . cell_inv, diagonal
-    int k = (int)floor(s);
-    if (k < 0 || k >= nn[d]) {
-      atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
-      k = k < 0 ? 0 : nn[d] - 1;
-    }
+    const double f = floor(s);
+    // range decided before the conversion (NaN or runaway positions flag and clamp)
+    if (!(f >= 0.0 && f < (double)nn[d])) atomicOr(&st->flags, MDX_FLAG_OVERFLOW);
+    const int k = clamped_floor_index(f, nn[d]);
     c[d] = k;
     if (cell_index) cell_index[3 * i + d] = k;
   }
@@ -142,6 +141,7 @@ __global__ void k_ref_reset(DevState* st, int* __restrict__ counts, int nc) {
 extern "C" int mdx_wrap_positions(mdx_ctx* c) {
   MDX_CHECK(c && c->pos, MDX_ERR_INVALID, "mdx_wrap_positions: no system bound");
   MDX_ENTER();
+  c->forces_current = false;
   double inv[3];
   for (int d = 0; d < 3; ++d) inv[d] = 1.0 / c->box[d];
   MDX_HIP(hipMemsetAsync(&c->dstate.p->flags, 0, sizeof(unsigned), c->stream));
@@ -270,8 +270,7 @@ __global__ void k_grid_bin(const double* __restrict__ pos, int n, GridArgs g, co
   for (int d = 0; d < 3; ++d) {
     double x = pos[3 * i + d];
     x -= L[d] * floor(x / L[d]);
-    int k = (int)(x / w[d]);
-    c[d] = k < 0 ? 0 : (k >= nn[d] ? nn[d] - 1 : k);
+    c[d] = clamped_floor_index(x / w[d], nn[d]);
   }
   int lin = (c[0] * g.ny + c[1]) * g.nz + c[2];
   atom_cell[i] = lin;
@@ -434,7 +433,11 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
   const GridArgs& gr = a.g;
   const int cell = blockIdx.x;
   const int hb = cell_start[cell], hn = cell_count[cell];
-  if (hn == 0 || hn > SORT_MAX) return;  // oversize cells are flagged by k_cell_scan
+  if (hn > SORT_MAX) {  // oversize cells are flagged where the cells are counted: empty lists
+    for (int t = threadIdx.x; t < hn; t += blockDim.x) nbr_count[hb + t] = 0;
+    return;
+  }
+  if (hn == 0) return;
   const int tid = threadIdx.x;
   const int hz = cell % gr.nz, hy = (cell / gr.nz) % gr.ny, hx = cell / (gr.nz * gr.ny);
   const bool shifted = FAST || a.shifted;
@@ -703,7 +706,11 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_fast(
   const GridArgs& gr = a.g;
   const int cell = blockIdx.x;
   const int hb = cell_start[cell], hn = cell_count[cell];
-  if (hn == 0 || hn > SORT_MAX) return;  // oversize cells are flagged by k_cell_scan
+  if (hn > SORT_MAX) {  // oversize cells are flagged where the cells are counted: empty lists
+    for (int t = threadIdx.x; t < hn; t += blockDim.x) nbr_count[hb + t] = 0;
+    return;
+  }
+  if (hn == 0) return;
   const int tid = threadIdx.x;
   const int hz = cell % gr.nz, hy = (cell / gr.nz) % gr.ny, hx = cell / (gr.nz * gr.ny);
   for (int k = tid; k < a.nstencil; k += blockDim.x) {
@@ -1127,6 +1134,7 @@ __global__ void k_emit_pairs(int n, int cap, const int* __restrict__ nbr_count, 
 extern "C" int mdx_neighbor_pairs(mdx_ctx* c, int* d_pairs, int64_t capacity, int64_t* h_count) {
   MDX_CHECK(c && c->pos && c->nb_configured, MDX_ERR_INVALID, "mdx_neighbor_pairs: system/nonbonded not set");
   MDX_ENTER();
+  c->forces_current = false;
   MDX_HIP(hipMemsetAsync(&c->dstate.p->flags, 0, sizeof(unsigned), c->stream));
   int rc = mdx_nlist_update(c, true);
   if (rc) return rc;

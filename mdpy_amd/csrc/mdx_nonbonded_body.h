@@ -182,7 +182,8 @@ __device__ __forceinline__ void nb_block(const NbArgs& a, const NbPtrs& P, int v
           const float zt = r2 * a.ct_scale;
           const float fl = floorf(zt);
           const float t = zt - fl;
-          const float4 cf = ctab[min((int)fl, MDX_CTAB_N - 1)];
+          // clamp in float before the conversion (a NaN distance reads the last piece)
+          const float4 cf = ctab[(int)fminf(fl, (float)(MDX_CTAB_N - 1))];
           const float gz = fmaf(t, fmaf(t, fmaf(t, cf.w, cf.z), cf.y), cf.x);
           fr -= xi.w * xj[u].w * (rinv * r2inv - gz);
         } else if (COUL) {

@@ -319,7 +319,8 @@ __device__ __forceinline__ void frac_coord(double x, double iL, int K, int* base
   s -= floor(s);
   double u = s * K;
   double f = floor(u);
-  int b = (int)f;
+  // NaN or infinite x: base 0 (range decided before the conversion); u == K after rounding wraps
+  int b = f >= 0.0 && f <= (double)K ? (int)f : 0;
   *w = (float)(u - f);
   if (b >= K) b -= K;
   *base = b;

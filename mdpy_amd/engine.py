@@ -189,6 +189,7 @@ class Engine:
             self.pos.copy_(x.to(device=self.device, dtype=self.torch.float64))
         else:
             self.pos.copy_(self.torch.as_tensor(np.asarray(x, dtype=np.float64), device=self.device))
+        L.call("mdx_positions_changed", self.ctx)
         if wrap:
             L.call("mdx_wrap_positions", self.ctx)
 
