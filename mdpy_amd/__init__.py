@@ -14,7 +14,7 @@ def __getattr__(name):
     # lazy: the device-facing modules import torch only when used
     import importlib
     lazy = {"core": ".core", "constraint": ".constraint", "integrator": ".integrator",
-            "engine": ".engine"}
+            "engine": ".engine", "io": ".io", "forcefield": ".forcefield"}
     if name in lazy:
         return importlib.import_module(lazy[name], __name__)
     if name == "Ensemble":

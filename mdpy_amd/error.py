@@ -59,3 +59,27 @@ class DeviceError(RuntimeError):
 
 class NumericalInstabilityError(RuntimeError):
     """Non-finite positions or forces were detected on the device."""
+
+
+class FileFormatError(Exception):
+    """mdpy/error.py:96 — a file whose name or content is not of the expected format."""
+
+
+class TrajectoryPoorDefinedError(Exception):
+    """mdpy/error.py:133 — inconsistent trajectory frames."""
+
+
+class ParameterPoorDefinedError(Exception):
+    """mdpy/error.py:142 — a force-field parameter the topology needs is missing."""
+
+
+class HDF5FilePoorDefinedError(Exception):
+    """mdpy/error.py:151 — an HDF5 file that does not follow the mdpy layout."""
+
+
+class ParserPoorDefinedError(Exception):
+    """mdpy/error.py:160 — a parser asked for data it was told not to read."""
+
+
+class AnalyserPoorDefinedError(Exception):
+    """mdpy/error.py:193 — an analyser given inconsistent selections or options."""

@@ -1,0 +1,6 @@
+"""File readers and writers (mdpy/io/__init__.py), without MDAnalysis or h5py."""
+from .charmm_toppar_parser import CharmmTopparParser
+from .psf_parser import PSFParser
+from .pdb_parser import PDBParser
+
+__all__ = ["CharmmTopparParser", "PSFParser", "PDBParser"]
