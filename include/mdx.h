@@ -174,10 +174,40 @@ int mdx_set_energy_interval(mdx_ctx* ctx, int every);
 /* Energies of the last mdx_integrate call, one row per energy step (steps 0, every, ...):
  * h_out[k*(MDX_NTERMS+1) + t] for k < min(rows, h_cap); *h_count = rows. Synchronises. */
 int mdx_energy_history_fetch(mdx_ctx* ctx, double* h_out, int h_cap, int* h_count);
+/* As mdx_energy_history_fetch without synchronising: the copy of min(rows, h_cap) rows into h_out
+ * (pinned host memory for an asynchronous copy) is enqueued on the context's stream, ordered
+ * after the integrate call; *h_count = the rows copied. The rows are valid in h_out once the
+ * caller's stream has reached this point (an event recorded after the call). */
+int mdx_energy_history_copy_async(mdx_ctx* ctx, double* h_out, int h_cap, int* h_count);
 /* Status flags raised on the device since the last call (bit 0 overflow, 1 particle loss,
  * 2 non-finite, 3 PME coverage: an atom missed by the reciprocal-space gather, 4 the fused
  * PME kernel's grid barrier gave up waiting for its blocks); clears them. Synchronises. */
 int mdx_status_fetch(mdx_ctx* ctx, unsigned* h_flags);
+
+/* ---- energy minimisation (mdpy/minimizer) ----------------------------------------------- */
+/* SteepestDescentMinimizer.minimize (steepest_descent_minimizer.py:30-53): x_i <- x_i +
+ * alpha F_i/|F_i| (wrapped) until |(E - E_prev)/E_prev| < energy_tol or max_iter iterations; every
+ * evaluation is the fused evaluation of `terms` (energies included). Returns the iterations done
+ * and the final potential energy. Synchronises once per iteration (energy read-back). An atom
+ * with no force does not move (the reference divides by its zero norm). */
+int mdx_minimize_sd(mdx_ctx* ctx, unsigned terms, double alpha, double energy_tol, int max_iter, int* h_iters,
+                    double* h_energy);
+/* ConjugateGradientMinimizer.minimize (conjugate_gradient_minimizer.py:33-77) as written: up to
+ * max_sub_iter finite-difference probes of step theta along the current forces per iteration
+ * (each from the previous probe's positions), the last probe's step alpha t applied to the
+ * iteration's starting positions; the probe loop ends early when |F + alpha omega|^2/|F0|^2 <
+ * force_tol. Sums run on the device in a fixed order (deterministic). */
+int mdx_minimize_cg(mdx_ctx* ctx, unsigned terms, double theta, double force_tol, int max_sub_iter, double energy_tol,
+                    int max_iter, int* h_iters, double* h_energy);
+
+/* ---- analysis (mdpy/analyser/rdf_analyser.py:34-85, coordination_number_analyser.py:34-80) ---- */
+/* For each atom i = d_sel1[a] and every j in d_sel2, the minimum-image distance in float64 of
+ * frames d_pos[n_frames][n_atoms][3] in a box of edges h_box, binned on the nbins + 1 edges
+ * d_edges (numpy.linspace(0, rc, nbins + 1)) exactly as numpy.histogram bins it; d_counts
+ * (n1 x nbins int64, zeroed by the caller) accumulates over frames. Enqueued on hip_stream. */
+int mdx_pair_histogram(void* hip_stream, int n_frames, int n_atoms, const double* d_pos, const double* h_box,
+                       const int* d_sel1, int n1, const int* d_sel2, int n2, const double* d_edges, int nbins,
+                       int64_t* d_counts);
 
 /* ---- batched replicas (new; no reference counterpart) ---------------------------------- */
 /* Many independent systems, one context each (same device), advanced together: each context

@@ -14,13 +14,14 @@ def __getattr__(name):
     # lazy: the device-facing modules import torch only when used
     import importlib
     lazy = {"core": ".core", "constraint": ".constraint", "integrator": ".integrator",
-            "engine": ".engine", "io": ".io", "forcefield": ".forcefield"}
+            "engine": ".engine", "io": ".io", "forcefield": ".forcefield",
+            "minimizer": ".minimizer", "dumper": ".dumper", "analyser": ".analyser", "utils": ".utils"}
     if name in lazy:
         return importlib.import_module(lazy[name], __name__)
     if name == "Ensemble":
         from .ensemble import Ensemble
         return Ensemble
-    if name in ("Simulation", "LogDumper", "Dumper"):
+    if name in ("Simulation", "LogDumper", "Dumper", "PDBDumper"):
         from . import simulation
         return getattr(simulation, name)
     raise AttributeError(name)

@@ -74,6 +74,13 @@ SIGNATURES = {
     "mdx_set_impropers": (ctypes.c_int, [_vp, ctypes.c_int, _c_int_p, _c_dbl_p]),
     "mdx_wrap_positions": (ctypes.c_int, [_vp]),
     "mdx_positions_changed": (ctypes.c_int, [_vp]),
+    "mdx_energy_history_copy_async": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _c_int_p]),
+    "mdx_minimize_sd": (ctypes.c_int, [_vp, ctypes.c_uint, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                       _c_int_p, _c_dbl_p]),
+    "mdx_minimize_cg": (ctypes.c_int, [_vp, ctypes.c_uint, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_int, _c_int_p, _c_dbl_p]),
+    "mdx_pair_histogram": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _c_dbl_p, _vp, ctypes.c_int, _vp,
+                                          ctypes.c_int, _vp, ctypes.c_int, _vp]),
     "mdx_celllist_reference": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _c_int_p, _c_int_p]),
     "mdx_celllist_table": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
     "mdx_nonbonded_config": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double]),

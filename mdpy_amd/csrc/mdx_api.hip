@@ -1585,6 +1585,212 @@ extern "C" int mdx_batch_integrate(mdx_batch* b, unsigned terms, int nsteps, int
   return 0;
 }
 
+// ------------------------------------------------------------------ energy minimisation
+// SteepestDescentMinimizer and ConjugateGradientMinimizer (mdpy/minimizer/*.py) on the device
+// force path: the position updates and the reductions run on the GPU, every force evaluation
+// is the fused one of mdx_compute_forces, and per iteration only a few scalars (the potential
+// energy, the conjugate-gradient sums) come back to decide when to stop.
+#define MIN_BLOCK 256
+
+// x_i <- wrap(x_i + alpha F_i / |F_i|) (steepest_descent_minimizer.py:37-40). An atom with no
+// force at all stays where it is (the reference divides by its zero norm).
+__global__ void k_sd_move(IntArgs A, double* __restrict__ pos, const long long* __restrict__ ffix, double alpha,
+                          DevState* st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  double f[3], x[3];
+  for (int d = 0; d < 3; ++d) f[d] = fix_force(ffix, A, i, d);
+  const double nrm = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+  const double s = nrm > 0.0 ? alpha / nrm : 0.0;
+  for (int d = 0; d < 3; ++d) x[d] = pos[3 * i + d] + s * f[d];
+  wrap_into(A, x, pos + 3 * i, st);
+}
+
+// conjugate-gradient probe (conjugate_gradient_minimizer.py:46-53): t = cur_f = F(x), kept in
+// cf; x <- wrap(x + theta t); x0 (optional) keeps the starting positions of the outer iteration
+__global__ void k_cg_probe(IntArgs A, double* __restrict__ pos, const long long* __restrict__ ffix, double theta,
+                           double* __restrict__ cf, double* __restrict__ x0, DevState* st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  double x[3];
+  for (int d = 0; d < 3; ++d) {
+    const double f = fix_force(ffix, A, i, d);
+    cf[3 * i + d] = f;
+    if (x0) x0[3 * i + d] = pos[3 * i + d];
+    x[d] = pos[3 * i + d] + theta * f;
+  }
+  wrap_into(A, x, pos + 3 * i, st);
+}
+
+// per-block partial sums (fixed order inside the block) of sum cf^2, sum omega.cf, sum omega^2
+// with omega = -(F(x + theta t) - cf)/theta (:54); with cf == NULL, sum F^2 only
+__global__ void k_cg_partials(IntArgs A, const long long* __restrict__ ffix, const double* __restrict__ cf, double theta,
+                              double* __restrict__ part) {
+  __shared__ double sm[3][MIN_BLOCK];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double s1 = 0, s2 = 0, s3 = 0;
+  if (i < A.n)
+    for (int d = 0; d < 3; ++d) {
+      const double f = fix_force(ffix, A, i, d);
+      if (cf) {
+        const double c = cf[3 * i + d], w = -(f - c) / theta;
+        s1 += c * c;
+        s2 += w * c;
+        s3 += w * w;
+      } else {
+        s1 += f * f;
+      }
+    }
+  sm[0][threadIdx.x] = s1;
+  sm[1][threadIdx.x] = s2;
+  sm[2][threadIdx.x] = s3;
+  __syncthreads();
+  for (int h = MIN_BLOCK / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int k = 0; k < 3; ++k) sm[k][threadIdx.x] += sm[k][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+// sum of the block partials in block order (deterministic) into out[0..2]
+__global__ void k_cg_total(const double* __restrict__ part, int nblocks, double* __restrict__ out) {
+  if (threadIdx.x >= 3) return;
+  double s = 0;
+  for (int b = 0; b < nblocks; ++b) s += part[3 * b + threadIdx.x];
+  out[threadIdx.x] = s;
+}
+
+// x <- wrap(x0 + alpha cf) (the last sub-iteration's d = alpha t, :66-69)
+__global__ void k_cg_place(IntArgs A, double* __restrict__ pos, const double* __restrict__ x0,
+                           const double* __restrict__ cf, double alpha, DevState* st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  double x[3];
+  for (int d = 0; d < 3; ++d) x[d] = x0[3 * i + d] + alpha * cf[3 * i + d];
+  wrap_into(A, x, pos + 3 * i, st);
+}
+
+// one fused evaluation with energies; returns the potential energy (sum of the term slots) and
+// fails on a particle-loss or non-finite flag
+static int min_evaluate(mdx_ctx* c, unsigned terms, double* h_epot) {
+  int rc = mdx_enqueue_zero_accumulators(c);
+  if (!rc) rc = mdx_enqueue_forces(c, terms, true);
+  if (rc) return rc;
+  c->evals++;
+  long long e[MDX_NTERMS + 1];
+  unsigned flags = 0;
+  MDX_HIP(hipMemcpyAsync(e, c->efix.p, sizeof(e), hipMemcpyDeviceToHost, c->stream));
+  MDX_HIP(hipMemcpyAsync(&flags, &c->dstate.p->flags, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+  MDX_HIP(hipStreamSynchronize(c->stream));
+  MDX_CHECK(!(flags & MDX_FLAG_LOSS), MDX_ERR_PARTICLE_LOSS, "minimisation moved atom(s) beyond 2 PBC images");
+  double s = 0;
+  for (int t = 0; t < MDX_NTERMS; ++t) s += (double)e[t] * kFixInv;
+  MDX_CHECK(std::isfinite(s), MDX_ERR_NONFINITE, "minimisation reached a non-finite potential energy");
+  *h_epot = s;
+  return 0;
+}
+
+static int min_prepare(mdx_ctx* c) {
+  MDX_CHECK(c && c->n > 0 && c->pos, MDX_ERR_INVALID, "minimisation needs a bound system");
+  c->forces_current = false;
+  int rc;
+  if ((rc = c->xa.ensure(3 * (size_t)c->n)) || (rc = c->xb.ensure(3 * (size_t)c->n))) return rc;
+  if ((rc = c->tmp64.ensure(3 * (size_t)div_up(c->n, MIN_BLOCK) + 4))) return rc;
+  c->int_primed = false;  // the reference-compatible integrators keep history in xa / xb
+  return 0;
+}
+
+extern "C" int mdx_minimize_sd(mdx_ctx* c, unsigned terms, double alpha, double energy_tol, int max_iter,
+                               int* h_iters, double* h_energy) {
+  MDX_CHECK(c && alpha > 0 && max_iter >= 0, MDX_ERR_INVALID, "mdx_minimize_sd: bad arguments");
+  ENTER();
+  int rc = min_prepare(c);
+  double cur = 0, pre = 0;
+  int it = 0;
+  if (!rc) rc = min_evaluate(c, terms, &cur);
+  pre = cur;
+  const dim3 g(div_up(c->n, MIN_BLOCK)), b(MIN_BLOCK);
+  while (!rc && it < max_iter) {
+    IntArgs A = int_args(c);
+    k_sd_move<<<g, b, 0, c->stream>>>(A, c->pos, c->ffix.p, alpha, c->dstate.p);
+    if (hipGetLastError() != hipSuccess) { rc = MDX_ERR_HIP; break; }
+    if ((rc = min_evaluate(c, terms, &cur))) break;
+    ++it;
+    const double err = std::fabs((cur - pre) / pre);
+    if (err < energy_tol) break;
+    pre = cur;
+  }
+  if (h_iters) *h_iters = it;
+  if (h_energy) *h_energy = cur;
+  LEAVE_RET(rc);
+}
+
+extern "C" int mdx_minimize_cg(mdx_ctx* c, unsigned terms, double theta, double force_tol, int max_sub_iter,
+                               double energy_tol, int max_iter, int* h_iters, double* h_energy) {
+  MDX_CHECK(c && theta > 0 && max_sub_iter >= 1 && max_iter >= 0, MDX_ERR_INVALID, "mdx_minimize_cg: bad arguments");
+  ENTER();
+  int rc = min_prepare(c);
+  double cur = 0;
+  int it = 0;
+  if (!rc) rc = min_evaluate(c, terms, &cur);
+  const int nbk = div_up(c->n, MIN_BLOCK);
+  const dim3 g(nbk), b(MIN_BLOCK);
+  double* part = (double*)c->tmp64.p;
+  double* tot = part + 3 * (size_t)nbk;
+  double* x0 = c->xa.p;
+  double* cf = c->xb.p;
+  auto sums = [&](const double* cfp, double* h) -> int {
+    IntArgs A = int_args(c);
+    k_cg_partials<<<g, b, 0, c->stream>>>(A, c->ffix.p, cfp, theta, part);
+    k_cg_total<<<1, 64, 0, c->stream>>>(part, nbk, tot);
+    MDX_LAUNCH_CHECK();
+    MDX_HIP(hipMemcpyAsync(h, tot, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    MDX_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+  };
+  while (!rc && it < max_iter) {
+    const double pre = cur;
+    double s0[3];
+    if ((rc = sums(nullptr, s0))) break;  // sum f0^2
+    double alpha = 0;
+    for (int sub = 0; sub < max_sub_iter && !rc; ++sub) {
+      IntArgs A = int_args(c);
+      k_cg_probe<<<g, b, 0, c->stream>>>(A, c->pos, c->ffix.p, theta, cf, sub == 0 ? x0 : nullptr, c->dstate.p);
+      if (hipGetLastError() != hipSuccess) { rc = MDX_ERR_HIP; break; }
+      double e_probe = 0;
+      if ((rc = min_evaluate(c, terms, &e_probe))) break;
+      double s[3];
+      if ((rc = sums(cf, s))) break;
+      alpha = s[0] / s[1];  // (cur_f . cur_f) / (omega . t) with t = cur_f
+      // |cur_f + alpha omega|^2 / |f0|^2 < force_tol ends the sub-iterations (:57-60)
+      const double fn2 = s[0] + 2 * alpha * s[1] + alpha * alpha * s[2];
+      if (fn2 / s0[0] < force_tol) break;
+    }
+    if (rc) break;
+    IntArgs A = int_args(c);
+    k_cg_place<<<g, b, 0, c->stream>>>(A, c->pos, x0, cf, alpha, c->dstate.p);
+    if (hipGetLastError() != hipSuccess) { rc = MDX_ERR_HIP; break; }
+    if ((rc = min_evaluate(c, terms, &cur))) break;
+    ++it;
+    if (std::fabs((cur - pre) / pre) < energy_tol) break;
+  }
+  if (h_iters) *h_iters = it;
+  if (h_energy) *h_energy = cur;
+  LEAVE_RET(rc);
+}
+
+extern "C" int mdx_energy_history_copy_async(mdx_ctx* c, double* h_out, int h_cap, int* h_count) {
+  MDX_CHECK(c && h_out && h_count, MDX_ERR_INVALID, "mdx_energy_history_copy_async: bad arguments");
+  ENTER();
+  const int cnt = std::min(h_cap, c->ehist_count);
+  const int ne = MDX_NTERMS + 1;
+  if (cnt > 0)
+    MDX_HIP(hipMemcpyAsync(h_out, c->ehist.p, (size_t)cnt * ne * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  *h_count = cnt;
+  LEAVE_RET(0);
+}
+
 extern "C" int mdx_energy_history_fetch(mdx_ctx* c, double* h_out, int h_cap, int* h_count) {
   MDX_CHECK(c && h_out && h_count, MDX_ERR_INVALID, "mdx_energy_history_fetch: bad arguments");
   int cnt = std::min(h_cap, c->ehist_count);

@@ -313,6 +313,15 @@ class Engine:
         L.call("mdx_set_energy_interval", self.ctx, int(energy_every))
         L.call("mdx_integrate", self.ctx, int(terms), int(nsteps), 1 if use_graph else 0)
 
+    def energy_history_async(self, out):
+        """Enqueue a copy of the last integrate call's energy-history rows into the pinned host
+        tensor `out` ((rows, NTERMS + 1) float64) and return the row count without waiting; the copy
+        is ordered on PyTorch's current stream after the steps."""
+        cnt = ctypes.c_int()
+        L.call("mdx_energy_history_copy_async", self.ctx, ctypes.c_void_p(out.data_ptr()), int(out.shape[0]),
+               ctypes.byref(cnt))
+        return cnt.value
+
     def energy_history(self, cap=None):
         cap = cap or 1 << 20
         out = np.zeros((cap, L.NTERMS + 1))

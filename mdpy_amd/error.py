@@ -83,3 +83,7 @@ class ParserPoorDefinedError(Exception):
 
 class AnalyserPoorDefinedError(Exception):
     """mdpy/error.py:193 — an analyser given inconsistent selections or options."""
+
+
+class SelectionConditionPoorDefinedError(Exception):
+    """mdpy/error.py:184 — a selection condition with an unknown keyword or shape."""

@@ -35,11 +35,23 @@ class Integrator:
         raise NotImplementedError
 
     def integrate(self, ensemble, num_steps: int = 1):
+        self._advance(ensemble, num_steps)
+        self._finish(ensemble)
+
+    def _advance(self, ensemble, num_steps, energy_every=None):
+        """Enqueue num_steps steps on the device and return without waiting for them; potential
+        energies every energy_every steps (default: every step)."""
         engine = ensemble.state.engine
         if self._cached_engine is not engine:
             engine.reset_integrator()
         self._configure(engine)
-        engine.integrate(ensemble.term_bits, int(num_steps), use_graph=self.use_graph)
+        every = int(energy_every) if energy_every else 1
+        engine.integrate(ensemble.term_bits, int(num_steps), use_graph=self.use_graph, energy_every=every)
+        self._cached_engine = engine
+
+    def _finish(self, ensemble):
+        """Wait for the device, raise on flags, refresh the ensemble's forces and energies."""
+        engine = ensemble.state.engine
         status = engine.status()
         if status & 2:
             from ..error import ParticleLossError

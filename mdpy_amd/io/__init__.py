@@ -2,5 +2,6 @@
 from .charmm_toppar_parser import CharmmTopparParser
 from .psf_parser import PSFParser
 from .pdb_parser import PDBParser
+from .pdb_writer import PDBWriter
 
-__all__ = ["CharmmTopparParser", "PSFParser", "PDBParser"]
+__all__ = ["CharmmTopparParser", "PSFParser", "PDBParser", "PDBWriter"]

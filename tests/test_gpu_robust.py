@@ -55,8 +55,9 @@ def test_all_atoms_nonfinite_oversize_cell():
     e = Engine(s, skin=1.5)
     e.set_positions(s.positions)
     F0 = e.compute(terms)[0].cpu().numpy().copy()
-    print("non-finite case: water23328, every atom NaN", flush=True)
-    e.set_positions(np.full_like(s.positions, np.nan), wrap=False)
+    print("non-finite case: water23328, every atom NaN, list rebuilt", flush=True)
+    # through the wrap: it requests a list rebuild, so the NaN atoms are binned (all into one cell)
+    e.set_positions(np.full_like(s.positions, np.nan), wrap=True)
     e.compute(terms)
     torch.cuda.synchronize()
     f = e.status()

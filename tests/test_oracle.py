@@ -469,3 +469,65 @@ def test_golden_water648_consistency():
     assert d["F_lj"].sum(0) == pytest.approx(np.zeros(3), abs=1e-10)
     # SPME does not conserve momentum exactly; its net force is a small fraction of the total
     assert np.abs(d["F_pme"].sum(0)).max() < 1e-4 * np.abs(d["F_pme"]).sum()
+
+
+# ----------------------------------------------------------------------------- minimisers
+def _harmonic(k, centre):
+    def fe(x):
+        d = x - centre
+        return -k * d, 0.5 * k * float((d * d).sum()) + 1.0  # +1: the relative-change test divides by E
+    return fe
+
+
+def test_steepest_descent_closed_form():
+    # steepest_descent_minimizer.py:37-40: every atom steps alpha along its force direction, so an
+    # atom 0.35 from the minimum of an isotropic well is 0.35 - k alpha away after k steps
+    from oracle import minimize as M
+    centre = np.array([[1.0, 2.0, 3.0]])
+    x0 = centre + np.array([[0.35, 0.0, 0.0]])
+    box = 1e6
+    wrap = lambda y: R.wrap_positions(y, np.diag([box] * 3), np.diag([1 / box] * 3))
+    x, it, en = M.steepest_descent(x0, _harmonic(2.0, centre), wrap, 0.1, 1e-12, 3)
+    assert it == 3 and x[0, 0] == pytest.approx(1.05, abs=1e-12)
+    assert en[-1] == pytest.approx(0.5 * 2.0 * 0.05 ** 2 + 1.0, abs=1e-12)
+
+
+def test_conjugate_gradient_as_written_closed_form():
+    # conjugate_gradient_minimizer.py:46-69 on E = k/2 |x - c|^2. Each sub-iteration probes from the
+    # already-probed positions: t_s = F(x_s) = k (1 - k theta)^s (c - x0); the probe gives
+    # omega = k t_s, so alpha = 1/k; the update cur_f + alpha omega = 2 t_s never meets force_tol,
+    # and only the last sub-iteration's d = t_s / k is applied:
+    # x = x0 + (1 - k theta)^(S - 1) (c - x0) after S sub-iterations
+    from oracle import minimize as M
+    rng = np.random.default_rng(1)
+    k, theta, S = 3.0, 1e-3, 10
+    centre = rng.uniform(-1, 1, (5, 3))
+    x0 = centre + rng.uniform(-0.3, 0.3, (5, 3))
+    box = 1e6
+    wrap = lambda y: R.wrap_positions(y, np.diag([box] * 3), np.diag([1 / box] * 3))
+    x, it, en = M.conjugate_gradient(x0, _harmonic(k, centre), wrap, theta, 0.01, S, 1e-9, 1)
+    assert it == 1
+    assert np.allclose(x, x0 + (1 - k * theta) ** (S - 1) * (centre - x0), rtol=0, atol=1e-9)
+
+
+# ----------------------------------------------------------------------------- analysers
+def test_coordination_number_simple_cubic_closed_form():
+    # a simple cubic lattice of spacing a: 6 neighbours at a, 12 at a sqrt 2, 8 at a sqrt 3,
+    # 6 at 2a; 45 bins over 2.1a put every shell at least a tenth of a bin from an edge
+    from oracle import analyse as A
+    a, m = 3.0, 6
+    g = np.arange(m) * a
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(1, -1, 3)
+    box = np.array([m * a] * 3)
+    ids = list(range(x.shape[1]))
+    rc, nb = 2.1 * a, 45
+    counts, edges = A.pair_counts(x, box, ids[:20], ids, rc, nb)
+    w = rc / nb
+    expect = {0: 1, int(a / w): 6, int(a * 2 ** 0.5 / w): 12, int(a * 3 ** 0.5 / w): 8, int(2 * a / w): 6}
+    for i in range(20):
+        assert {k: c for k, c in enumerate(counts[i]) if c} == expect
+    mean, std, _ = A.coordination_number(x, box, ids, ids, rc, nb)
+    assert mean[-1] == 1 + 6 + 12 + 8 + 6 and np.all(std == 0)
+    # rdf of an ideal lattice: zero between shells
+    rm, _, _ = A.rdf(x, box, ids, ids, rc, nb)
+    assert rm[10] == 0 and rm[int(a / w)] > 0
