@@ -111,6 +111,16 @@ int mdx_celllist_reference(mdx_ctx* ctx, double cell_cutoff, int* d_cell_index, 
  * point of this library that changes positions, parameters or the force buffers drops that cache
  * by itself. No counterpart in the reference, whose integrate() always re-evaluates. */
 int mdx_positions_changed(mdx_ctx* ctx);
+/* mdx_celllist_reference without a context and without any device allocation. No counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code around stream
+ * capture, tight serving loops). Positions d_pos (n x 3 f64, device) in a box of edges h_box;
+ * enqueued on `hip_stream` (NULL = default stream); writes d_cell_index (n x 3 int32) and returns
+ * the grid in h_n[3] and the largest cell population in h_maxpc (synchronises). Its reductions
+ * and per-cell counts use one fixed device array per GPU (held until the process ends) of up to
+ * 16,384 cells: larger grids return MDX_ERR_INVALID. Calls that overlap on one GPU race on that
+ * array; each call clears what it uses first. */
+int mdx_celllist_reference_noalloc(void* hip_stream, int n, const double* d_pos, const double* h_box,
+                                   double cell_cutoff, int* d_cell_index, int* h_n, int* h_maxpc);
 /* Dense cell table [nx][ny][nz][maxpc], atoms ascending within a cell, -1 padded (cell_list.py:105-116),
  * for the binning of the last mdx_celllist_reference call. */
 int mdx_celllist_table(mdx_ctx* ctx, int* d_table, int maxpc);

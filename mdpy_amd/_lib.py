@@ -81,6 +81,8 @@ SIGNATURES = {
                                        ctypes.c_double, ctypes.c_int, _c_int_p, _c_dbl_p]),
     "mdx_pair_histogram": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _c_dbl_p, _vp, ctypes.c_int, _vp,
                                           ctypes.c_int, _vp, ctypes.c_int, _vp]),
+    "mdx_celllist_reference_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _c_dbl_p, ctypes.c_double, _vp,
+                                                      _c_int_p, _c_int_p]),
     "mdx_celllist_reference": (ctypes.c_int, [_vp, ctypes.c_double, _vp, _c_int_p, _c_int_p]),
     "mdx_celllist_table": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
     "mdx_nonbonded_config": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double]),

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmdx.so")
 SOURCES = ["mdx_api.hip", "mdx_nlist.hip", "mdx_nonbonded.hip", "mdx_pme.hip", "mdx_bonded.hip", "mdx_mlp.hip",
-           "mdx_analyse.hip"]
+           "mdx_noalloc.hip", "mdx_analyse.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -401,6 +401,8 @@ int mdx_leave(mdx_ctx* c);
 // drop the context's captured step graph (and mark graphs of batches holding it stale)
 void mdx_invalidate_graph(mdx_ctx* c);
 // celllist / neighbour list
+// reference grid of a cell-list cutoff: n = max(3, floor((L + 2)/rc)), w = (L + 2)/n, 1/w
+void mdx_ref_grid(const double* box, double rc, int* n, double* w, double* inv);
 int mdx_nlist_prepare(mdx_ctx* c);                    // allocate grid/list buffers for current config
 int mdx_nlist_update(mdx_ctx* c, bool force_rebuild);  // enqueue displacement check + conditional rebuild + pack
 int mdx_nlist_update_order(mdx_ctx* c, bool force_rebuild);  // its first part: up to the sorted arrays and pack

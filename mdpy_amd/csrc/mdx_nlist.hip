@@ -79,7 +79,7 @@ __global__ void k_ref_bin(const double* __restrict__ pos, int n, int nx, int ny,
     if (cell_index) cell_index[3 * i + d] = k;
   }
   int lin = (c[0] * ny + c[1]) * nz + c[2];
-  cell_lin[i] = lin;
+  if (cell_lin) cell_lin[i] = lin;
   int cnt = atomicAdd(&counts[lin], 1) + 1;
   atomicMax(&st->maxpc, cnt);
 }
@@ -128,13 +128,15 @@ __global__ void k_fill_i32(int* __restrict__ p, int v, long long n) {
   if (i < n) p[i] = v;
 }
 
-// reset the reference-binning reductions (minimum position per axis, largest population)
+// reset the reference-binning reductions (minimum position per axis, largest population,
+// flags) and the per-cell counts
 __global__ void k_ref_reset(DevState* st, int* __restrict__ counts, int nc) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nc) counts[i] = 0;
   if (i == 0) {
     st->minpos[0] = st->minpos[1] = st->minpos[2] = ~0ull;
     st->maxpc = 0;
+    st->flags = 0;
   }
 }
 
@@ -158,7 +160,7 @@ extern "C" int mdx_wrap_positions(mdx_ctx* c) {
 }
 
 // Reference grid for a cell-list cutoff (cell_list.py:74-81).
-static void ref_grid(const double* box, double rc, int* n, double* w, double* inv) {
+void mdx_ref_grid(const double* box, double rc, int* n, double* w, double* inv) {
   for (int d = 0; d < 3; ++d) {
     int k = (int)std::floor((box[d] + 2.0) / rc);
     if (k < 3) k = 3;
@@ -169,7 +171,7 @@ static void ref_grid(const double* box, double rc, int* n, double* w, double* in
 }
 
 static int ref_bin_enqueue(mdx_ctx* c, double cell_cutoff, int* d_cell_index) {
-  ref_grid(c->box, cell_cutoff, c->ref_n, c->ref_w, c->ref_inv);
+  mdx_ref_grid(c->box, cell_cutoff, c->ref_n, c->ref_w, c->ref_inv);
   int nc = c->ref_n[0] * c->ref_n[1] * c->ref_n[2];
   int rc;
   if ((rc = c->ref_cell_lin.ensure(c->n))) return rc;
