@@ -7,7 +7,7 @@
 //                |T_{e,n,l}|^2, T_l = sum_j g_n(r) rhat^(x l) (4 channels x l = 0..3), i.e.
 //                sum_{j,k} g_n g_n (cos theta_jk)^l computed from moments in O(neighbours)
 //   network      per central element: (G - mean)/std -> 3 x (256, SiLU) -> 1, BF16 weights and
-//                activations, FP32 accumulation on MFMA (v_mfma_f32_32x32x8_bf16)
+//                activations, FP32 accumulation on MFMA (v_mfma_f32_32x32x16_bf16)
 //   forces       F_j = sum_i dE_j/dr_ji - dE_i/dr_ij, one lane per neighbour pair, int64
 //                fixed-point atomics (deterministic)
 //   training     force matching; dL/dW by a forward-mode pass along u = dL/d(dE/dG) followed by
@@ -40,6 +40,7 @@
 
 typedef unsigned short bf16_t;
 typedef short __attribute__((ext_vector_type(4))) bf16x4;
+typedef __bf16 __attribute__((ext_vector_type(8))) bf16x8;  // one gfx950 K=16 MFMA operand per lane
 typedef float __attribute__((ext_vector_type(16))) f32x16;
 
 __device__ __forceinline__ bf16_t f2bf(float f) {  // round to nearest even
@@ -481,8 +482,8 @@ __global__ __launch_bounds__(64) void k_ml_jvp(MlAtoms a, MlConst k, const float
 // ---------------------------------------------------------------- BF16 MFMA GEMM
 // C[m][n] = sum_k A[m][k] B[k][n] over k in this block's K range, A and B read through
 // arbitrary (row, col) strides (transposes are strides). 64x64 tile per 256-lane workgroup,
-// each wave a 32x32 sub-tile on v_mfma_f32_32x32x8_bf16 (lane l: A row l%32, k 4(l/32)..+3;
-// B col l%32, same k; D row 8(i/4) + 4(l/32) + i%4, col l%32 for accumulator i).
+// each wave a 32x32 sub-tile on gfx950's v_mfma_f32_32x32x16_bf16 (lane l: A row l%32, k
+// 8(l/32)..+7; B col l%32, same k; D row 8(i/4) + 4(l/32) + i%4, col l%32 for accumulator i).
 // Rows >= row_split use the second operand B1 / bias1 (the second element's weights).
 enum { EPI_F32 = 0, EPI_F32_ACC = 1, EPI_BIAS_SILU = 2, EPI_MUL_S = 3 };
 struct GemmArgs {
@@ -572,12 +573,14 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g, int epi) {
     __syncthreads();
     if (k0 + GK < ke) fetch(k0 + GK, va, vb);  // next step's loads in flight during the MFMAs
 #pragma unroll
-    for (int ks = 0; ks < GK; ks += 8) {
-      const int ar = wm * 32 + (lane & 31), kq = ks + (lane >> 5) * 4;
+    for (int ks = 0; ks < GK; ks += 16) {
+      // v_mfma_f32_32x32x16_bf16: lane l supplies row (or column) l%32 and k 8(l/32)..+7, one
+      // 16-byte LDS read per operand (rows are 80 B, k offsets multiples of 8: 16-byte aligned)
+      const int ar = wm * 32 + (lane & 31), kq = ks + (lane >> 5) * 8;
       const int bc = wn * 32 + (lane & 31);
-      const bf16x4 av = *(const bf16x4*)&As[ar][kq];
-      const bf16x4 bv = *(const bf16x4*)&Bs[bc][kq];
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+      const bf16x8 av = *(const bf16x8*)&As[ar][kq];
+      const bf16x8 bv = *(const bf16x8*)&Bs[bc][kq];
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
     }
     __syncthreads();
   }

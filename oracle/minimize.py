@@ -40,7 +40,7 @@ def conjugate_gradient(x, force_energy, wrap, theta, force_tol, max_sub_iteratio
             x = wrap(x + theta * t.reshape(-1, 3))
             F, E = force_energy(x)
             omega = -(F.reshape(-1, 1) - cur_f) / theta
-            alpha = float(cur_f.T @ cur_f) / float(omega.T @ t)
+            alpha = (cur_f.T @ cur_f).item() / (omega.T @ t).item()
             d = alpha * t
             cur_f = cur_f + alpha * omega
             if (cur_f ** 2).sum() / (f0 ** 2).sum() < force_tol:
