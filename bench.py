@@ -39,6 +39,28 @@ def maxwell_velocities(masses, temperature, seed):
     return v
 
 
+def host_cpu():
+    """Model, clock and core count of the host that runs the CPU baseline (from lscpu)."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "usable_cores": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Model name":
+                info["model"] = v
+            elif k in ("CPU max MHz", "CPU MHz") and "mhz" not in info:
+                info["mhz"] = float(v)
+            elif k == "Socket(s)":
+                info["sockets"] = v
+            elif k == "Thread(s) per core":
+                info["threads_per_core"] = v
+    except (OSError, ValueError, subprocess.SubprocessError) as exc:
+        info["lscpu_error"] = str(exc)
+    return info
+
+
 def cpu_baseline(system, steps=2):
     """Time the C restatement of the reference CPU Langevin step (1 thread)."""
     from mdpy_amd.unit import COULOMB_K, KB
@@ -50,7 +72,7 @@ def cpu_baseline(system, steps=2):
     wall = time.perf_counter() - t0
     evals = 2 + 3 * (steps - 1)            # langevin_integrator.py:51-53,64
     s_per_step = wall / evals * 3          # steady state: three evaluations per step
-    return {"value": 0.0864 / s_per_step, "unit": "ns/day", "cores": 1, "kind": "port",
+    return {"value": 0.0864 / s_per_step, "unit": "ns/day", "cores": 1, "kind": "port", "host": host_cpu(),
             "sample": "%d reference Langevin steps (%d force evaluations) of the same 23,328-atom box, "
                       "%.1f s wall; steady-state step = 3 evaluations" % (steps, evals, wall)}
 
@@ -165,6 +187,10 @@ def replicas_leg(steps=200):
     return out
 
 
+def progress(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def mlp_leg(s, e, terms, plain_ns_day, n_frames=1000, every=10, train_budget_s=40.0, max_epochs=8):
     """Train the surrogate on frames of this box (forces of the plain LJ + PME path, 10 steps
     apart) within a wall-clock budget, then measure its force error on held-out frames, its step
@@ -240,9 +266,14 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
-    # untimed steps before the timed ones; the GPU reaches its steady clock only after ~0.3 s of
-    # sustained stepping (200 warm-up steps measure ~3 % below 3,000)
-    ap.add_argument("--warmup", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=3000, help="untimed steps before the timed ones")
+    # The synthetic box starts from a lattice: over its first few picoseconds it relaxes and its
+    # atoms move faster than in the 300 K liquid, so the neighbour list is rebuilt about every 6
+    # steps instead of every ~10 (a rebuild step costs about twice a plain one). Preparing the
+    # input relaxes it first, whatever --warmup is, so short and long runs time the same liquid.
+    ap.add_argument("--equilibrate", type=int, default=3000,
+                    help="untimed G-JF steps that relax the synthetic lattice-start box into the 300 K "
+                         "liquid before the warm-up (part of preparing the input)")
     ap.add_argument("--skin", type=float, default=1.5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -288,6 +319,9 @@ def main():
     use_graph = not args.no_graph
 
     e.set_nonbonded_persist(args.persist)
+    if args.equilibrate > 0:
+        progress("relaxing the synthetic box: %d steps" % args.equilibrate)
+        e.integrate(terms, args.equilibrate, use_graph, args.energy_every)
     phases = None
     if args.phases != "off":
         phases = [float(v) for v in args.phases.split(",")]
@@ -300,10 +334,12 @@ def main():
         e.tune_nonbonded(L.TERM_LJ | L.TERM_COUL_DIRECT, args.lanes)
     e.integrate(terms, min(args.warmup, 500), use_graph, args.energy_every)
     torch.cuda.synchronize()
+    rebuilds0 = e.stats()["rebuilds"]
     t0 = time.perf_counter()
     e.integrate(terms, args.steps, use_graph, args.energy_every)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    rebuilds_timed = e.stats()["rebuilds"] - rebuilds0
     s_per_step = (t1 - t0) / args.steps
     ns_day = 0.0864 / s_per_step  # 1 fs steps
     status = e.status()
@@ -337,7 +373,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 pair/PME, fp64 x/v/integrator, int64 fixed-point forces",
-        "data": "synthetic seeded water box (no files)",
+        "data": "synthetic seeded water box (no files), relaxed for %d untimed steps before the warm-up"
+                % args.equilibrate,
         "config": {"workload": "water23328: 7,776 TIP3P-like waters, 61.9 A cube, r_c 12 A, PME 64^3 "
                                "order 4, G-JF Langevin 300 K, dt 1 fs, plain FP32 path",
                    "atoms": s.num_atoms, "list_skin_A": args.skin, "hipgraph": use_graph, "nb_lanes": lanes,
@@ -350,12 +387,10 @@ def main():
                      "pairs_per_s": pairs / (nb_ms * 1e-3), "flops_per_pair": FLOPS_PER_PAIR},
         "breakdown_ms": {"force_eval": eval_ms, "pme_reciprocal": pme_ms, "nonbonded": nb_ms},
         "list_rebuilds": stats["rebuilds"],
+        "list_rebuilds_timed": rebuilds_timed,
         "energies_finite": finite,
         "device_flags": status,
     }
-    def progress(msg):
-        print("bench: " + msg, file=sys.stderr, flush=True)
-
     if not args.no_nve:
         progress("NVE drift leg")
         out["nve"] = nve_leg(e, terms, s.masses)

@@ -92,11 +92,6 @@ __global__ void k_zero_acc(long long* __restrict__ ffix, long long n3, long long
 
 // start of an integrate call: its energy-history rows count from the current step, and the
 // kinetic-energy slot is recomputed from the current velocities
-__global__ void k_begin_call(DevState* st, long long* __restrict__ ke_slot) {
-  st->hist_base = st->step;
-  *ke_slot = 0;
-}
-
 __global__ void k_record(const long long* __restrict__ efix, int ne, double* __restrict__ ehist, int ehist_cap,
                          DevState* st, int every) {
   long long k = st->step - st->hist_base;
@@ -217,18 +212,21 @@ __global__ void k_vv_a(IntArgs A, double* __restrict__ x, double* __restrict__ v
   }
 }
 // velocity Verlet, second half: v += dt/2 a; kinetic energy
+__device__ __forceinline__ double vv_half(const IntArgs& A, double* __restrict__ v, const double* __restrict__ im,
+                                          const double* __restrict__ m, const long long* __restrict__ ffix, int i) {
+  double e = 0;
+  for (int d = 0; d < 3; ++d) {
+    double vv = v[3 * i + d] + 0.5 * A.dt * fix_force(ffix, A, i, d) * im[i];
+    v[3 * i + d] = vv;
+    e += vv * vv;
+  }
+  return e * (0.5 * m[i]);
+}
 __global__ void k_vv_b(IntArgs A, double* __restrict__ v, const double* __restrict__ im, const double* __restrict__ m,
                        const long long* __restrict__ ffix, long long* __restrict__ ke) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double e = 0;
-  if (i < A.n) {
-    for (int d = 0; d < 3; ++d) {
-      double vv = v[3 * i + d] + 0.5 * A.dt * fix_force(ffix, A, i, d) * im[i];
-      v[3 * i + d] = vv;
-      e += vv * vv;
-    }
-    e *= 0.5 * m[i];
-  }
+  if (i < A.n) e = vv_half(A, v, im, m, ffix, i);
   block_add_energy(e, ke);
 }
 
@@ -470,23 +468,27 @@ __global__ __launch_bounds__(PRO_THREADS) void k_step_pro(IntArgs A, StepPro P, 
 }
 
 // velocity half: v_{n+1} = a v_n + dt/2 (a a_n + a_{n+1}) + b sigma sqrt(dt) xi_{n+1}/sqrt(m)
+__device__ __forceinline__ double gjf_half(const IntArgs& A, double* __restrict__ v, const double* __restrict__ im,
+                                           const double* __restrict__ m, const long long* __restrict__ ffix,
+                                           const double* __restrict__ acc_prev, int i) {
+  double e = 0;
+  double g[3];  // xi_{n+1}: drawn by the position half
+  for (int d = 0; d < 3; ++d) g[d] = A.gsave[3 * i + d];
+  double sm = sqrt(im[i]);
+  for (int d = 0; d < 3; ++d) {
+    double acc = fix_force(ffix, A, i, d) * im[i];
+    double vv = A.a * v[3 * i + d] + 0.5 * A.dt * (A.a * acc_prev[3 * i + d] + acc) + A.b * A.sigma * sqrt(A.dt) * g[d] * sm;
+    v[3 * i + d] = vv;
+    e += vv * vv;
+  }
+  return e * (0.5 * m[i]);
+}
 __global__ void k_gjf_b(IntArgs A, double* __restrict__ v, const double* __restrict__ im, const double* __restrict__ m,
                         const long long* __restrict__ ffix, const double* __restrict__ acc_prev, const DevState* st,
                         long long* __restrict__ ke) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double e = 0;
-  if (i < A.n) {
-    double g[3];  // xi_{n+1}: drawn by the position half
-    for (int d = 0; d < 3; ++d) g[d] = A.gsave[3 * i + d];
-    double sm = sqrt(im[i]);
-    for (int d = 0; d < 3; ++d) {
-      double acc = fix_force(ffix, A, i, d) * im[i];
-      double vv = A.a * v[3 * i + d] + 0.5 * A.dt * (A.a * acc_prev[3 * i + d] + acc) + A.b * A.sigma * sqrt(A.dt) * g[d] * sm;
-      v[3 * i + d] = vv;
-      e += vv * vv;
-    }
-    e *= 0.5 * m[i];
-  }
+  if (i < A.n) e = gjf_half(A, v, im, m, ffix, acc_prev, i);
   block_add_energy(e, ke);
 }
 
@@ -612,6 +614,66 @@ __global__ void k_wrap_state(IntArgs A, double* __restrict__ pos, DevState* st) 
   wrap_into(A, x, pos + 3 * i, st);
 }
 
+// true in one lane (thread 0) of the grid's last workgroup to finish; every workgroup's earlier
+// global writes are visible to it. The ticket is left at 0 for the next launch.
+__device__ bool last_workgroup(unsigned* ticket) {
+  __shared__ int last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    if (last) {
+      __threadfence();
+      atomicExch(ticket, 0u);
+    }
+  }
+  __syncthreads();
+  return last && threadIdx.x == 0;
+}
+
+// An integrate call's opening kernel: the kinetic energy of the incoming velocities (k_kinetic's
+// sum), written (not added) into its slot by the last workgroup, which also sets the call's
+// energy-history base (rows are device step counter - base: no host round trip).
+__global__ void k_call_start(int n, const double* __restrict__ v, const double* __restrict__ m,
+                             long long* __restrict__ ke_slot, DevState* st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0;
+  if (i < n) e = 0.5 * m[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+  block_add_energy(e, (long long*)&st->ke_acc);
+  if (last_workgroup(&st->call_ticket)) {
+    *ke_slot = (long long)atomicExch(&st->ke_acc, 0ull);
+    st->hist_base = st->step;
+  }
+}
+
+// An integrate call's closing kernel for G-JF / velocity Verlet (the closing velocity half as
+// k_gjf_b / k_vv_b, k_wrap_state and k_record in one launch): the last velocity half and its kinetic energy, the
+// positions wrapped as the reference leaves them, then the last workgroup records the energies.
+template <bool GJF>
+__global__ void k_close(IntArgs A, double* __restrict__ v, double* __restrict__ pos, const double* __restrict__ im,
+                        const double* __restrict__ m, const long long* __restrict__ ffix,
+                        const double* __restrict__ acc_prev, DevState* st, long long* __restrict__ efix, int ne,
+                        double* __restrict__ ehist, int ehist_cap, int every) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0;
+  if (i < A.n) {
+    e = GJF ? gjf_half(A, v, im, m, ffix, acc_prev, i) : vv_half(A, v, im, m, ffix, i);
+    double x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    wrap_into(A, x, pos + 3 * i, st);
+  }
+  block_add_energy(e, efix + MDX_E_KINETIC);
+  if (last_workgroup(&st->call_ticket)) {
+    const long long k = st->step - st->hist_base;
+    if (k >= 0 && k % every == 0 && k / every < ehist_cap)
+      for (int t = 0; t < ne; ++t) {
+        // read through the atomic path (the other workgroups' sums are L2 atomics); the slots
+        // hold signed fixed-point values
+        const long long v = (long long)atomicAdd((unsigned long long*)&efix[t], 0ull);
+        ehist[(k / every) * ne + t] = (double)v * kFixInv;
+      }
+  }
+}
+
 // ------------------------------------------------------------------ context API
 extern "C" int mdx_ctx_create(int device, void* hip_stream, mdx_ctx** out) {
   MDX_CHECK(out, MDX_ERR_INVALID, "mdx_ctx_create: out is NULL");
@@ -696,7 +758,7 @@ extern "C" int mdx_ctx_destroy(mdx_ctx* c) {
 void mdx_invalidate_graph(mdx_ctx* c) {
   c->graph_gen++;
   c->forces_current = false;
-  bool any = c->graph_e != nullptr;
+  bool any = c->graph_e != nullptr || c->graph_head != nullptr || c->graph_tail != nullptr;
   for (hipGraphExec_t g : c->graph_f) any |= g != nullptr;
   if (any) {
     // a replay of these graphs may still be in flight on the context's streams (launches return
@@ -707,8 +769,10 @@ void mdx_invalidate_graph(mdx_ctx* c) {
       if (g) (void)hipGraphExecDestroy(g);
       g = nullptr;
     }
-    if (c->graph_e) (void)hipGraphExecDestroy(c->graph_e);
-    c->graph_e = nullptr;
+    for (hipGraphExec_t* g : {&c->graph_e, &c->graph_head, &c->graph_tail}) {
+      if (*g) (void)hipGraphExecDestroy(*g);
+      *g = nullptr;
+    }
   }
   c->graph_kind = -1;
 }
@@ -721,7 +785,7 @@ extern "C" int mdx_positions_changed(mdx_ctx* c) {
 
 extern "C" int mdx_system_set(mdx_ctx* c, int n, double* d_pos, double* d_vel, const double* h_box) {
   MDX_CHECK(c && n > 0 && d_pos && h_box, MDX_ERR_INVALID, "mdx_system_set: bad arguments");
-  MDX_CHECK(n < (1 << 30), MDX_ERR_INVALID, "too many atoms");
+  MDX_CHECK(n < (1 << 28) - 1, MDX_ERR_INVALID, "too many atoms (at most 2^28 - 2)");
   MDX_HIP(hipSetDevice(c->device));
   mdx_invalidate_graph(c);
   bool resized = c->n != n;
@@ -1088,6 +1152,13 @@ extern "C" int mdx_integrator_config(mdx_ctx* c, int kind, double dt, double gam
 
 extern "C" int mdx_set_energy_interval(mdx_ctx* c, int every) {
   MDX_CHECK(c && every >= 1, MDX_ERR_INVALID, "energy interval must be >= 1");
+  if (every != c->energy_every) {
+    // the captured step graphs hold the interval (energy-history rows): capture again; the
+    // forces of the last step stay current
+    const bool cur = c->forces_current;
+    mdx_invalidate_graph(c);
+    c->forces_current = cur;
+  }
   c->energy_every = every;
   return 0;
 }
@@ -1280,23 +1351,23 @@ static int integ_alloc(mdx_ctx* c, int nsteps) {
   return 0;
 }
 
+static bool prime_skipped(const mdx_ctx* c, unsigned terms) {
+  return c->forces_current && c->forces_terms == terms && fused_steps(c, terms);
+}
+
 static int integ_begin(mdx_ctx* c, unsigned terms, int nsteps) {
   int rc = integ_alloc(c, nsteps);
   if (rc) return rc;
   dim3 g(div_up(c->n, 256)), b(256);
-  // energy-history rows are (device step counter - hist_base); the base of this call is set on
-  // the device (no host round trip)
-  k_begin_call<<<1, 1, 0, c->stream>>>(c->dstate.p, c->efix.p + MDX_E_KINETIC);
-  MDX_LAUNCH_CHECK();
   c->ehist_count = 0;
   // prime: forces at the current positions (every mdpy integrate call starts with ensemble.update,
   // integrator.py:26-28). The step kernels of G-JF and velocity Verlet keep the last step's forces
   // and potential energies (the last step of a call always evaluates energies); when nothing has
   // changed since, they are the forces at the current positions and the prime is skipped.
-  const bool reuse = c->forces_current && c->forces_terms == terms && fused_steps(c, terms);
+  const bool reuse = prime_skipped(c, terms);
   if (!reuse && ((rc = mdx_enqueue_zero_accumulators(c)) || (rc = mdx_enqueue_forces(c, terms, true)))) return rc;
   IntArgs A = int_args(c);
-  k_kinetic<<<g, b, 0, c->stream>>>(c->n, c->vel, c->mass.p, c->efix.p + MDX_E_KINETIC);
+  k_call_start<<<g, b, 0, c->stream>>>(c->n, c->vel, c->mass.p, c->efix.p + MDX_E_KINETIC, c->dstate.p);
   if (c->int_kind == MDX_INT_VERLET_REF) {
     k_vref_prime<<<g, b, 0, c->stream>>>(A, c->pos, c->vel, c->inv_mass.p, c->ffix.p, c->xa.p, c->xb.p,
                                          c->int_primed ? 1 : 0);
@@ -1309,13 +1380,13 @@ static int integ_begin(mdx_ctx* c, unsigned terms, int nsteps) {
   return 0;
 }
 
-static int integ_end(mdx_ctx* c, int nsteps) {
+// the closing kernels of a call: the last energy record, then the state as the reference leaves it
+static int integ_end_kernels(mdx_ctx* c) {
   const int ne = MDX_NTERMS + 1;
   dim3 g(div_up(c->n, 256)), b(256);
   IntArgs A = int_args(c);
   k_record<<<1, 1, 0, c->stream>>>(c->efix.p, ne, c->ehist.p, c->ehist_cap, c->dstate.p, c->energy_every);
   MDX_LAUNCH_CHECK();
-  c->ehist_count = nsteps / c->energy_every + 1;
   // finish: state velocities / wrapped positions as the reference leaves them
   if (c->int_kind == MDX_INT_VERLET_REF) {
     k_vref_finish<<<g, b, 0, c->stream>>>(A, c->xa.p, c->xb.p, c->vel);
@@ -1325,6 +1396,15 @@ static int integ_end(mdx_ctx* c, int nsteps) {
     k_wrap_state<<<g, b, 0, c->stream>>>(A, c->pos, c->dstate.p);
   }
   MDX_LAUNCH_CHECK();
+  return 0;
+}
+
+static int integ_end(mdx_ctx* c, int nsteps, bool kernels) {
+  if (kernels) {
+    int rc = integ_end_kernels(c);
+    if (rc) return rc;
+  }
+  c->ehist_count = nsteps / c->energy_every + 1;
   c->evals += nsteps + 1;
   // The wrap moves atoms by whole box edges: the list's displacement test is a minimum-image
   // distance and every kernel re-wraps positions, so the list stays valid. The reference-
@@ -1342,7 +1422,7 @@ static bool energy_step(const mdx_ctx* c, int local, int nsteps) {
 
 // steps local0 .. local0+count-1 of the current call. With the step prologue every step but
 // the call's first folds the previous step's velocity half into its prologue (fuse_first for the
-// first of these), and the call ends with one standalone velocity half (integ_final_b), so graph
+// first of these), and the call ends with one standalone velocity half (integ_close), so graph
 // replays and direct launches run the same kernels in the same order.
 static int integ_steps(mdx_ctx* c, unsigned terms, int local0, int count, bool fuse_first, int nsteps) {
   int rc = 0;
@@ -1351,15 +1431,20 @@ static int integ_steps(mdx_ctx* c, unsigned terms, int local0, int count, bool f
   return rc;
 }
 
-static int integ_final_b(mdx_ctx* c, unsigned terms, int nsteps) {
-  if (nsteps <= 0 || !fused_steps(c, terms)) return 0;  // other integrators finish every step
+// the closing kernels of a call: for G-JF / velocity Verlet with the step prologue, one launch
+// (the closing velocity half, the wrap and the last energy record); otherwise the energy record
+// and the reference integrators' finishing kernels
+static int integ_close(mdx_ctx* c, unsigned terms, int nsteps) {
+  if (nsteps <= 0 || !fused_steps(c, terms)) return integ_end_kernels(c);
   IntArgs A = int_args(c);
   dim3 g(div_up(c->n, 256)), b(256);
-  long long* ke = c->efix.p + MDX_E_KINETIC;
+  const int ne = MDX_NTERMS + 1;
   if (c->int_kind == MDX_INT_LANGEVIN)
-    k_gjf_b<<<g, b, 0, c->stream>>>(A, c->vel, c->inv_mass.p, c->mass.p, c->ffix.p, c->acc_prev.p, c->dstate.p, ke);
+    k_close<true><<<g, b, 0, c->stream>>>(A, c->vel, c->pos, c->inv_mass.p, c->mass.p, c->ffix.p, c->acc_prev.p,
+                                          c->dstate.p, c->efix.p, ne, c->ehist.p, c->ehist_cap, c->energy_every);
   else
-    k_vv_b<<<g, b, 0, c->stream>>>(A, c->vel, c->inv_mass.p, c->mass.p, c->ffix.p, ke);
+    k_close<false><<<g, b, 0, c->stream>>>(A, c->vel, c->pos, c->inv_mass.p, c->mass.p, c->ffix.p, c->acc_prev.p,
+                                           c->dstate.p, c->efix.p, ne, c->ehist.p, c->ehist_cap, c->energy_every);
   MDX_LAUNCH_CHECK();
   return 0;
 }
@@ -1369,13 +1454,14 @@ static int integ_final_b(mdx_ctx* c, unsigned terms, int nsteps) {
 // its first are a sequence of forces-only runs separated by energy steps; each run is covered
 // by the largest graphs that fit, so the same executable graphs serve every call length and are
 // captured once per (terms, integrator kind).
-static const int kGraphSizes[4] = {32, 16, 4, 1};
+static const int kGraphSizes[MDX_NGRAPH_SIZES] = {32, 16, 8, 4, 2, 1};
+static const int kGraphEnergy = MDX_NGRAPH_SIZES, kGraphTail = MDX_NGRAPH_SIZES + 1;  // graph_of indices
 
-static int capture_steps(mdx_ctx* c, unsigned terms, int count, bool energy, hipGraphExec_t* out) {
+template <class Enqueue>
+static int capture(mdx_ctx* c, Enqueue enqueue, hipGraphExec_t* out) {
   hipGraph_t graph;
   MDX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  int rc = 0;
-  for (int k = 0; k < count && !rc; ++k) rc = mdx_integrator_enqueue_step(c, terms, energy, true, false);
+  int rc = enqueue();
   hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
   if (rc) return rc;
   if (e2 != hipSuccess) { mdx_set_error("graph capture failed: %s", hipGetErrorString(e2)); return MDX_ERR_HIP; }
@@ -1385,14 +1471,43 @@ static int capture_steps(mdx_ctx* c, unsigned terms, int count, bool energy, hip
   return 0;
 }
 
+static int capture_steps(mdx_ctx* c, unsigned terms, int count, bool energy, hipGraphExec_t* out) {
+  return capture(c, [&]() {
+    int rc = 0;
+    for (int k = 0; k < count && !rc; ++k) rc = mdx_integrator_enqueue_step(c, terms, energy, true, false);
+    return rc;
+  }, out);
+}
+
+// a call's set-up and first step when the prime is skipped (integ_begin + integ_steps(0, 1))
+static int capture_head(mdx_ctx* c, unsigned terms, hipGraphExec_t* out) {
+  return capture(c, [&]() {
+    dim3 g(div_up(c->n, 256)), b(256);
+    k_call_start<<<g, b, 0, c->stream>>>(c->n, c->vel, c->mass.p, c->efix.p + MDX_E_KINETIC, c->dstate.p);
+    MDX_LAUNCH_CHECK();
+    return mdx_integrator_enqueue_step(c, terms, false, false, false);
+  }, out);
+}
+
+// a call's last step (an energy step), its closing velocity half and the closing kernels
+static int capture_tail(mdx_ctx* c, unsigned terms, hipGraphExec_t* out) {
+  return capture(c, [&]() {
+    int rc = mdx_integrator_enqueue_step(c, terms, true, true, false);
+    if (!rc) rc = integ_close(c, terms, 1);
+    return rc;
+  }, out);
+}
+
 static int ensure_graphs(mdx_ctx* c, unsigned terms) {
   if (c->graph_e && c->graph_terms == terms && c->graph_kind == c->int_kind) return 0;
   const bool cur = c->forces_current;  // capturing enqueues nothing that runs
   const unsigned cur_terms = c->forces_terms;
   mdx_invalidate_graph(c);
   int rc = 0;
-  for (int k = 0; k < 4 && !rc; ++k) rc = capture_steps(c, terms, kGraphSizes[k], false, &c->graph_f[k]);
+  for (int k = 0; k < MDX_NGRAPH_SIZES && !rc; ++k) rc = capture_steps(c, terms, kGraphSizes[k], false, &c->graph_f[k]);
   if (!rc) rc = capture_steps(c, terms, 1, true, &c->graph_e);
+  if (!rc && fused_steps(c, terms)) rc = capture_head(c, terms, &c->graph_head);
+  if (!rc && fused_steps(c, terms)) rc = capture_tail(c, terms, &c->graph_tail);
   if (rc) {
     mdx_invalidate_graph(c);
     return rc;
@@ -1405,31 +1520,54 @@ static int ensure_graphs(mdx_ctx* c, unsigned terms) {
   return 0;
 }
 
-// the launches covering steps 1 .. nsteps-1 of a call (graph index, 4 = the energy step)
+// the launches covering steps 1 .. nsteps-1 of a call (graph_of indices: a forces-only size, or
+// kGraphEnergy for an energy step)
 static void graph_schedule(const mdx_ctx* c, int nsteps, std::vector<int>& seq) {
   seq.clear();
   int s = 1;
   while (s < nsteps) {
     if (energy_step(c, s, nsteps)) {
-      seq.push_back(4);
+      seq.push_back(kGraphEnergy);
       ++s;
       continue;
     }
     int run = 0;
     while (s + run < nsteps && !energy_step(c, s + run, nsteps)) ++run;
     s += run;
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < MDX_NGRAPH_SIZES; ++k)
       for (; run >= kGraphSizes[k]; run -= kGraphSizes[k]) seq.push_back(k);
   }
 }
 
-static hipGraphExec_t graph_of(const mdx_ctx* c, int k) { return k == 4 ? c->graph_e : c->graph_f[k]; }
+static hipGraphExec_t graph_of(const mdx_ctx* c, int k) {
+  return k == kGraphTail ? c->graph_tail : (k == kGraphEnergy ? c->graph_e : c->graph_f[k]);
+}
 
 extern "C" int mdx_integrate(mdx_ctx* c, unsigned terms, int nsteps, int use_graph) {
   MDX_CHECK(c, MDX_ERR_INVALID, "null context");
   ENTER();
   int rc = integ_alloc(c, nsteps);
   if (!rc && use_graph && nsteps > 1) rc = ensure_graphs(c, terms);
+  if (rc) LEAVE_RET(rc);
+  // a call of n >= 2 steps whose first step needs no prime and no energies: the head graph
+  // (set-up + step 0), the step graphs, then the tail graph (step n-1 + the closing kernels)
+  if (use_graph && nsteps > 1 && c->graph_head && c->graph_tail && prime_skipped(c, terms) &&
+      !energy_step(c, 0, nsteps)) {
+    c->ehist_count = 0;
+    c->int_primed = true;
+    std::vector<int> seq;
+    graph_schedule(c, nsteps, seq);  // ends with step n-1, an energy step: the tail graph's
+    seq.back() = kGraphTail;
+    hipError_t e = hipGraphLaunch(c->graph_head, c->stream);
+    for (size_t k = 0; k < seq.size() && e == hipSuccess; ++k) e = hipGraphLaunch(graph_of(c, seq[k]), c->stream);
+    if (e != hipSuccess) { mdx_set_error("hipGraphLaunch: %s", hipGetErrorString(e)); LEAVE_RET(MDX_ERR_HIP); }
+    rc = integ_end(c, nsteps, false);
+    if (!rc) {
+      c->forces_current = true;
+      c->forces_terms = terms;
+    }
+    LEAVE_RET(rc);
+  }
   if (!rc) rc = integ_begin(c, terms, nsteps);
   if (rc) LEAVE_RET(rc);
   // step 0 of the call is launched directly (its prologue has no previous velocity half), the
@@ -1445,9 +1583,9 @@ extern "C" int mdx_integrate(mdx_ctx* c, unsigned terms, int nsteps, int use_gra
   } else {
     rc = integ_steps(c, terms, 0, nsteps, false, nsteps);
   }
-  if (!rc) rc = integ_final_b(c, terms, nsteps);
+  if (!rc) rc = integ_close(c, terms, nsteps);
   if (rc) LEAVE_RET(rc);
-  rc = integ_end(c, nsteps);
+  rc = integ_end(c, nsteps, false);
   if (!rc && fused_steps(c, terms)) {
     c->forces_current = true;
     c->forces_terms = terms;
@@ -1570,8 +1708,8 @@ extern "C" int mdx_batch_integrate(mdx_batch* b, unsigned terms, int nsteps, int
     }
   }
   for (mdx_ctx* c : b->ctx) {
-    if ((rc = integ_final_b(c, terms, nsteps))) return rc;
-    if ((rc = integ_end(c, nsteps))) return rc;
+    if ((rc = integ_close(c, terms, nsteps))) return rc;
+    if ((rc = integ_end(c, nsteps, false))) return rc;
     if (fused_steps(c, terms)) {
       c->forces_current = true;
       c->forces_terms = terms;

@@ -87,6 +87,8 @@ struct DevBuf {
 };
 
 // Per-context counters and flags kept in one small device block (zeroed at creation).
+#define MDX_NGRAPH_SIZES 6  // forces-only step graphs of 32, 16, 8, 4, 2 and 1 steps
+
 struct DevState {
   unsigned flags;           // MDX_FLAG_*
   int rebuild;              // 1 when the neighbour list must be rebuilt before use
@@ -103,6 +105,9 @@ struct DevState {
   unsigned step_ticket;     // finished blocks of the fused step prologue
   unsigned bar_count;       // grid barrier of the fused PME kernel: blocks arrived
   unsigned bar_gen;         // and its generation (advanced by the last arrival)
+  unsigned call_ticket;     // finished blocks of an integrate call's opening / closing kernel
+  unsigned pad2;
+  unsigned long long ke_acc;  // kinetic-energy partial sums of the opening kernel (0 between calls)
 };
 
 // floor(u) clamped to [0, n) for a cell or grid coordinate u >= 0. The range is decided in
@@ -248,8 +253,13 @@ struct mdx_ctx {
   long long evals = 0;
   // step graphs: a few fixed sizes of forces-only steps plus one energy step, captured once per
   // (terms, integrator kind) and replayed whatever the call's length (mdx_api.hip kGraphSizes)
-  hipGraphExec_t graph_f[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipGraphExec_t graph_f[MDX_NGRAPH_SIZES] = {};
   hipGraphExec_t graph_e = nullptr;
+  // a call's first step with its set-up (energy-history base, kinetic energy) when the last
+  // call's forces are current, and its last step with the closing velocity half, energy record
+  // and wrap: a call of n >= 2 steps is then only graph launches
+  hipGraphExec_t graph_head = nullptr;
+  hipGraphExec_t graph_tail = nullptr;
   unsigned graph_terms = 0;
   int graph_kind = -1;
   int graph_lanes = 0;           // nonbonded lanes per atom the graphs were captured with

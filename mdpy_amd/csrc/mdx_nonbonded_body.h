@@ -74,6 +74,11 @@ static __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+template <bool B>
+struct NbTail {
+  static constexpr bool value = B;
+};
+
 // One workgroup (BS lanes, virtual index vb) of the nonbonded kernel. G lanes share one atom:
 // lane g walks list entries g, g + G, ... of the atom's contiguous (row-major) list, so a group's
 // index loads and its neighbours' position loads are contiguous; the group reduces its partial
@@ -132,24 +137,25 @@ __device__ __forceinline__ void nb_block(const NbArgs& a, const NbPtrs& P, int v
       if (S14) pi14 = ljs14[s];
     }
     const int cnt = nbr_count[s];
-    const unsigned* lp = nbr_list + (long long)s * a.cap;
+    const unsigned* lp = nbr_list + (long long)s * a.cap + g;
+    // neighbours are addressed by 32-bit byte offsets from the array bases (atom indices stay
+    // below 2^28), and the shifts that form them also drop the 1-4 flag in bit 31
+    const char* xqb = reinterpret_cast<const char*>(xq);
+    const char* ljb = reinterpret_cast<const char*>(ljs);
+    const char* lj14b = reinterpret_cast<const char*>(ljs14);
     int oi = -1;
-    // NB_U list entries per lane per pass: all index loads, then all neighbour loads, are
-    // issued before any of them is consumed, so each pass waits for memory once, not 3*NB_U times
-    for (int kb = g; kb < cnt; kb += G * NB_U) {
-      unsigned e[NB_U];
-#pragma unroll
-      for (int u = 0; u < NB_U; ++u) {
-        const int k = kb + u * G;
-        e[u] = k < cnt ? lp[k] : (unsigned)s | 0x40000000u;  // bit 30: padding slot
-      }
+    // one pass over NB_U list entries per lane: all neighbour loads are issued before any is
+    // consumed, so each pass waits for memory once, not 2*NB_U times. In the tail pass (tail =
+    // NbTail<true>) bit u of `valid` marks the slots inside the row; the others hold the atom
+    // itself (zero separation: finite, and zeroed below)
+    auto pass = [&](const unsigned* e, unsigned valid, auto tail) {
+      constexpr bool TAIL = decltype(tail)::value;
       float4 xj[NB_U];
       float2 pj[NB_U];
 #pragma unroll
       for (int u = 0; u < NB_U; ++u) {
-        const int t = (int)(e[u] & 0x3fffffffu);
-        xj[u] = xq[t];
-        if (LJ) pj[u] = ((S14 && (e[u] >> 31)) ? ljs14 : ljs)[t];
+        xj[u] = *reinterpret_cast<const float4*>(xqb + (e[u] << 4));
+        if (LJ) pj[u] = *reinterpret_cast<const float2*>(((S14 && (e[u] >> 31)) ? lj14b : ljb) + (e[u] << 3));
       }
 #pragma unroll
       for (int u = 0; u < NB_U; ++u) {
@@ -157,15 +163,16 @@ __device__ __forceinline__ void nb_block(const NbArgs& a, const NbPtrs& P, int v
         dx -= a.Lx * rintf(dx * a.ix);
         dy -= a.Ly * rintf(dy * a.iy);
         dz -= a.Lz * rintf(dz * a.iz);
-        const float r2raw = dx * dx + dy * dy + dz * dz;
-        bool in = r2raw <= a.rc2 && !(e[u] & 0x40000000u);
-        if (fabsf(r2raw - a.rc2) < a.guard2 && !(e[u] & 0x40000000u)) {
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        bool in = r2 <= a.rc2 && (!TAIL || ((valid >> u) & 1u));
+        if (fabsf(r2 - a.rc2) < a.guard2) {
           // near the cutoff: decide in float64 like the reference (rare)
           if (oi < 0) oi = sorted_orig[s];
-          in = within_cutoff64(pos, oi, sorted_orig[e[u] & 0x3fffffffu], a.Lxd, a.Lyd, a.Lzd, a.rcd);
+          in = within_cutoff64(pos, oi, sorted_orig[e[u] & 0x7fffffffu], a.Lxd, a.Lyd, a.Lzd, a.rcd);
         }
-        const float r2 = in ? r2raw : 1.0f;
-        const float rinv = __builtin_amdgcn_rsqf(r2);  // v_rsq_f32: r2 >= ~0.5 A^2, never denormal
+        // out-of-cutoff and padding slots compute with their own r2 (> rc^2, or 0) and are
+        // zeroed below; in-cutoff pairs have r2 >= ~0.5 A^2, never denormal for v_rsq_f32
+        const float rinv = __builtin_amdgcn_rsqf(r2);
         const float r2inv = rinv * rinv;
         float fr = 0.f;
         if (LJ) {
@@ -182,7 +189,8 @@ __device__ __forceinline__ void nb_block(const NbArgs& a, const NbPtrs& P, int v
           const float zt = r2 * a.ct_scale;
           const float fl = floorf(zt);
           const float t = zt - fl;
-          // clamp in float before the conversion (a NaN distance reads the last piece)
+          // clamp in float before the conversion: beyond the cutoff the index stays inside
+          // the table
           const float4 cf = ctab[(int)fminf(fl, (float)(MDX_CTAB_N - 1))];
           const float gz = fmaf(t, fmaf(t, fmaf(t, cf.w, cf.z), cf.y), cf.x);
           fr -= xi.w * xj[u].w * (rinv * r2inv - gz);
@@ -206,6 +214,24 @@ __device__ __forceinline__ void nb_block(const NbArgs& a, const NbPtrs& P, int v
         fy += fr * dy;
         fz += fr * dz;
       }
+    };
+    // whole chunks of G * NB_U entries: unpredicated index loads
+    int cb = 0;
+    for (; cb + G * NB_U <= cnt; cb += G * NB_U) {
+      unsigned e[NB_U];
+#pragma unroll
+      for (int u = 0; u < NB_U; ++u) e[u] = lp[cb + u * G];
+      pass(e, 0u, NbTail<false>());
+    }
+    if (cb < cnt) {  // the row's last, partial chunk: slots past the end take the atom itself
+      unsigned e[NB_U], valid = 0;
+#pragma unroll
+      for (int u = 0; u < NB_U; ++u) {
+        const bool v = cb + g + u * G < cnt;
+        e[u] = v ? lp[cb + u * G] : (unsigned)s;
+        valid |= (v ? 1u : 0u) << u;
+      }
+      pass(e, valid, NbTail<true>());
     }
   }
   for (int off = G / 2; off > 0; off >>= 1) {

@@ -308,9 +308,16 @@ class Engine:
         self._noise = t
         L.call("mdx_set_external_noise", self.ctx, ctypes.c_void_p(t.data_ptr()), int(t.shape[0]))
 
+    def set_energy_interval(self, every):
+        """Potential energies every `every` steps of integrate (a change re-captures the step
+        graphs, so an unchanged interval is not sent again)."""
+        if getattr(self, "_energy_every", None) != int(every):
+            L.call("mdx_set_energy_interval", self.ctx, int(every))
+            self._energy_every = int(every)
+
     def integrate(self, terms, nsteps, use_graph=True, energy_every=1):
         """Advance nsteps; potential energies are evaluated every `energy_every` steps."""
-        L.call("mdx_set_energy_interval", self.ctx, int(energy_every))
+        self.set_energy_interval(energy_every)
         L.call("mdx_integrate", self.ctx, int(terms), int(nsteps), 1 if use_graph else 0)
 
     def energy_history_async(self, out):

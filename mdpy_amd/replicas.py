@@ -59,7 +59,7 @@ class ReplicaBatch:
             e.set_velocities(maxwell_velocities(s.masses, temperature, seed * 7919 + r))
             e.configure_integrator(L.INT_LANGEVIN, dt=dt, temperature=temperature, friction=friction,
                                    seed=seed * 7919 + r)
-            L.call("mdx_set_energy_interval", e.ctx, self.energy_every)
+            e.set_energy_interval(self.energy_every)
             self.engines.append(e)
         self.skins = skins
         self.n_atoms = int(sum(s.num_atoms for s in self.systems))

@@ -95,10 +95,15 @@ def test_verlet_refcompat_matches_oracle():
 
 
 def test_graph_replay_bitwise_equals_direct_launches():
+    """Calls of 25, 7, 2, 40 and 3 G-JF steps: the first primes its forces (direct first step,
+    step graphs, direct closing kernels); the later ones start from the last call's forces and
+    run as head graph + step graphs + tail graph. Every call's state and energy history equal
+    those of direct launches bit for bit."""
     from mdpy_amd import _lib as L, systems
     from mdpy_amd.engine import Engine
-    say("case: hipGraph vs direct, water648, 25 G-JF steps")
+    say("case: hipGraph vs direct, water648, G-JF calls of 25, 7, 2, 40, 3 steps")
     s = systems.water_box_small()
+    calls = (25, 7, 2, 40, 3)
     out = []
     for graph in (False, True):
         e = Engine(s, skin=1.0)
@@ -106,12 +111,15 @@ def test_graph_replay_bitwise_equals_direct_launches():
         e.set_velocities(thermal_v(s, 300.0, 5))
         e.configure_integrator(L.INT_LANGEVIN, dt=1.0, temperature=300.0, friction=1e-3, seed=7)
         terms = L.TERM_LJ | L.TERM_PME | L.TERM_BOND | L.TERM_ANGLE
-        e.integrate(terms, 25, use_graph=graph)
-        out.append((e.positions(), e.velocities(), e.energy_history()))
+        run = []
+        for n in calls:
+            e.integrate(terms, n, use_graph=graph, energy_every=10)
+            run.append((e.positions(), e.velocities(), e.energy_history()))
+        out.append(run)
         e.close()
-    assert np.array_equal(out[0][0], out[1][0])
-    assert np.array_equal(out[0][1], out[1][1])
-    assert np.array_equal(out[0][2], out[1][2])
+    for k, n in enumerate(calls):
+        for a, b in zip(out[0][k], out[1][k]):
+            assert np.array_equal(a, b), "call %d (%d steps)" % (k, n)
 
 
 def test_nve_energy_conservation_pme_water():
