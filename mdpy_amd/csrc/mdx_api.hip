@@ -112,6 +112,24 @@ __device__ __forceinline__ double total_force(const long long* __restrict__ ffix
   return f;
 }
 
+// total_force for the three components, every load issued before the sums (a branch per
+// component would wait for memory three times); the same sums in the same order
+__device__ __forceinline__ void total_forces_loaded(const long long* __restrict__ ffix,
+                                                    const double* __restrict__ fpart, int n, int i, double* f) {
+  long long q[3];
+  double p[3][4];
+  for (int d = 0; d < 3; ++d) q[d] = ffix[(long long)d * n + i];
+  if (fpart) {
+    const long long n3 = 3LL * n;
+    for (int d = 0; d < 3; ++d)
+      for (int k = 0; k < 4; ++k) p[d][k] = fpart[k * n3 + 3LL * i + d];
+  }
+  for (int d = 0; d < 3; ++d) {
+    f[d] = (double)q[d] * kFixInv;
+    if (fpart) f[d] += (p[d][0] + p[d][1]) + (p[d][2] + p[d][3]);
+  }
+}
+
 __global__ void k_forces_out(const long long* __restrict__ ffix, const double* __restrict__ fpart, int n,
                              double* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -283,17 +301,30 @@ struct StepPro {
 #define PRO_SORT_MAX 1024         // largest cell population the per-cell sort accepts (as k_cell_scan)
 #define PRO_BIN_LDS_MAX (60 * 1024)  // dynamic LDS for last-workgroup binning (2 ints per cell)
 
-__device__ void prologue_atom(const IntArgs& A, const StepPro& P, DevState* st, int i, const double* xn, double* d2,
-                              bool* bad) {
+// An atom's loads for the list bookkeeping, issued with the integrator's loads before any store
+struct ProAtomIn {
+  double pbuild[3];
+  int s;
+  float q;
+};
+__device__ __forceinline__ ProAtomIn prologue_load(const StepPro& P, int i) {
+  ProAtomIn a;
+  for (int d = 0; d < 3; ++d) a.pbuild[d] = P.pos_build[3 * i + d];
+  a.s = P.orig_sorted[i];
+  a.q = P.qs ? P.qs[i] : 0.f;
+  return a;
+}
+
+__device__ void prologue_atom(const IntArgs& A, const StepPro& P, int i, const ProAtomIn& in, const double* xn,
+                              double* d2, bool* bad) {
   for (int d = 0; d < 3; ++d) P.ffix[(long long)d * A.n + i] = 0;
   for (int d = 0; d < 3; ++d) {
     if (!isfinite(xn[d])) *bad = true;
-    double dd = xn[d] - P.pos_build[3 * i + d];
+    double dd = xn[d] - in.pbuild[d];
     dd -= A.L[d] * rint(dd * A.inv[d]);
     *d2 += dd * dd;
   }
-  const int s = P.orig_sorted[i];
-  pack_atom(xn[0], xn[1], xn[2], P.qs ? P.qs[i] : 0.f, A.L[0], A.L[1], A.L[2], &P.xq[s], &P.pos_sorted[3 * s]);
+  pack_atom(xn[0], xn[1], xn[2], in.q, A.L[0], A.L[1], A.L[2], &P.xq[in.s], &P.pos_sorted[3 * in.s]);
   if (P.bin) {  // the list-grid cell, as k_grid_bin computes it
     int c[3];
     for (int d = 0; d < 3; ++d) {
@@ -303,7 +334,6 @@ __device__ void prologue_atom(const IntArgs& A, const StepPro& P, DevState* st, 
     }
     P.atom_cell[i] = (c[0] * P.gn[1] + c[1]) * P.gn[2] + c[2];
   }
-  (void)st;
 }
 
 // rebuild step, last workgroup: count the atoms per cell in LDS, exclusive scan into cell_start,
@@ -417,51 +447,57 @@ __global__ __launch_bounds__(PRO_THREADS) void k_step_pro(IntArgs A, StepPro P, 
   double d2 = 0.0, ke = 0.0;
   bool bad = false;
   if (i < A.n) {
-    double vv[3], xn[3], f[3];
+    // every load of the atom first: the stores below cannot then hold them back (the arrays
+    // are distinct, but the compiler cannot know it), so the workgroup waits for memory once
+    double vv[3], xn[3], f[3], xo[3], ap[3], gs[3];
     for (int d = 0; d < 3; ++d) {
-      f[d] = fix_force(P.ffix, A, i, d);
       vv[d] = v[3 * i + d];
+      xo[d] = x[3 * i + d];
+      ap[d] = GJF ? acc_prev[3 * i + d] : 0.0;
+      gs[d] = GJF && FUSE_B ? A.gsave[3 * i + d] : 0.0;
     }
+    const double imi = im[i];
+    const double mi = FUSE_B ? m[i] : 0.0;
+    const ProAtomIn pin = prologue_load(P, i);
+    total_forces_loaded(P.ffix, A.fpart, A.n, i, f);  // last: its sums wait for all of these
     if (FUSE_B) {
       if (GJF) {
-        double g[3];  // xi_{n+1}: drawn by the previous step's position half
-        for (int d = 0; d < 3; ++d) g[d] = A.gsave[3 * i + d];
-        const double sm = sqrt(im[i]);
+        const double sm = sqrt(imi);  // gs: xi_{n+1}, drawn by the previous step's position half
         for (int d = 0; d < 3; ++d) {
-          const double acc = f[d] * im[i];
-          vv[d] = A.a * vv[d] + 0.5 * A.dt * (A.a * acc_prev[3 * i + d] + acc) + A.b * A.sigma * sqrt(A.dt) * g[d] * sm;
+          const double acc = f[d] * imi;
+          vv[d] = A.a * vv[d] + 0.5 * A.dt * (A.a * ap[d] + acc) + A.b * A.sigma * sqrt(A.dt) * gs[d] * sm;
           ke += vv[d] * vv[d];
         }
       } else {
         for (int d = 0; d < 3; ++d) {
-          vv[d] = vv[d] + 0.5 * A.dt * f[d] * im[i];
+          vv[d] = vv[d] + 0.5 * A.dt * f[d] * imi;
           ke += vv[d] * vv[d];
         }
       }
-      ke *= 0.5 * m[i];
+      ke *= 0.5 * mi;
     }
     if (GJF) {
       double g[3];
       noise3(A.ns, st->step + 1, i, A.n, g);
       for (int d = 0; d < 3; ++d) A.gsave[3 * i + d] = g[d];
-      const double sm = sqrt(im[i]);
+      const double sm = sqrt(imi);
       for (int d = 0; d < 3; ++d) {
-        const double acc = f[d] * im[i];
+        const double acc = f[d] * imi;
         acc_prev[3 * i + d] = acc;
-        xn[d] = x[3 * i + d] + A.b * A.dt * vv[d] + A.b * A.dt * A.dt * 0.5 * acc +
+        xn[d] = xo[d] + A.b * A.dt * vv[d] + A.b * A.dt * A.dt * 0.5 * acc +
                 A.b * A.sigma * A.dt * sqrt(A.dt) * 0.5 * g[d] * sm;
         x[3 * i + d] = xn[d];
         if (FUSE_B) v[3 * i + d] = vv[d];
       }
     } else {
       for (int d = 0; d < 3; ++d) {
-        const double vh = vv[d] + 0.5 * A.dt * f[d] * im[i];
+        const double vh = vv[d] + 0.5 * A.dt * f[d] * imi;
         v[3 * i + d] = vh;
-        xn[d] = x[3 * i + d] + A.dt * vh;
+        xn[d] = xo[d] + A.dt * vh;
         x[3 * i + d] = xn[d];
       }
     }
-    prologue_atom(A, P, st, i, xn, &d2, &bad);
+    prologue_atom(A, P, i, pin, xn, &d2, &bad);
   }
   if (FUSE_B) block_add_energy(ke, ke_slot);  // before this block's ticket: the last block records it
   prologue_block_end(P, st, d2, bad);
