@@ -187,6 +187,37 @@ def replicas_leg(steps=200):
     return out
 
 
+PEAK_HBM_GBPS = 8000.0
+
+
+def secondary_rooflines(e, s, terms, slots):
+    """HBM rooflines of the step's other kernels, each timed alone with HIP events on the library
+    stream (mdx_time_kernel): achieved = the kernel's algorithmic bytes / its launch time."""
+    n = s.num_atoms
+    kx, ky, kz = (int(k) for k in s.pme_grid)
+    k3 = kx * ky * kz
+    items = [
+        ("k_pme_plane_fwd", 6, 50, 16 * n + 4 * k3,
+         "spreading: 16 B/atom (packed position + charge) + 4 B per grid point"),
+        ("k_fft_axis", 7, 50, 2 * (4 * k3 + 8 * ky * kz * (kx // 2 + 1)),
+         "x-axis transform pair + B.C multiply: 2 x (4 K^3 + 8 K^2 (K/2+1)) bytes"),
+        ("k_pme_plane_inv_gather", 8, 50, 40 * n + 4 * k3,
+         "gather: 40 B/atom (position, charge, force out) + 4 B per grid point"),
+        ("k_step_pro", 4, 50, 424 * n,
+         "424 B/atom: reads forces 24 + reciprocal planes 96 + x, v, a_prev, xi, build position 5x24 + "
+         "masses 16 + ids 8; writes x, v, a_prev, xi, zeroed forces 5x24 + packed copies 40"),
+        ("k_build_list_fast", 5, 10, 4 * slots + 24 * n,
+         "rebuild step: 4 B per list slot written + 24 B/atom of cell data read"),
+    ]
+    out = {}
+    for name, which, reps, nbytes, rule in items:
+        ms = e.time_kernel(which, terms, reps=reps)
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out[name] = {"bound": "hbm", "kernel_ms": ms, "bytes": nbytes, "achieved": gbs, "peak": PEAK_HBM_GBPS,
+                     "unit": "GB/s", "frac": gbs / PEAK_HBM_GBPS, "bytes_rule": rule}
+    return out
+
+
 def progress(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
@@ -386,6 +417,7 @@ def main():
                      "kernel_ms": nb_ms, "unique_pairs": pairs, "list_slots": slots,
                      "pairs_per_s": pairs / (nb_ms * 1e-3), "flops_per_pair": FLOPS_PER_PAIR},
         "breakdown_ms": {"force_eval": eval_ms, "pme_reciprocal": pme_ms, "nonbonded": nb_ms},
+        "secondary_rooflines": secondary_rooflines(e, s, terms, slots),
         "list_rebuilds": stats["rebuilds"],
         "list_rebuilds_timed": rebuilds_timed,
         "energies_finite": finite,

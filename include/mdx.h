@@ -272,7 +272,10 @@ int mdx_mlp_gemm_check(mdx_mlp* mlp, int M, int N, int K, const void* d_A, const
 int mdx_stats_fetch(mdx_ctx* ctx, int64_t* h_rebuilds, int64_t* h_evals);
 /* Average duration (ms) of one launch of a hot kernel, timed with HIP events on the context's
  * stream: which 0 = nonbonded kernel (current list, forces only), 1 = force evaluation of
- * `terms`, 2 = PME reciprocal chain, 3 = one integrator step of `terms`. Synchronises. */
+ * `terms`, 2 = PME reciprocal chain, 3 = one integrator step of `terms`, 4 = the fused step
+ * prologue alone (state saved and restored), 5 = the neighbour-list build of a rebuild step (the
+ * next evaluation rebuilds in full), 6 / 7 / 8 = the PME plane forward kernel / x-axis pass /
+ * inverse + gather kernel alone. Synchronises. */
 int mdx_time_kernel(mdx_ctx* ctx, int which, unsigned terms, int reps, double* h_avg_ms);
 /* Lanes per atom of the nonbonded kernel: 16, 32 or 64 set directly, or 0 = time each variant
  * on the current neighbour list (`reps` forces-only launches of `terms`' LJ/direct parts) and

@@ -2059,11 +2059,103 @@ extern "C" int mdx_nonbonded_persist(mdx_ctx* c, int per_cu) {
   return 0;
 }
 
+// The step prologue (G-JF or velocity Verlet, folding the previous velocity half) timed alone:
+// the state it writes is saved first and put back after, no rebuild is decided and nothing is
+// recorded, so the trajectory continues as if it had not run.
+static int time_prologue(mdx_ctx* c, int reps, double* h_avg_ms) {
+  MDX_CHECK(c->int_configured && c->acc_prev.p && c->gsave.p && c->xq.p && c->pos_sorted.p &&
+                (c->int_kind == MDX_INT_LANGEVIN || c->int_kind == MDX_INT_VELOCITY_VERLET),
+            MDX_ERR_INVALID, "prologue timing needs a G-JF or velocity-Verlet context with a list grid");
+  const size_t n3 = 3 * (size_t)c->n;
+  struct Part {
+    void* p;
+    size_t bytes;
+  } parts[] = {{c->pos, n3 * 8}, {c->vel, n3 * 8}, {c->acc_prev.p, n3 * 8}, {c->gsave.p, n3 * 8},
+               {c->ffix.p, n3 * 8}, {c->efix.p, (MDX_NTERMS + 1) * 8}, {c->dstate.p, sizeof(DevState)},
+               {c->xq.p, (size_t)c->n * sizeof(float4)}, {c->pos_sorted.p, n3 * 8}};
+  size_t total = 0;
+  for (const Part& q : parts) total += (q.bytes + 255) & ~(size_t)255;
+  struct Save {  // freed on every return (hipFree waits for the device)
+    DevBuf<char> b;
+    ~Save() { b.release(); }
+  } keep;
+  DevBuf<char>& save = keep.b;
+  int rc = save.ensure(total);
+  if (rc) return rc;
+  size_t off = 0;
+  for (const Part& q : parts) {
+    MDX_HIP(hipMemcpyAsync(save.p + off, q.p, q.bytes, hipMemcpyDeviceToDevice, c->stream));
+    off += (q.bytes + 255) & ~(size_t)255;
+  }
+  IntArgs A = int_args(c);
+  StepPro P = step_pro(c);
+  P.bin = 0;
+  P.record = 0;
+  P.thr2 = HUGE_VAL;
+  long long* ke = c->efix.p + MDX_E_KINETIC;
+  const bool gjf = c->int_kind == MDX_INT_LANGEVIN;
+  dim3 gp(div_up(c->n, PRO_THREADS)), bp(PRO_THREADS);
+  hipEvent_t e0, e1;
+  MDX_HIP(hipEventCreate(&e0));
+  MDX_HIP(hipEventCreate(&e1));
+  MDX_HIP(hipEventRecord(e0, c->stream));
+  for (int k = 0; k < reps; ++k) {
+    if (gjf)
+      k_step_pro<true, true><<<gp, bp, 0, c->stream>>>(A, P, c->pos, c->vel, c->inv_mass.p, c->mass.p, c->acc_prev.p, ke,
+                                                       c->dstate.p);
+    else
+      k_step_pro<false, true><<<gp, bp, 0, c->stream>>>(A, P, c->pos, c->vel, c->inv_mass.p, c->mass.p, c->acc_prev.p,
+                                                        ke, c->dstate.p);
+  }
+  MDX_LAUNCH_CHECK();
+  MDX_HIP(hipEventRecord(e1, c->stream));
+  off = 0;
+  for (const Part& q : parts) {
+    MDX_HIP(hipMemcpyAsync(q.p, save.p + off, q.bytes, hipMemcpyDeviceToDevice, c->stream));
+    off += (q.bytes + 255) & ~(size_t)255;
+  }
+  MDX_HIP(hipStreamSynchronize(c->stream));
+  float ms = 0;
+  MDX_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *h_avg_ms = ms / reps;
+  return 0;
+}
+
+// The neighbour-list build of a rebuild step timed alone: each repetition requests a rebuild and
+// builds from the current cell order; the next evaluation rebuilds in full (binning included).
+static int time_list_build(mdx_ctx* c, int reps, double* h_avg_ms) {
+  MDX_CHECK(c->nb_configured && c->list_valid, MDX_ERR_INVALID, "list-build timing needs a built neighbour list");
+  hipEvent_t e0, e1;
+  MDX_HIP(hipEventCreate(&e0));
+  MDX_HIP(hipEventCreate(&e1));
+  double total = 0;
+  int rc = 0;
+  for (int k = 0; k < reps && !rc; ++k) {
+    if ((rc = mdx_enqueue_request_rebuild(c))) break;
+    MDX_HIP(hipEventRecord(e0, c->stream));
+    if ((rc = mdx_nlist_update_list(c))) break;
+    MDX_HIP(hipEventRecord(e1, c->stream));
+    MDX_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    MDX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    total += ms;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!rc) rc = mdx_enqueue_request_rebuild(c);
+  *h_avg_ms = total / reps;
+  return rc;
+}
+
 extern "C" int mdx_time_kernel(mdx_ctx* c, int which, unsigned terms, int reps, double* h_avg_ms) {
   MDX_CHECK(c && h_avg_ms && reps > 0, MDX_ERR_INVALID, "mdx_time_kernel: bad arguments");
   MDX_ENTER();
   c->forces_current = false;  // the timed launches accumulate into the force buffers
   int rc = 0;
+  if (which == 4) MDX_LEAVE_RET(time_prologue(c, reps, h_avg_ms));
+  if (which == 5) MDX_LEAVE_RET(time_list_build(c, reps, h_avg_ms));
   bool lj = (terms & MDX_TERM_LJ) != 0, cd = (terms & MDX_TERM_COUL_DIRECT) != 0;
   if (which == 0) {  // make sure the list is current
     MDX_CHECK(c->nb_configured && (lj || cd), MDX_ERR_INVALID, "nonbonded timing needs LJ or direct Coulomb");
@@ -2080,6 +2172,7 @@ extern "C" int mdx_time_kernel(mdx_ctx* c, int which, unsigned terms, int reps, 
       if (!rc) rc = mdx_enqueue_forces(c, terms, true);
     } else if (which == 2) rc = mdx_pme_enqueue(c, true);
     else if (which == 3) rc = mdx_integrator_enqueue_step(c, terms, false, false, true);
+    else if (which >= 6 && which <= 8) rc = mdx_pme_enqueue_part(c, which - 6);
     else rc = MDX_ERR_INVALID;
   }
   MDX_HIP(hipEventRecord(e1, c->stream));

@@ -433,6 +433,7 @@ int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy, unsigned 
 int mdx_coulomb_table_update(mdx_ctx* c);
 int mdx_enqueue_coulomb_plain(mdx_ctx* c);
 int mdx_pme_enqueue(mdx_ctx* c, bool energy);
+int mdx_pme_enqueue_part(mdx_ctx* c, int part);  // one plane-path launch alone (timing)
 int mdx_bonded_enqueue(mdx_ctx* c, unsigned terms);
 int mdx_enqueue_zero_accumulators(mdx_ctx* c);
 int mdx_enqueue_forces_out(mdx_ctx* c, double* d_out);

@@ -1074,7 +1074,9 @@ static int fft_pass(mdx_ctx* c, int axis, int dir, int mode) {
 // plane path: spread + 2D forward FFT per x-plane, x-axis pass with the influence function,
 // 2D inverse FFT + gather per x-plane. Needs the cell-sorted atom order of the neighbour-list
 // grid (current for this evaluation) and two complex planes in LDS.
-static int pme_enqueue_planes(mdx_ctx* c) {
+// parts: bit 0 the plane forward kernel, bit 1 the x-axis transform pass, bit 2 the inverse +
+// gather kernel (all three for an evaluation; one at a time for timing)
+static int pme_enqueue_planes(mdx_ctx* c, unsigned parts = 7u) {
   const int Kx = c->K[0], Ky = c->K[1], Kz = c->K[2];
   PlaneArgs a;
   a.n = c->n;
@@ -1101,7 +1103,7 @@ static int pme_enqueue_planes(mdx_ctx* c) {
   const int Kmax = std::max(Kx, std::max(Ky, Kz));
   const float2* twy = c->twiddle.p + (size_t)Kmax;
   const float2* twz = c->twiddle.p + 2 * (size_t)Kmax;
-  if (c->pme_fused_B > 0 && c->pme_fused_on && !c->in_batch) {
+  if (parts == 7u && c->pme_fused_B > 0 && c->pme_fused_on && !c->in_batch) {
     FftPlan plx;
     factorize(Kx, &plx);
     FftLines ln;
@@ -1119,17 +1121,29 @@ static int pme_enqueue_planes(mdx_ctx* c) {
     return 0;
   }
   a.W = c->pme_wf;
-  k_pme_plane_fwd<<<Kx, PLANE_THREADS, plane_fwd_lds(Ky, Kz, a.W), c->stream>>>(
-      a, plz, ply, twz, twy, c->sorted_orig.p, c->cell_start.p, c->pos_sorted.p, c->xq.p, c->cgrid.p, grec,
-      c->pme_grec_n.p, c->dstate.p);
-  MDX_LAUNCH_CHECK();
+  if (parts & 1u) {
+    k_pme_plane_fwd<<<Kx, PLANE_THREADS, plane_fwd_lds(Ky, Kz, a.W), c->stream>>>(
+        a, plz, ply, twz, twy, c->sorted_orig.p, c->cell_start.p, c->pos_sorted.p, c->xq.p, c->cgrid.p, grec,
+        c->pme_grec_n.p, c->dstate.p);
+    MDX_LAUNCH_CHECK();
+  }
   int rc;
-  if ((rc = fft_pass(c, 0, -1, 3))) return rc;  // x forward, * B C, energy, x inverse
-  k_pme_plane_inv_gather<<<Kx, PLANE_THREADS, plane_gather_lds(Ky, Kz), c->stream>>>(
-      a, plz, ply, twz, twy, c->sorted_orig.p, c->cell_start.p, c->pos_sorted.p, c->xq.p, c->cgrid.p, grec,
-      c->pme_grec_n.p, c->fpart.p, c->dstate.p);
-  MDX_LAUNCH_CHECK();
+  if ((parts & 2u) && (rc = fft_pass(c, 0, -1, 3))) return rc;  // x forward, * B C, energy, x inverse
+  if (parts & 4u) {
+    k_pme_plane_inv_gather<<<Kx, PLANE_THREADS, plane_gather_lds(Ky, Kz), c->stream>>>(
+        a, plz, ply, twz, twy, c->sorted_orig.p, c->cell_start.p, c->pos_sorted.p, c->xq.p, c->cgrid.p, grec,
+        c->pme_grec_n.p, c->fpart.p, c->dstate.p);
+    MDX_LAUNCH_CHECK();
+  }
   return 0;
+}
+
+// One of the plane path's three launches alone (part 0, 1 or 2 of pme_enqueue_planes), for
+// timing: each reads only what the previous evaluation's launches left in place.
+int mdx_pme_enqueue_part(mdx_ctx* c, int part) {
+  MDX_CHECK(c->pme_configured && c->pme_planes && c->nb_configured && part >= 0 && part <= 2, MDX_ERR_INVALID,
+            "PME part timing needs the plane path");
+  return pme_enqueue_planes(c, 1u << part);
 }
 
 bool mdx_force_phases_ok(const mdx_ctx* c) {

@@ -275,7 +275,9 @@ class Engine:
 
     def time_kernel(self, which, terms, reps=20):
         """Average ms per launch (HIP events on the library stream): which 0 nonbonded,
-        1 force evaluation, 2 PME reciprocal chain, 3 integrator step."""
+        1 force evaluation, 2 PME reciprocal chain, 3 integrator step, 4 step prologue alone
+        (state restored), 5 list build of a rebuild step, 6 / 7 / 8 PME plane forward / x-axis
+        pass / inverse + gather alone."""
         ms = ctypes.c_double()
         L.call("mdx_time_kernel", self.ctx, int(which), int(terms), int(reps), ctypes.byref(ms))
         return ms.value

@@ -122,6 +122,37 @@ def test_graph_replay_bitwise_equals_direct_launches():
             assert np.array_equal(a, b), "call %d (%d steps)" % (k, n)
 
 
+def test_kernel_timing_leaves_the_trajectory_unchanged():
+    """Timing the step prologue (state saved and restored) and the three PME plane launches
+    between two integrate calls changes nothing: the next call starts from a fresh evaluation at
+    the same positions and follows the untimed trajectory bit for bit. Timing the list build
+    (which forces a full rebuild on the next evaluation) leaves a valid, finite run."""
+    from mdpy_amd import _lib as L, systems
+    from mdpy_amd.engine import Engine
+    say("case: kernel timing modes between integrate calls, water648, G-JF")
+    s = systems.water_box_small()
+    terms = L.TERM_LJ | L.TERM_PME | L.TERM_BOND | L.TERM_ANGLE
+    out = []
+    for timed in (False, True):
+        e = Engine(s, skin=1.0)
+        e.set_positions(s.positions)
+        e.set_velocities(thermal_v(s, 300.0, 11))
+        e.configure_integrator(L.INT_LANGEVIN, dt=1.0, temperature=300.0, friction=1e-3, seed=3)
+        e.integrate(terms, 12, use_graph=True, energy_every=4)
+        if timed:
+            for which in (4, 6, 7, 8):
+                assert e.time_kernel(which, terms, reps=5) > 0
+        e.integrate(terms, 12, use_graph=True, energy_every=4)
+        out.append((e.positions(), e.velocities(), e.energy_history()))
+        if timed:
+            assert e.time_kernel(5, terms, reps=3) > 0
+            e.integrate(terms, 12, use_graph=True, energy_every=4)
+            assert e.status() == 0 and np.all(np.isfinite(e.positions()))
+        e.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
 def test_nve_energy_conservation_pme_water():
     from mdpy_amd import _lib as L, systems
     from mdpy_amd.engine import Engine

@@ -11,8 +11,15 @@ from conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
-FORCE_RTOL = 1e-5      # relative force RMS, float32 pair/PME arithmetic vs float64 oracle
+FORCE_RTOL = 1e-5      # relative force RMS, float32 pair/PME arithmetic vs float64 oracle, per term
+                       # and in total (measured per term: LJ 3.4e-6 / 8.2e-6 / 6.0e-6, direct
+                       # Coulomb 0.9e-6 / 2.0e-6 / 2.7e-6 at water648 / water23328 / chain)
 ENERGY_RTOL = 1e-5     # relative energy error per term (large terms)
+# The direct-space Coulomb energy is a sum of float32 pair energies that cancel: |E_direct| is
+# 4.6e-3 (water648) and 1.2e-4 (water23328, chain) of the sum of |pair energies|. Measured errors:
+# 0.9e-6, 2.0e-6 and 1.55e-5 |E_direct| (on the chain, with its +-0.6 e charges, 1.9e-9 of the
+# sum of |pair energies|); the chain's bound is therefore 2e-5, the others 1e-5.
+DIRECT_ENERGY_RTOL = {"water648": 1e-5, "water23328": 1e-5, "chain": 2e-5}
 
 
 def say(*a):
@@ -157,8 +164,8 @@ def oracle_terms(s, xw, pairs, beta):
     # sum of |pair energy| (the same sum over |q|): float32 pair arithmetic errs per pair, so the
     # error of a total that cancels to a small value scales with this, not with the total
     _, Ed_abs = EW.direct_terms(xw, np.abs(s.charges), pairs, s.box, beta, ONE_OVER_4PI_EPS0)
-    out["_scale"] = {"coulomb_reciprocal": abs(Er) + abs(Ex) + abs(Es) + abs(Eb),
-                     "coulomb_direct": max(abs(Ed), 1e-2 * Ed_abs)}
+    out["_scale"] = {"coulomb_reciprocal": abs(Er) + abs(Ex) + abs(Es) + abs(Eb)}
+    out["_direct_abs"] = Ed_abs
     out["bond"] = R.bond_terms(xw, s.bonds, s.bond_params, s.box)
     out["angle"] = R.angle_terms(xw, s.angles, s.angle_params, s.box)
     out["dihedral"] = R.dihedral_terms(xw, s.dihedrals, s.dihedral_params, s.box)
@@ -183,6 +190,7 @@ def test_forces_match_oracle(name, request):
     pairs = R.exact_pairs(xw, s.box, s.cutoff, ep, ei)
     ref = oracle_terms(s, xw, pairs, e.beta)
     ref_scale = ref.pop("_scale")
+    direct_abs = ref.pop("_direct_abs")
     Ftot_ref = np.zeros_like(xw)
     Etot_ref = 0.0
     for term, bit in TERM_BITS.items():
@@ -201,12 +209,15 @@ def test_forces_match_oracle(name, request):
         err = rel_rms(F, Fref)
         say("  %-20s rel force RMS %.3e  E %.10e ref %.10e" % (term, err, en[term], Eref))
         # float64 bonded kernels: only the int64 fixed-point quantum (2^-40) separates them
-        tol = 1e-8 if term in ("bond", "angle", "dihedral", "improper") else 3 * FORCE_RTOL
+        tol = 1e-8 if term in ("bond", "angle", "dihedral", "improper") else FORCE_RTOL
         assert err < tol, (term, err)
-        # energies: relative to the magnitude of the parts that cancel in the sum (the reciprocal
-        # term is grid energy + self + exclusion correction, each much larger than the total)
+        # energies: relative to |E| of the term, except the reciprocal term, which is grid energy
+        # + self + exclusion correction, each much larger than their sum
         scale = ref_scale[term] if term in ref_scale else abs(Eref)
-        assert abs(en[term] - Eref) <= max(ENERGY_RTOL * scale, 1e-9), (term, en[term], Eref, scale)
+        rtol = DIRECT_ENERGY_RTOL[name] if term == "coulomb_direct" else ENERGY_RTOL
+        if term == "coulomb_direct":
+            say("  direct energy: |E| / sum |pair E| = %.4f" % (abs(Eref) / direct_abs))
+        assert abs(en[term] - Eref) <= max(rtol * scale, 1e-9), (term, en[term], Eref, scale)
     F, en = e.compute(sum(TERM_BITS.values()))
     err = rel_rms(F.cpu().numpy(), Ftot_ref)
     say("  total rel force RMS %.3e" % err)
@@ -254,13 +265,11 @@ def test_nonbonded_variants_match_oracle(name, request):
     pairs = R.exact_pairs(xw, s.box, s.cutoff, ep, ei)
     Flj, Elj = R.lj_pair_terms(xw, s.lj, pairs, s.box, sp, si)
     Fd, Ed = EW.direct_terms(xw, s.charges, pairs, s.box, e.beta, ONE_OVER_4PI_EPS0)
-    _, Ed_abs = EW.direct_terms(xw, np.abs(s.charges), pairs, s.box, e.beta, ONE_OVER_4PI_EPS0)
     for lanes in (16, 32, 64):
         say("case: nonbonded variant", lanes, name)
         assert e.tune_nonbonded(L.TERM_LJ | L.TERM_COUL_DIRECT, lanes) == lanes
-        for bit, key, Fref, Eref, scale in ((L.TERM_LJ, "lj", Flj, Elj, abs(Elj)),
-                                           (L.TERM_COUL_DIRECT, "coulomb_direct", Fd, Ed,
-                                            max(abs(Ed), 1e-2 * Ed_abs))):
+        for bit, key, Fref, Eref, rtol in ((L.TERM_LJ, "lj", Flj, Elj, ENERGY_RTOL),
+                                          (L.TERM_COUL_DIRECT, "coulomb_direct", Fd, Ed, DIRECT_ENERGY_RTOL[name])):
             F1, en1 = e.compute(bit)
             F1 = F1.cpu().numpy().copy()
             F2, en2 = e.compute(bit)
@@ -271,8 +280,8 @@ def test_nonbonded_variants_match_oracle(name, request):
             err3 = rel_rms(F3, Fref)
             say("  %-16s rel force RMS %.3e (forces-only launch %.3e)  E %.10e ref %.10e"
                 % (key, err, err3, en1[key], Eref))
-            assert err < 3 * FORCE_RTOL and err3 < 3 * FORCE_RTOL, (lanes, key, err, err3)
-            assert abs(en1[key] - Eref) <= max(ENERGY_RTOL * scale, 1e-9), (lanes, key, en1[key], Eref)
+            assert err < FORCE_RTOL and err3 < FORCE_RTOL, (lanes, key, err, err3)
+            assert abs(en1[key] - Eref) <= max(rtol * abs(Eref), 1e-9), (lanes, key, en1[key], Eref)
         assert e.status() == 0
     e.close()
 
