@@ -77,6 +77,68 @@ class Topology:
         self._particles[p1].add_scaling_particle(p4, scaling_factor)
         self._particles[p4].add_scaling_particle(p1, scaling_factor)
 
+    def del_particles(self, particles):
+        """Remove the given particles (matched by identity) and every bond, angle, dihedral and
+        improper that names one of them (topology.py:101-120). The remaining particles keep
+        their matrix ids, as the remaining terms refer to them."""
+        self._check_joined()
+        gone = [p for p in particles if any(p is q for q in self._particles)]
+        ids = {p.matrix_id for p in gone}
+        self._particles = [q for q in self._particles if not any(q is p for p in gone)]
+        self._bonds = [t for t in self._bonds if not ids.intersection(t)]
+        self._angles = [t for t in self._angles if not ids.intersection(t)]
+        self._dihedrals = [t for t in self._dihedrals if not ids.intersection(t)]
+        self._impropers = [t for t in self._impropers if not ids.intersection(t)]
+
+    def _remove_term(self, terms, term, reversible=True):
+        """Remove `term` (or, when reversible, its reversed order) from `terms`; True if found."""
+        for candidate in ([list(term), list(term)[::-1]] if reversible else [list(term)]):
+            if candidate in terms:
+                terms.remove(candidate)
+                return True
+        return False
+
+    def del_bond(self, bond):
+        """topology.py:136-153: the bond in either order; its ends leave each other's lists."""
+        self._check_joined()
+        if len(bond) != 2:
+            raise GeomtryDimError("a bond joins 2 atoms; %d indices were given" % len(bond))
+        p1, p2 = bond
+        self._check_matrix_ids(p1, p2)
+        if self._remove_term(self._bonds, bond):
+            self._particles[p1].del_bonded_particle(p2)
+            self._particles[p2].del_bonded_particle(p1)
+
+    def del_angle(self, angle):
+        """topology.py:169-186: the angle as given or reversed."""
+        self._check_joined()
+        if len(angle) != 3:
+            raise GeomtryDimError("an angle spans 3 atoms; %d indices were given" % len(angle))
+        p1, p2, p3 = angle
+        self._check_matrix_ids(p1, p2, p3)
+        if self._remove_term(self._angles, angle):
+            self._particles[p1].del_bonded_particle(p3)
+            self._particles[p3].del_bonded_particle(p1)
+
+    def del_dihedral(self, dihedral):
+        """topology.py:202-219: the dihedral as given or reversed; its ends lose their 1-4 link."""
+        self._check_joined()
+        if len(dihedral) != 4:
+            raise GeomtryDimError("a dihedral spans 4 atoms; %d indices were given" % len(dihedral))
+        p1, p2, p3, p4 = dihedral
+        self._check_matrix_ids(p1, p2, p3, p4)
+        if self._remove_term(self._dihedrals, dihedral):
+            self._particles[p1].del_scaling_particle(p4)
+            self._particles[p4].del_scaling_particle(p1)
+
+    def del_improper(self, improper):
+        """topology.py:232-241: the improper exactly as given."""
+        self._check_joined()
+        if len(improper) != 4:
+            raise GeomtryDimError("an improper spans 4 atoms; %d indices were given" % len(improper))
+        self._check_matrix_ids(*improper)
+        self._remove_term(self._impropers, improper, reversible=False)
+
     def add_improper(self, improper):
         self._check_joined()
         if len(improper) != 4:

@@ -377,6 +377,44 @@ def improper_terms(positions, impropers, params, box):
     return F, float(np.sum(k * (psi - psi0) ** 2))
 
 
+# --------------------------------------------------------------------------- topology tables
+class TopologyConflict(Exception):
+    """mdpy/error.py ParticleConflictError, as the topology rules below raise it."""
+
+
+def topology_tables(n, bonds, angles, dihedrals):
+    """Per-atom bonded (1-2, 1-3) and scaling (1-4) partner lists in insertion order, as
+    Topology.add_bond / add_angle / add_dihedral build them through the particles
+    (mdpy/core/topology.py:122-133,155-166,188-199; mdpy/core/particle.py:53-80): a bond adds
+    both ends to each other's bonded list, an angle its two ends, a dihedral its two ends to the
+    scaling lists; a bonded partner already present or the atom itself raises, a repeated
+    scaling partner is skipped."""
+    bonded = [[] for _ in range(n)]
+    scaling = [[] for _ in range(n)]
+
+    def add_bonded(i, j):
+        if j in bonded[i] or j == i:
+            raise TopologyConflict("bonded partner %d of %d" % (j, i))
+        bonded[i].append(j)
+
+    def add_scaling(i, j):
+        if j == i:
+            raise TopologyConflict("scaling partner %d of itself" % i)
+        if j not in scaling[i]:
+            scaling[i].append(j)
+
+    for a, b in bonds:
+        add_bonded(int(a), int(b))
+        add_bonded(int(b), int(a))
+    for a, _, c in angles:
+        add_bonded(int(a), int(c))
+        add_bonded(int(c), int(a))
+    for d in dihedrals:
+        add_scaling(int(d[0]), int(d[3]))
+        add_scaling(int(d[3]), int(d[0]))
+    return bonded, scaling
+
+
 def kinetic_energy(velocities, masses):
     """mdpy/ensemble.py:63-71: sum(m v^2)/2 (Da A^2/fs^2 is the internal energy unit)."""
     return float(np.sum(np.sum(velocities ** 2, axis=1) * np.asarray(masses).reshape(-1)) / 2)

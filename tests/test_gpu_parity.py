@@ -390,3 +390,71 @@ def test_dihedral_refcompat(chain):
     assert rel_rms(F.cpu().numpy(), Fref) < 1e-9   # int64 fixed-point quantum
     assert en["dihedral"] == pytest.approx(Eref, rel=1e-9)
     e.close()
+
+
+@pytest.mark.parametrize("name", ["water23328", "chain"])
+def test_ensemble_through_topology_matches_oracle(name, request):
+    """The mdpy-shaped path end to end: Topology.from_system -> Ensemble with the CHARMM and PME
+    constraints (type-keyed parameter tables, as a force field would give them) -> update(),
+    against the float64 oracle whose exclusion and 1-4 tables come from its own restatement of
+    the topology rules (oracle.mdref.topology_tables), not from the System."""
+    from mdpy_amd import Ensemble
+    from mdpy_amd.constraint import (CharmmAngleConstraint, CharmmBondConstraint, CharmmDihedralConstraint,
+                                     CharmmImproperConstraint, CharmmNonbondedConstraint,
+                                     ElectrostaticPMEConstraint)
+    from mdpy_amd.core import Topology
+    from mdpy_amd.unit import ONE_OVER_4PI_EPS0
+    from oracle import ewald as EW, mdref as R
+    s = request.getfixturevalue(name)
+    say("case: Ensemble through Topology vs oracle", name)
+    ty = list(s.types)
+    key = lambda ids: "-".join(ty[int(i)] for i in ids)
+    nb = {}
+    for i, t in enumerate(ty):
+        nb.setdefault(t, list(s.lj[i]))
+    bonds = {key(b): list(p) for b, p in zip(s.bonds, s.bond_params)}
+    bonds.update({key(b[::-1]): list(p) for b, p in zip(s.bonds, s.bond_params)})
+    angles = {key(a): list(p) for a, p in zip(s.angles, s.angle_params)}
+    dihedrals = {}
+    for d, p in zip(s.dihedrals, s.dihedral_params):
+        dihedrals.setdefault(key(d), []).append(list(p))
+    impropers = {key(m): list(p) for m, p in zip(s.impropers, s.improper_params)}
+    top = Topology.from_system(s)
+    ens = Ensemble(top, s.pbc_matrix)
+    ens.state.set_positions(s.positions)
+    cons = [CharmmNonbondedConstraint(nb, cutoff_radius=s.cutoff),
+            ElectrostaticPMEConstraint(cutoff_radius=s.cutoff, grid=s.pme_grid),
+            CharmmBondConstraint(bonds), CharmmAngleConstraint(angles)]
+    if len(s.dihedrals):
+        cons.append(CharmmDihedralConstraint(dihedrals))
+    if len(s.impropers):
+        cons.append(CharmmImproperConstraint(impropers))
+    ens.add_constraints(*cons)
+    ens.update()
+    F = np.asarray(ens.forces, dtype=np.float64)
+    pme = cons[1]
+    # oracle, with its own topology tables
+    quads = sorted({tuple(int(v) for v in d) for d in s.dihedrals})
+    bonded, scaling = R.topology_tables(s.num_atoms, s.bonds, s.angles, quads)
+    csr = lambda rows: (np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int64),
+                        np.array([j for r in rows for j in sorted(r)], dtype=np.int64))
+    ep, ei = csr(bonded)
+    sp, si = csr(scaling)
+    # the mdpy-shaped State and Topology hold positions and charges at the environment's precision
+    # (float32 by default, as the reference's env): the oracle takes the same rounded values
+    x32 = np.asarray(s.positions, dtype=np.float32).astype(np.float64)
+    xw = R.wrap_positions(x32, s.pbc_matrix, np.linalg.inv(s.pbc_matrix))
+    q = np.asarray(top.charges, dtype=np.float64).reshape(-1)
+    pairs = R.exact_pairs(xw, s.box, s.cutoff, ep, ei)
+    Fl, El = R.lj_pair_terms(xw, s.lj, pairs, s.box, sp, si)
+    ex = np.array(sorted({(a, int(b)) for a in range(s.num_atoms) for b in bonded[a] if b > a}), dtype=np.int64)
+    Fp, Ep, _ = EW.pme_total(xw, q, s.box, pairs, ex, s.pme_grid, 4, pme.beta, ONE_OVER_4PI_EPS0)
+    Fref = Fl + Fp + R.bond_terms(xw, s.bonds, s.bond_params, s.box)[0] + \
+        R.angle_terms(xw, s.angles, s.angle_params, s.box)[0]
+    if len(s.dihedrals):
+        Fref += R.dihedral_terms(xw, s.dihedrals, s.dihedral_params, s.box)[0]
+    if len(s.impropers):
+        Fref += R.improper_terms(xw, s.impropers, s.improper_params, s.box)[0]
+    err = rel_rms(F, Fref)
+    say("  total rel force RMS %.3e" % err)
+    assert err < FORCE_RTOL

@@ -531,3 +531,19 @@ def test_coordination_number_simple_cubic_closed_form():
     # rdf of an ideal lattice: zero between shells
     rm, _, _ = A.rdf(x, box, ids, ids, rc, nb)
     assert rm[10] == 0 and rm[int(a / w)] > 0
+
+
+def test_topology_tables_pins():
+    """oracle.mdref.topology_tables on hand-worked cases: a 4-atom chain (1-2 and 1-3 partners
+    in insertion order, one 1-4 pair), a repeated dihedral end pair (benzene-like, kept once),
+    a three-ring whose angle closes an existing bond (the reference raises), a self bond."""
+    from oracle import mdref as R
+    b, s = R.topology_tables(4, [(0, 1), (1, 2), (2, 3)], [(0, 1, 2), (1, 2, 3)], [(0, 1, 2, 3)])
+    assert b == [[1, 2], [0, 2, 3], [1, 3, 0], [2, 1]]
+    assert s == [[3], [], [], [0]]
+    _, s2 = R.topology_tables(4, [], [], [(0, 1, 2, 3), (0, 2, 1, 3)])
+    assert s2 == [[3], [], [], [0]]
+    with pytest.raises(R.TopologyConflict):
+        R.topology_tables(3, [(0, 1), (1, 2), (2, 0)], [(0, 1, 2)], [])
+    with pytest.raises(R.TopologyConflict):
+        R.topology_tables(2, [(1, 1)], [], [])
