@@ -72,9 +72,54 @@ def cpu_baseline(system, steps=2):
     wall = time.perf_counter() - t0
     evals = 2 + 3 * (steps - 1)            # langevin_integrator.py:51-53,64
     s_per_step = wall / evals * 3          # steady state: three evaluations per step
+    progress("CPU baseline: float64 SPME oracle lines")
     return {"value": 0.0864 / s_per_step, "unit": "ns/day", "cores": 1, "kind": "port", "host": host_cpu(),
+            "oracle_lines": spme_oracle_lines(system),
             "sample": "%d reference Langevin steps (%d force evaluations) of the same 23,328-atom box, "
                       "%.1f s wall; steady-state step = 3 evaluations" % (steps, evals, wall)}
+
+
+def spme_oracle_lines(big, c1_steps=100):
+    """The float64 numpy oracle (exact pair set, LJ, smooth PME, bonds, angles) timed on the host:
+    100 G-JF Langevin steps of the 648-atom water box (r_c 9 A, PME 24^3; one evaluation per
+    step), and one force evaluation of the 23,328-atom box. numpy, one process."""
+    from mdpy_amd import systems
+    from mdpy_amd.unit import KB, ONE_OVER_4PI_EPS0
+    from oracle import ewald as EW, mdref as R
+
+    def force_fn(s):
+        ep, ei = s.exclusions()
+        beta = EW.beta_from_tolerance(s.cutoff, 1e-5)
+        ex = np.array(sorted({(a, int(b)) for a in range(s.num_atoms) for b in ei[ep[a]:ep[a + 1]] if b > a}))
+        pbc, inv = s.pbc_matrix, np.linalg.inv(s.pbc_matrix)
+
+        def f(x):
+            xw = R.wrap_positions(x, pbc, inv)
+            pairs = R.exact_pairs(xw, s.box, s.cutoff, ep, ei)
+            F = R.lj_pair_terms(xw, s.lj, pairs, s.box)[0]
+            F = F + EW.pme_total(xw, s.charges, s.box, pairs, ex, s.pme_grid, 4, beta, ONE_OVER_4PI_EPS0)[0]
+            F = F + R.bond_terms(xw, s.bonds, s.bond_params, s.box)[0]
+            return F + R.angle_terms(xw, s.angles, s.angle_params, s.box)[0]
+        return f
+
+    small = systems.water_box_small()
+    f = force_fn(small)
+    rng = np.random.default_rng(0)
+    x = np.asarray(small.positions, dtype=np.float64)
+    v = maxwell_velocities(small.masses, 300.0, 0)
+    t0 = time.perf_counter()
+    F0 = f(x)
+    R.langevin_gjf(x, v, F0, small.masses, f, 1.0, KB * 300.0, 1e-3, rng.standard_normal, c1_steps)
+    wall_small = time.perf_counter() - t0
+    fb = force_fn(big)
+    t0 = time.perf_counter()
+    fb(np.asarray(big.positions, dtype=np.float64))
+    wall_big = time.perf_counter() - t0
+    return [{"workload": "water648 (r_c 9 A, PME 24^3), %d G-JF Langevin steps, float64 SPME oracle" % c1_steps,
+             "value": 0.0864 / ((wall_small) / c1_steps), "unit": "ns/day", "wall_s": wall_small, "cores": 1},
+            {"workload": "water23328 (r_c 12 A, PME 64^3), one float64 SPME-oracle force evaluation",
+             "value": 0.0864 / wall_big, "unit": "ns/day at one evaluation per step", "wall_s": wall_big,
+             "cores": 1}]
 
 
 def drift_fit(etot, dt, every, n_atoms, block=20):
