@@ -110,6 +110,10 @@ struct DevState {
   unsigned long long ke_acc;  // kinetic-energy partial sums of the opening kernel (0 between calls)
 };
 
+// neighbour-list entries allocated past the last row (at least the longest read-ahead: one
+// chunk of 64 lanes x 4 entries)
+#define MDX_LIST_SLACK 512
+
 // floor(u) clamped to [0, n) for a cell or grid coordinate u >= 0. The range is decided in
 // floating point before the conversion: a float-to-int conversion of NaN, or of a value outside
 // int's range, has no defined result (LLVM treats it as poison, which a later integer clamp does

@@ -976,7 +976,8 @@ int mdx_nlist_prepare(mdx_ctx* c) {
   if (cap < c->nbr_cap) cap = c->nbr_cap;
   c->nbr_cap = cap;
   c->list_stride = cap;
-  if ((rc = c->nbr_list.ensure((size_t)cap * c->n))) return rc;
+  // slack after the last row: the nonbonded kernel loads the next chunk's indices ahead
+  if ((rc = c->nbr_list.ensure((size_t)cap * c->n + MDX_LIST_SLACK))) return rc;
   MDX_HIP(hipMemset(c->pos_at_build.p, 0, 3 * sizeof(double) * c->n));
   int one = 1;
   MDX_HIP(hipMemcpy(&c->dstate.p->rebuild, &one, sizeof(int), hipMemcpyHostToDevice));

@@ -215,13 +215,22 @@ __device__ __forceinline__ void nb_block(const NbArgs& a, const NbPtrs& P, int v
         fz += fr * dz;
       }
     };
-    // whole chunks of G * NB_U entries: unpredicated index loads
+    // whole chunks of G * NB_U entries: unpredicated index loads, the next chunk's issued before
+    // this chunk is evaluated (the rows are followed by slack, so a load past a row's end reads
+    // the next row or the slack and is never used)
     int cb = 0;
-    for (; cb + G * NB_U <= cnt; cb += G * NB_U) {
+    if (G * NB_U <= cnt) {
       unsigned e[NB_U];
 #pragma unroll
-      for (int u = 0; u < NB_U; ++u) e[u] = lp[cb + u * G];
-      pass(e, 0u, NbTail<false>());
+      for (int u = 0; u < NB_U; ++u) e[u] = lp[u * G];
+      for (; cb + G * NB_U <= cnt; cb += G * NB_U) {
+        unsigned en[NB_U];
+#pragma unroll
+        for (int u = 0; u < NB_U; ++u) en[u] = lp[cb + G * NB_U + u * G];
+        pass(e, 0u, NbTail<false>());
+#pragma unroll
+        for (int u = 0; u < NB_U; ++u) e[u] = en[u];
+      }
     }
     if (cb < cnt) {  // the row's last, partial chunk: slots past the end take the atom itself
       unsigned e[NB_U], valid = 0;

@@ -94,6 +94,35 @@ def test_verlet_refcompat_matches_oracle():
     e.close()
 
 
+def test_verlet_refcompat_two_calls_match_oracle():
+    """Two integrate calls (3 then 4 steps, graph replays): the second continues from the
+    position history the first left (verlet_integrator.py:28-33 initialises only when nothing is
+    cached), on the device as in the oracle."""
+    from mdpy_amd import _lib as L, systems
+    from mdpy_amd.engine import Engine
+    from oracle import mdref as R
+    say("case: Verlet refcompat two calls vs oracle, water648, 3 + 4 steps")
+    s = systems.water_box_small()
+    v0 = thermal_v(s, 300.0, 6)
+    e = Engine(s, skin=1.0)
+    e.set_positions(s.positions)
+    e.set_velocities(v0)
+    e.configure_integrator(L.INT_VERLET_REF, dt=1.0)
+    pbc = s.pbc_matrix
+    inv = np.linalg.inv(pbc)
+    it = R.VerletReference(1.0)
+    xo, vo = R.wrap_positions(s.positions, pbc, inv), v0
+    for n in (3, 4):
+        e.integrate(REF_TERMS, n, use_graph=True)
+        xo, vo = it.integrate(xo, vo, s.masses, ref_force_fn(s), pbc, inv, n)
+        xg, vg = e.positions(), e.velocities()
+        dx = np.abs(xg - xo).max()
+        dv = np.abs(vg - vo).max() / np.abs(vo).max()
+        say("  after %d steps: max |dx| %.3e A, max rel dv %.3e" % (n, dx, dv))
+        assert dx < 1e-7 and dv < 1e-5
+    e.close()
+
+
 def test_graph_replay_bitwise_equals_direct_launches():
     """Calls of 25, 7, 2, 40 and 3 G-JF steps: the first primes its forces (direct first step,
     step graphs, direct closing kernels); the later ones start from the last call's forces and
