@@ -1034,6 +1034,11 @@ extern "C" int mdx_nonbonded_config(mdx_ctx* c, double cutoff, double skin, int 
 // ------------------------------------------------------------------ force evaluation
 // reciprocal forces held outside the fixed-point accumulator by the last evaluation
 static const double* live_fpart(const mdx_ctx* c) { return c->fpart_live ? c->fpart.p : nullptr; }
+// whether an evaluation of `terms` leaves per-plane reciprocal forces in fpart (the PME plane
+// kernels, with or without the phase launches; mdx_pme_enqueue's condition)
+static bool evaluation_leaves_fpart(const mdx_ctx* c, unsigned terms) {
+  return (terms & MDX_TERM_COUL_RECIP) && c->pme_configured && c->pme_planes && c->nb_configured;
+}
 
 int mdx_enqueue_zero_accumulators(mdx_ctx* c) {
   long long n3 = 3LL * c->n;
@@ -1538,12 +1543,18 @@ static int ensure_graphs(mdx_ctx* c, unsigned terms) {
   if (c->graph_e && c->graph_terms == terms && c->graph_kind == c->int_kind) return 0;
   const bool cur = c->forces_current;  // capturing enqueues nothing that runs
   const unsigned cur_terms = c->forces_terms;
+  const bool cur_fpart = c->fpart_live;
   mdx_invalidate_graph(c);
+  // every captured step follows an evaluation of `terms` when it replays: its prologue must read
+  // the reciprocal forces that evaluation leaves, whatever the context evaluated last (a fresh
+  // context has evaluated nothing yet)
+  c->fpart_live = evaluation_leaves_fpart(c, terms);
   int rc = 0;
   for (int k = 0; k < MDX_NGRAPH_SIZES && !rc; ++k) rc = capture_steps(c, terms, kGraphSizes[k], false, &c->graph_f[k]);
   if (!rc) rc = capture_steps(c, terms, 1, true, &c->graph_e);
   if (!rc && fused_steps(c, terms)) rc = capture_head(c, terms, &c->graph_head);
   if (!rc && fused_steps(c, terms)) rc = capture_tail(c, terms, &c->graph_tail);
+  c->fpart_live = cur_fpart;
   if (rc) {
     mdx_invalidate_graph(c);
     return rc;

@@ -151,6 +151,28 @@ def test_graph_replay_bitwise_equals_direct_launches():
             assert np.array_equal(a, b), "call %d (%d steps)" % (k, n)
 
 
+def test_graph_replay_on_fresh_context_reads_reciprocal_forces():
+    """A fresh context's first call captures its step graphs before it has evaluated anything:
+    the 32-step graph's first prologue must still read the reciprocal forces of the evaluation
+    that precedes it on replay. One 60-step call (energies every 50: step graphs of 32 + 16, an
+    energy step, 8 + 1, the closing step) equals direct launches bit for bit."""
+    from mdpy_amd import _lib as L, systems
+    from mdpy_amd.engine import Engine
+    say("case: hipGraph vs direct on a fresh context, water648, one G-JF call of 60 steps")
+    s = systems.water_box_small()
+    out = []
+    for graph in (False, True):
+        e = Engine(s, skin=1.0)
+        e.set_positions(s.positions)
+        e.set_velocities(thermal_v(s, 300.0, 11))
+        e.configure_integrator(L.INT_LANGEVIN, dt=1.0, temperature=300.0, friction=1e-3, seed=3)
+        e.integrate(L.TERM_LJ | L.TERM_PME | L.TERM_BOND | L.TERM_ANGLE, 60, use_graph=graph, energy_every=50)
+        out.append((e.positions(), e.velocities(), e.energy_history()))
+        e.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
 def test_kernel_timing_leaves_the_trajectory_unchanged():
     """Timing the step prologue (state saved and restored) and the three PME plane launches
     between two integrate calls changes nothing: the next call starts from a fresh evaluation at
