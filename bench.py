@@ -212,24 +212,48 @@ def chain_leg(steps=1000):
 
 
 def replicas_leg(steps=200):
-    """64 uneven water boxes (300-3000 waters, r_c 9 A): aggregate atom-steps/s per batch mode."""
+    """64 uneven water boxes (300-3,000 waters at 0.0334 waters/A^3, each its own cube and PME
+    grid of ~1 point/A, r_c 9 A, G-JF 300 K): aggregate atom-steps/s of the fused batch (every
+    kernel of a step one launch over all replicas, steps replayed from hipGraphs; list skin and
+    lanes per atom autotuned on the whole batch) against the sequential per-replica loop (each
+    replica's own integrate call, one after another, direct launches). The per-replica graph
+    modes are limited to 8 replicas: they are compared with the fused batch on 8 replicas."""
     import torch
     from mdpy_amd import systems
     from mdpy_amd.replicas import ReplicaBatch
     reps = systems.water_replicas(64, seed=0)
+    progress("replicas: 64 contexts, %d atoms" % sum(s.num_atoms for s in reps))
     b = ReplicaBatch(reps, energy_every=50)
-    skins = b.autotune_skins(candidates=(1.0, 1.5, 2.0), steps=30, warmup=10)
-    rates = {}
-    for mode in ("sequential", "streams"):
-        b.integrate(20, mode)
-        rates[mode] = b.atom_steps_per_s(steps, mode)
+    best, tune = b.autotune_fused(skins=(1.0, 1.5, 2.0), lanes=(16, 32), steps=40, warmup=20)
+    progress("replicas: fused batch at skin %.1f A, %d lanes per atom" % best)
+    b.integrate(40, "fused")
+    fused = b.atom_steps_per_s(steps, "fused")
+    seq_steps = 50
+    for e in b.engines:  # warm: list built at the chosen skin, lanes set
+        e.integrate(b.terms, 10, False, 50)
     torch.cuda.synchronize()
-    best = max(rates, key=rates.get)
-    out = {"metric": "aggregate atom-steps/s, 64 water replicas", "atoms": b.n_atoms, "steps": steps,
-           "rates": rates, "best_mode": best, "speedup_vs_sequential": rates[best] / rates["sequential"],
-           "skins": {str(k): skins.count(k) for k in sorted(set(skins))}, "device_flags": max(b.status())}
+    t0 = time.perf_counter()
+    for e in b.engines:
+        e.integrate(b.terms, seq_steps, False, 50)
+    torch.cuda.synchronize()
+    sequential = b.n_atoms * seq_steps / (time.perf_counter() - t0)
+    flags = max(b.status())
     b.close()
-    return out
+    progress("replicas: 8-replica comparison with the per-replica graph modes")
+    b8 = ReplicaBatch(reps[:8], energy_every=50, skins=[best[0]] * 8)
+    b8.set_lanes(best[1])
+    r8 = {}
+    for mode in ("fused", "streams", "sequential"):
+        b8.integrate(20, mode)
+        r8[mode] = b8.atom_steps_per_s(steps, mode)
+    flags = max(flags, max(b8.status()))
+    b8.close()
+    return {"metric": "aggregate atom-steps/s, 64 water replicas (fused batch)", "value": fused,
+            "atoms": int(sum(s.num_atoms for s in reps)), "steps": steps,
+            "sequential_per_replica_loop": sequential, "speedup_vs_sequential": fused / sequential,
+            "autotuned": {"skin_A": best[0], "lanes": best[1],
+                          "candidates": {"%.1f/%d" % k: v for k, v in tune.items()}},
+            "first_8_replicas": r8, "device_flags": flags}
 
 
 PEAK_HBM_GBPS = 8000.0
@@ -362,8 +386,9 @@ def main():
     ap.add_argument("--phases", default="off",
                     help="'off': nonbonded and PME on two streams joined per step; 'f0,f1': the PME "
                          "launches carry the nonbonded work in shares f0, f1 and the rest, on one stream")
-    ap.add_argument("--replicas", action="store_true",
-                    help="also measure the 64 batched water replicas (aggregate atom-steps/s per batch mode)")
+    ap.add_argument("--no-replicas", action="store_true",
+                    help="skip the 64 batched water replicas leg (aggregate atom-steps/s of the fused batch "
+                         "against the sequential per-replica loop)")
     ap.add_argument("--mlp-frames", type=int, default=1000, help="training + test frames of the MLP leg")
     ap.add_argument("--mlp-train-s", type=float, default=40.0, help="training time budget of the MLP leg (s)")
     ap.add_argument("--no-mlp", action="store_true",
@@ -474,10 +499,9 @@ def main():
     if args.chain:
         progress("solvated chain leg")
         out["chain"] = chain_leg()
-    if args.replicas:
-        # the 64-replica leg ended in a host-side segmentation fault whose cause is not yet
-        # identified; it stays off until it is
-        out["replicas"] = {"error": "disabled: the 64-replica run crashed on the host (cause under investigation)"}
+    if not args.no_replicas:
+        progress("batched replicas leg")
+        out["replicas"] = replicas_leg()
     if not args.no_mlp:
         progress("MLP surrogate leg")
         out["mlp"] = mlp_leg(s, e, terms, ns_day, n_frames=args.mlp_frames, train_budget_s=args.mlp_train_s)

@@ -228,6 +228,13 @@ typedef struct mdx_batch mdx_batch;
                                     MDX_ERR_INVALID until a host crash is understood)          */
 #define MDX_BATCH_STREAMS     1  /* each replica replays its own graph on its own stream         */
 #define MDX_BATCH_SEQUENTIAL  2  /* replica after replica (the per-replica loop, for comparison) */
+#define MDX_BATCH_FUSED       3  /* one launch per kernel per step covering every replica's
+                                    workgroups (prologue, sort/fill, PME planes, list build,
+                                    nonbonded + bonded), the steps captured in hipGraphs. Needs
+                                    G-JF or velocity Verlet, LJ + PME (+ bonded terms), the PME
+                                    plane path, a list skin > 0 and the same lanes per atom in
+                                    every replica; trajectories equal single-context runs bit
+                                    for bit. No replica-count limit beyond device memory.       */
 int mdx_batch_create(mdx_ctx** ctxs, int n_ctx, void* hip_stream, mdx_batch** out);
 /* Advance every replica `nsteps` steps of `terms` (each with its own integrator settings; all
  * replicas need the same energy interval). Energy histories are per context, as after

@@ -46,7 +46,9 @@ INT_LANGEVIN_REF = 3
 BATCH_ONE_GRAPH = 0
 BATCH_STREAMS = 1
 BATCH_SEQUENTIAL = 2
-BATCH_MODES = {"one_graph": BATCH_ONE_GRAPH, "streams": BATCH_STREAMS, "sequential": BATCH_SEQUENTIAL}
+BATCH_FUSED = 3
+BATCH_MODES = {"one_graph": BATCH_ONE_GRAPH, "streams": BATCH_STREAMS, "sequential": BATCH_SEQUENTIAL,
+               "fused": BATCH_FUSED}
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_dbl_p = ctypes.POINTER(ctypes.c_double)

@@ -396,7 +396,7 @@ __device__ void last_block_bin(const StepPro& P, DevState* st) {
   }
 }
 
-__device__ void prologue_block_end(const StepPro& P, DevState* st, double d2, bool bad) {
+__device__ void prologue_block_end(const StepPro& P, DevState* st, double d2, bool bad, int nblocks) {
   __shared__ double sm[16];
   __shared__ int last;
   if (bad) atomicOr(&st->flags, MDX_FLAG_NONFINITE);
@@ -408,7 +408,7 @@ __device__ void prologue_block_end(const StepPro& P, DevState* st, double d2, bo
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, sm[w]);
     atomicMax(&st->max_disp2_bits, enc_double_order(v));
     __threadfence();
-    last = atomicAdd(&st->step_ticket, 1u) == gridDim.x - 1;
+    last = atomicAdd(&st->step_ticket, 1u) == (unsigned)nblocks - 1;
   }
   __syncthreads();
   if (!last) return;
@@ -434,16 +434,38 @@ __device__ void prologue_block_end(const StepPro& P, DevState* st, double d2, bo
   }
 }
 
+// The arguments of one context's step prologue launch (passed by value to k_step_pro, or one
+// replica's record of a batched launch).
+struct ProCall {
+  IntArgs A;
+  StepPro P;
+  double* x;
+  double* v;
+  const double* im;
+  const double* m;
+  double* acc_prev;
+  long long* ke_slot;
+  DevState* st;
+  int blocks;  // workgroups of the context's launch: the last-workgroup ticket counts to it
+};
+
 // The step prologue: with FUSE_B it first completes the previous step (the velocity half of
 // G-JF or velocity Verlet with the forces of the last evaluation, plus the kinetic energy; the
 // same arithmetic as k_gjf_b / k_vv_b), then runs this step's position half (as k_gjf_a /
-// k_vv_a) and the list bookkeeping of prologue_atom / prologue_block_end.
+// k_vv_a) and the list bookkeeping of prologue_atom / prologue_block_end. blk: the context's
+// workgroup.
 template <bool GJF, bool FUSE_B>
-__global__ __launch_bounds__(PRO_THREADS) void k_step_pro(IntArgs A, StepPro P, double* __restrict__ x,
-                                                          double* __restrict__ v, const double* __restrict__ im,
-                                                          const double* __restrict__ m, double* __restrict__ acc_prev,
-                                                          long long* __restrict__ ke_slot, DevState* st) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void step_pro_body(const ProCall& c, int blk) {
+  const IntArgs& A = c.A;
+  const StepPro& P = c.P;
+  double* __restrict__ x = c.x;
+  double* __restrict__ v = c.v;
+  const double* __restrict__ im = c.im;
+  const double* __restrict__ m = c.m;
+  double* __restrict__ acc_prev = c.acc_prev;
+  long long* __restrict__ ke_slot = c.ke_slot;
+  DevState* st = c.st;
+  const int i = blk * blockDim.x + threadIdx.x;
   double d2 = 0.0, ke = 0.0;
   bool bad = false;
   if (i < A.n) {
@@ -500,7 +522,19 @@ __global__ __launch_bounds__(PRO_THREADS) void k_step_pro(IntArgs A, StepPro P, 
     prologue_atom(A, P, i, pin, xn, &d2, &bad);
   }
   if (FUSE_B) block_add_energy(ke, ke_slot);  // before this block's ticket: the last block records it
-  prologue_block_end(P, st, d2, bad);
+  prologue_block_end(P, st, d2, bad, c.blocks);
+}
+
+template <bool GJF, bool FUSE_B>
+__global__ __launch_bounds__(PRO_THREADS) void k_step_pro(ProCall c) {
+  step_pro_body<GJF, FUSE_B>(c, blockIdx.x);
+}
+// every replica's prologue in one launch
+template <bool GJF, bool FUSE_B>
+__global__ __launch_bounds__(PRO_THREADS) void k_step_pro_batch(BatchMap map, const ProCall* __restrict__ calls) {
+  int blk;
+  const int r = batch_replica(map, &blk);
+  step_pro_body<GJF, FUSE_B>(calls[r], blk);
 }
 
 // velocity half: v_{n+1} = a v_n + dt/2 (a a_n + a_{n+1}) + b sigma sqrt(dt) xi_{n+1}/sqrt(m)
@@ -1290,6 +1324,25 @@ static bool fused_steps(const mdx_ctx* c, unsigned terms) {
   return grid_used && (c->int_kind == MDX_INT_LANGEVIN || c->int_kind == MDX_INT_VELOCITY_VERLET);
 }
 
+// the step prologue launch of a context (A.fpart: the reciprocal forces of the last evaluation)
+static ProCall pro_call(mdx_ctx* c) {
+  ProCall k;
+  k.A = int_args(c);
+  k.A.fpart = live_fpart(c);
+  k.P = step_pro(c);
+  k.x = c->pos;
+  k.v = c->vel;
+  k.im = c->inv_mass.p;
+  k.m = c->mass.p;
+  k.acc_prev = c->acc_prev.p;
+  k.ke_slot = c->efix.p + MDX_E_KINETIC;
+  k.st = c->dstate.p;
+  k.blocks = div_up(c->n, PRO_THREADS);
+  return k;
+}
+// its dynamic LDS: the last workgroup's binning counts and starts
+static size_t pro_lds_bytes(const mdx_ctx* c, const ProCall& k) { return k.P.bin ? 2 * (size_t)c->ncell * sizeof(int) : 0; }
+
 // One integrator step. fuse_prev_b: this step's prologue also completes the previous step's
 // velocity half (the previous call to this function had emit_b = false); emit_b: finish this
 // step with the standalone velocity half.
@@ -1300,13 +1353,11 @@ int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy, bool fu
   int rc;
   long long* ke = c->efix.p + MDX_E_KINETIC;
   if (fused_steps(c, terms)) {
-    StepPro P = step_pro(c);
-    const size_t lds = P.bin ? 2 * (size_t)c->ncell * sizeof(int) : 0;
-    dim3 gp(div_up(c->n, PRO_THREADS)), bp(PRO_THREADS);
-    A.fpart = live_fpart(c);  // the forces the prologue reads are the last evaluation's
+    const ProCall pc = pro_call(c);  // the forces the prologue reads are the last evaluation's
+    const size_t lds = pro_lds_bytes(c, pc);
+    dim3 gp(pc.blocks), bp(PRO_THREADS);
     const bool gjf = c->int_kind == MDX_INT_LANGEVIN;
-#define PRO_L(G, F) \
-  k_step_pro<G, F><<<gp, bp, lds, c->stream>>>(A, P, c->pos, c->vel, c->inv_mass.p, c->mass.p, c->acc_prev.p, ke, c->dstate.p)
+#define PRO_L(G, F) k_step_pro<G, F><<<gp, bp, lds, c->stream>>>(pc)
     if (gjf && fuse_prev_b) PRO_L(true, true);
     else if (gjf) PRO_L(true, false);
     else if (fuse_prev_b) PRO_L(false, true);
@@ -1314,7 +1365,7 @@ int mdx_integrator_enqueue_step(mdx_ctx* c, unsigned terms, bool energy, bool fu
 #undef PRO_L
     MDX_LAUNCH_CHECK();
     c->prologue_done = true;
-    c->prologue_binned = P.bin != 0;
+    c->prologue_binned = pc.P.bin != 0;
     rc = mdx_enqueue_forces(c, terms, energy);
     c->prologue_done = false;
     c->prologue_binned = false;
@@ -1644,13 +1695,48 @@ extern "C" int mdx_integrate(mdx_ctx* c, unsigned terms, int nsteps, int use_gra
 // Independent systems (one context each, same device) advanced together. The contexts keep
 // their own streams, lists, grids and graphs; the batch joins the caller's stream to all of
 // them and back.
+// MDX_BATCH_FUSED: each kernel of a step is one launch over every replica's workgroups, the
+// replicas' arguments in device tables (BatchKind); steps 1 .. n-1 of a call replay graphs of
+// such launches captured on the lead stream, the PME planes forked onto the batch's side stream.
+struct BatchFused {
+  BatchKind pro, fill, lfast, lcell, pfwd, pfft, pgat, nb;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipGraphExec_t gf[MDX_NGRAPH_SIZES] = {};
+  hipGraphExec_t ge = nullptr;
+  std::vector<unsigned long long> gens;  // the contexts' graph generations the tables were built at
+  unsigned terms = 0;
+  bool ready = false;
+  bool gjf = false, s14 = false;
+  int lanes = 0;
+};
+
 struct mdx_batch {
   std::vector<mdx_ctx*> ctx;
   hipStream_t ext = nullptr;     // caller's stream
-  hipStream_t lead = nullptr;    // capture / launch stream of the one-graph mode
+  hipStream_t lead = nullptr;    // capture / launch stream of the one-graph and fused modes
   hipEvent_t ev_in = nullptr, ev_fork = nullptr;
   std::vector<hipEvent_t> ev_out;
+  BatchFused fz;
 };
+
+// drop the fused mode's graphs (after the batch's streams have finished with them)
+static void fused_drop_graphs(mdx_batch* b) {
+  BatchFused& f = b->fz;
+  bool any = f.ge != nullptr;
+  for (hipGraphExec_t g : f.gf) any |= g != nullptr;
+  if (any) {
+    if (b->lead) (void)hipStreamSynchronize(b->lead);
+    if (f.side) (void)hipStreamSynchronize(f.side);
+    for (hipGraphExec_t& g : f.gf) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+    if (f.ge) (void)hipGraphExecDestroy(f.ge);
+    f.ge = nullptr;
+  }
+  f.ready = false;
+}
 
 extern "C" int mdx_batch_create(mdx_ctx** ctxs, int n, void* hip_stream, mdx_batch** out) {
   MDX_CHECK(ctxs && n > 0 && out, MDX_ERR_INVALID, "mdx_batch_create: bad arguments");
@@ -1671,8 +1757,11 @@ extern "C" int mdx_batch_create(mdx_ctx** ctxs, int n, void* hip_stream, mdx_bat
   b->ext = (hipStream_t)hip_stream;
   b->ev_out.resize(n, nullptr);
   bool ok = hipStreamCreateWithFlags(&b->lead, hipStreamNonBlocking) == hipSuccess &&
+            hipStreamCreateWithFlags(&b->fz.side, hipStreamNonBlocking) == hipSuccess &&
             hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) == hipSuccess &&
-            hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess;
+            hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&b->fz.ev_fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&b->fz.ev_join, hipEventDisableTiming) == hipSuccess;
   for (int i = 0; i < n && ok; ++i) ok = hipEventCreateWithFlags(&b->ev_out[i], hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     mdx_batch_destroy(b);
@@ -1685,10 +1774,17 @@ extern "C" int mdx_batch_create(mdx_ctx** ctxs, int n, void* hip_stream, mdx_bat
 
 extern "C" int mdx_batch_destroy(mdx_batch* b) {
   if (!b) return 0;
+  fused_drop_graphs(b);
   for (mdx_ctx* c : b->ctx) {  // standalone again
     c->in_batch = false;
     mdx_invalidate_graph(c);
   }
+  if (b->fz.ev_fork) hipEventDestroy(b->fz.ev_fork);
+  if (b->fz.ev_join) hipEventDestroy(b->fz.ev_join);
+  if (b->fz.side) hipStreamDestroy(b->fz.side);
+  for (BatchKind* k : {&b->fz.pro, &b->fz.fill, &b->fz.lfast, &b->fz.lcell, &b->fz.pfwd, &b->fz.pfft, &b->fz.pgat,
+                       &b->fz.nb})
+    k->buf.release();
   for (hipEvent_t e : b->ev_out)
     if (e) hipEventDestroy(e);
   if (b->ev_in) hipEventDestroy(b->ev_in);
@@ -1705,9 +1801,161 @@ static int stream_after(hipStream_t s, hipStream_t from, hipEvent_t ev) {
   return 0;
 }
 
+// ---- MDX_BATCH_FUSED
+// The terms a fused batch step evaluates: LJ and PME (direct and reciprocal), plus bonded terms.
+static const unsigned kFusedBonded = MDX_TERM_BOND | MDX_TERM_ANGLE | MDX_TERM_DIHEDRAL | MDX_TERM_IMPROPER;
+
+// Build (or keep) the per-replica argument tables of every batched kernel. They hold the
+// contexts' buffers and settings, so they are rebuilt whenever a context's captured graphs went
+// stale (every setting that changes a launch invalidates them) or the terms change.
+static int fused_prepare(mdx_batch* b, unsigned terms) {
+  BatchFused& f = b->fz;
+  const int n = (int)b->ctx.size();
+  bool fresh = f.ready && f.terms == terms && f.gens.size() == (size_t)n && f.lanes == b->ctx[0]->nb_lanes &&
+               f.gjf == (b->ctx[0]->int_kind == MDX_INT_LANGEVIN);
+  for (int r = 0; r < n && fresh; ++r) fresh = f.gens[r] == b->ctx[r]->graph_gen;
+  if (fresh) return 0;
+  fused_drop_graphs(b);
+  // the tables are rewritten in place: no earlier batched launch may still be reading them
+  MDX_HIP(hipStreamSynchronize(b->lead));
+  MDX_HIP(hipStreamSynchronize(f.side));
+  const unsigned need = MDX_TERM_LJ | MDX_TERM_COUL_DIRECT | MDX_TERM_COUL_RECIP;
+  MDX_CHECK((terms & need) == need && !(terms & ~(need | kFusedBonded)), MDX_ERR_INVALID,
+            "MDX_BATCH_FUSED evaluates LJ + PME (+ bond, angle, dihedral, improper terms) only");
+  mdx_ctx* c0 = b->ctx[0];
+  std::vector<ProCall> pro;
+  std::vector<int> pblocks;
+  size_t plds = 0;
+  for (int r = 0; r < n; ++r) {
+    mdx_ctx* c = b->ctx[r];
+    MDX_CHECK(fused_steps(c, terms) && c->int_kind == c0->int_kind, MDX_ERR_INVALID,
+              "replica %d: MDX_BATCH_FUSED needs G-JF or velocity Verlet (the same in every replica) on a "
+              "neighbour-list context", r);
+    MDX_CHECK(c->has_lj && c->has_charges && c->pme_configured && c->skin > 0, MDX_ERR_INVALID,
+              "replica %d: MDX_BATCH_FUSED needs LJ parameters, charges, PME and a list skin > 0", r);
+    // the steps' prologues read the reciprocal forces of the batched evaluation before them
+    const bool keep = c->fpart_live;
+    c->fpart_live = true;
+    const ProCall pc = pro_call(c);
+    c->fpart_live = keep;
+    MDX_CHECK(pc.P.bin, MDX_ERR_INVALID, "replica %d: list grid too large for the prologue's binning", r);
+    pro.push_back(pc);
+    pblocks.push_back(pc.blocks);
+    plds = std::max(plds, pro_lds_bytes(c, pc));
+  }
+  int rc;
+  if ((rc = batch_kind_upload(&f.pro, pro, pblocks, plds)) ||
+      (rc = mdx_batch_prepare_list(b->ctx.data(), n, &f.fill, &f.lfast, &f.lcell)) ||
+      (rc = mdx_batch_prepare_pme(b->ctx.data(), n, &f.pfwd, &f.pfft, &f.pgat)) ||
+      (rc = mdx_batch_prepare_nb(b->ctx.data(), n, terms & (kFusedBonded | MDX_TERM_COUL_RECIP), &f.nb)))
+    return rc;
+  f.gjf = c0->int_kind == MDX_INT_LANGEVIN;
+  f.lanes = c0->nb_lanes;
+  f.s14 = c0->has_s14;
+  f.terms = terms;
+  f.gens.resize(n);
+  for (int r = 0; r < n; ++r) f.gens[r] = b->ctx[r]->graph_gen;
+  f.ready = true;
+  return 0;
+}
+
+// one step of every replica on the lead stream (fuse_prev: the prologue completes the previous
+// step's velocity half)
+static int fused_step(mdx_batch* b, bool energy, bool fuse_prev) {
+  BatchFused& f = b->fz;
+  hipStream_t s = b->lead;
+  const ProCall* pc = (const ProCall*)f.pro.calls;
+#define PRO_B(G, F) k_step_pro_batch<G, F><<<f.pro.blocks, PRO_THREADS, f.pro.lds, s>>>(f.pro.map, pc)
+  if (f.gjf && fuse_prev) PRO_B(true, true);
+  else if (f.gjf) PRO_B(true, false);
+  else if (fuse_prev) PRO_B(false, true);
+  else PRO_B(false, false);
+#undef PRO_B
+  MDX_LAUNCH_CHECK();
+  int rc;
+  if ((rc = mdx_batch_launch_fill(f.fill, s))) return rc;
+  // PME planes beside the list build and the pair kernel, as in a single context's step
+  MDX_HIP(hipEventRecord(f.ev_fork, s));
+  MDX_HIP(hipStreamWaitEvent(f.side, f.ev_fork, 0));
+  if ((rc = mdx_batch_launch_pme(f.pfwd, f.pfft, f.pgat, f.side))) return rc;
+  MDX_HIP(hipEventRecord(f.ev_join, f.side));
+  if ((rc = mdx_batch_launch_list(f.lfast, f.lcell, s)) || (rc = mdx_batch_launch_nb(f.nb, f.lanes, f.s14, energy, s)))
+    return rc;
+  MDX_HIP(hipStreamWaitEvent(s, f.ev_join, 0));
+  return 0;
+}
+
+static int fused_capture(mdx_batch* b, int count, bool energy, hipGraphExec_t* out) {
+  hipGraph_t graph;
+  MDX_HIP(hipStreamBeginCapture(b->lead, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int k = 0; k < count && !rc; ++k) rc = fused_step(b, energy, true);
+  hipError_t e2 = hipStreamEndCapture(b->lead, &graph);
+  if (rc) return rc;
+  if (e2 != hipSuccess) { mdx_set_error("graph capture failed: %s", hipGetErrorString(e2)); return MDX_ERR_HIP; }
+  hipError_t e3 = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e3 != hipSuccess) { *out = nullptr; mdx_set_error("graph instantiate failed: %s", hipGetErrorString(e3)); return MDX_ERR_HIP; }
+  return 0;
+}
+
+static int fused_graphs(mdx_batch* b) {
+  BatchFused& f = b->fz;
+  if (f.ge) return 0;
+  int rc = 0;
+  for (int k = 0; k < MDX_NGRAPH_SIZES && !rc; ++k) rc = fused_capture(b, kGraphSizes[k], false, &f.gf[k]);
+  if (!rc) rc = fused_capture(b, 1, true, &f.ge);
+  if (rc) fused_drop_graphs(b);
+  return rc;
+}
+
+// a replica's own call set-up / closing kernels, enqueued on the batch's lead stream
+template <class Fn>
+static int on_lead(mdx_batch* b, mdx_ctx* c, Fn fn) {
+  hipStream_t own = c->stream;
+  c->stream = b->lead;
+  const int rc = fn();
+  c->stream = own;
+  return rc;
+}
+
+static int fused_integrate(mdx_batch* b, unsigned terms, int nsteps) {
+  int rc = 0;
+  for (mdx_ctx* c : b->ctx)
+    if ((rc = integ_alloc(c, nsteps))) return rc;
+  if ((rc = fused_prepare(b, terms))) return rc;
+  if (nsteps > 1 && (rc = fused_graphs(b))) return rc;
+  MDX_HIP(hipEventRecord(b->ev_in, b->ext));
+  MDX_HIP(hipStreamWaitEvent(b->lead, b->ev_in, 0));
+  // per replica: the prime (forces at the current positions, unless the last call left them)
+  // and the call's opening kernel
+  for (mdx_ctx* c : b->ctx)
+    if ((rc = on_lead(b, c, [&]() { return integ_begin(c, terms, nsteps); }))) return rc;
+  mdx_ctx* c0 = b->ctx[0];
+  if (nsteps > 0 && (rc = fused_step(b, energy_step(c0, 0, nsteps), false))) return rc;
+  std::vector<int> seq;
+  graph_schedule(c0, nsteps, seq);
+  for (int k : seq) {
+    hipError_t e = hipGraphLaunch(k == kGraphEnergy ? b->fz.ge : b->fz.gf[k], b->lead);
+    if (e != hipSuccess) { mdx_set_error("hipGraphLaunch: %s", hipGetErrorString(e)); return MDX_ERR_HIP; }
+  }
+  for (mdx_ctx* c : b->ctx) {
+    if ((rc = on_lead(b, c, [&]() { return integ_close(c, terms, nsteps); })) || (rc = integ_end(c, nsteps, false)))
+      return rc;
+    if (nsteps > 0) {
+      c->fpart_live = true;
+      c->forces_current = true;
+      c->forces_terms = terms;
+    }
+  }
+  MDX_HIP(hipEventRecord(b->ev_out[0], b->lead));
+  MDX_HIP(hipStreamWaitEvent(b->ext, b->ev_out[0], 0));
+  return 0;
+}
+
 extern "C" int mdx_batch_integrate(mdx_batch* b, unsigned terms, int nsteps, int mode) {
   MDX_CHECK(b && !b->ctx.empty(), MDX_ERR_INVALID, "mdx_batch_integrate: empty batch");
-  MDX_CHECK(mode >= MDX_BATCH_ONE_GRAPH && mode <= MDX_BATCH_SEQUENTIAL, MDX_ERR_INVALID, "unknown batch mode %d", mode);
+  MDX_CHECK(mode >= MDX_BATCH_ONE_GRAPH && mode <= MDX_BATCH_FUSED, MDX_ERR_INVALID, "unknown batch mode %d", mode);
   // The one-graph mode (every replica's steps captured into one graph from a lead stream, each
   // replica's step forking its PME onto a side stream inside that capture) ended its first GPU
   // run in a host-side segmentation fault whose cause has not been identified; it is switched
@@ -1719,6 +1967,7 @@ extern "C" int mdx_batch_integrate(mdx_batch* b, unsigned terms, int nsteps, int
   for (mdx_ctx* c : b->ctx)
     MDX_CHECK(c->energy_every == every, MDX_ERR_INVALID, "all replicas of a batch need the same energy interval");
   MDX_HIP(hipSetDevice(b->ctx[0]->device));
+  if (mode == MDX_BATCH_FUSED) return fused_integrate(b, terms, nsteps);
   // join: every replica stream (and the lead stream) waits for the caller's stream
   MDX_HIP(hipEventRecord(b->ev_in, b->ext));
   MDX_HIP(hipStreamWaitEvent(b->lead, b->ev_in, 0));
@@ -2098,25 +2347,21 @@ static int time_prologue(mdx_ctx* c, int reps, double* h_avg_ms) {
     MDX_HIP(hipMemcpyAsync(save.p + off, q.p, q.bytes, hipMemcpyDeviceToDevice, c->stream));
     off += (q.bytes + 255) & ~(size_t)255;
   }
-  IntArgs A = int_args(c);
-  StepPro P = step_pro(c);
-  P.bin = 0;
-  P.record = 0;
-  P.thr2 = HUGE_VAL;
-  long long* ke = c->efix.p + MDX_E_KINETIC;
+  ProCall pc = pro_call(c);
+  pc.P.bin = 0;
+  pc.P.record = 0;
+  pc.P.thr2 = HUGE_VAL;
   const bool gjf = c->int_kind == MDX_INT_LANGEVIN;
-  dim3 gp(div_up(c->n, PRO_THREADS)), bp(PRO_THREADS);
+  dim3 gp(pc.blocks), bp(PRO_THREADS);
   hipEvent_t e0, e1;
   MDX_HIP(hipEventCreate(&e0));
   MDX_HIP(hipEventCreate(&e1));
   MDX_HIP(hipEventRecord(e0, c->stream));
   for (int k = 0; k < reps; ++k) {
     if (gjf)
-      k_step_pro<true, true><<<gp, bp, 0, c->stream>>>(A, P, c->pos, c->vel, c->inv_mass.p, c->mass.p, c->acc_prev.p, ke,
-                                                       c->dstate.p);
+      k_step_pro<true, true><<<gp, bp, 0, c->stream>>>(pc);
     else
-      k_step_pro<false, true><<<gp, bp, 0, c->stream>>>(A, P, c->pos, c->vel, c->inv_mass.p, c->mass.p, c->acc_prev.p,
-                                                        ke, c->dstate.p);
+      k_step_pro<false, true><<<gp, bp, 0, c->stream>>>(pc);
   }
   MDX_LAUNCH_CHECK();
   MDX_HIP(hipEventRecord(e1, c->stream));

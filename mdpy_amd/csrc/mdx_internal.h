@@ -114,6 +114,58 @@ struct DevState {
 // chunk of 64 lanes x 4 entries)
 #define MDX_LIST_SLACK 512
 
+// ------------------------------------------------------------------ batched launches
+// One launch of a kernel covers the workgroups of many contexts (replicas): workgroup b belongs
+// to replica rep[b] and is that replica's workgroup b - base[rep[b]]; the replica's arguments
+// are a record in device memory (the same record a single-context launch passes by value).
+struct BatchMap {
+  const int* rep;   // replica of each workgroup of the launch
+  const int* base;  // first workgroup of each replica
+};
+__device__ __forceinline__ int batch_replica(const BatchMap& m, int* local) {
+  const int r = m.rep[blockIdx.x];
+  *local = (int)blockIdx.x - m.base[r];
+  return r;
+}
+
+// Host side of one batched kernel: the per-replica argument records, the workgroup map and the
+// dynamic LDS of the launch (the largest replica's), in one device allocation.
+struct BatchKind {
+  DevBuf<char> buf;
+  BatchMap map{};
+  const void* calls = nullptr;
+  int blocks = 0;  // workgroups of the launch, all replicas
+  size_t lds = 0;
+};
+template <class Call>
+int batch_kind_upload(BatchKind* k, const std::vector<Call>& calls, const std::vector<int>& blocks, size_t lds) {
+  const size_t n = calls.size();
+  std::vector<int> base(n);
+  int total = 0;
+  for (size_t r = 0; r < n; ++r) {
+    base[r] = total;
+    total += blocks[r];
+  }
+  std::vector<int> rep((size_t)total);
+  for (size_t r = 0; r < n; ++r)
+    for (int b = 0; b < blocks[r]; ++b) rep[(size_t)base[r] + b] = (int)r;
+  const size_t cbytes = (n * sizeof(Call) + 255) & ~(size_t)255;
+  const size_t bytes = cbytes + (n + (size_t)total) * sizeof(int);
+  int rc = k->buf.ensure(bytes);
+  if (rc) return rc;
+  std::vector<char> h(bytes);
+  memcpy(h.data(), calls.data(), n * sizeof(Call));
+  memcpy(h.data() + cbytes, base.data(), n * sizeof(int));
+  if (total) memcpy(h.data() + cbytes + n * sizeof(int), rep.data(), (size_t)total * sizeof(int));
+  MDX_HIP(hipMemcpy(k->buf.p, h.data(), bytes, hipMemcpyHostToDevice));
+  k->calls = k->buf.p;
+  k->map.base = (const int*)(k->buf.p + cbytes);
+  k->map.rep = k->map.base + n;
+  k->blocks = total;
+  k->lds = lds;
+  return 0;
+}
+
 // floor(u) clamped to [0, n) for a cell or grid coordinate u >= 0. The range is decided in
 // floating point before the conversion: a float-to-int conversion of NaN, or of a value outside
 // int's range, has no defined result (LLVM treats it as poison, which a later integer clamp does
@@ -432,6 +484,14 @@ int mdx_enqueue_request_rebuild(mdx_ctx* c);
 // nonbonded launch; the bonded terms in `bonded_terms` (bond/angle/dihedral/improper bits, and the
 // PME exclusion correction with MDX_TERM_COUL_RECIP) ride along as extra workgroups
 int mdx_enqueue_nonbonded(mdx_ctx* c, bool lj, bool coul, bool energy, unsigned bonded_terms = 0u);
+// batched replicas (MDX_BATCH_FUSED): each kind's per-replica records, and its launches
+int mdx_batch_prepare_list(mdx_ctx* const* cs, int n, BatchKind* fill, BatchKind* fast, BatchKind* cell);
+int mdx_batch_launch_fill(const BatchKind& k, hipStream_t s);
+int mdx_batch_launch_list(const BatchKind& fast, const BatchKind& cell, hipStream_t s);
+int mdx_batch_prepare_pme(mdx_ctx* const* cs, int n, BatchKind* fwd, BatchKind* fft, BatchKind* gat);
+int mdx_batch_launch_pme(const BatchKind& fwd, const BatchKind& fft, const BatchKind& gat, hipStream_t s);
+int mdx_batch_prepare_nb(mdx_ctx* const* cs, int n, unsigned bonded_terms, BatchKind* out);
+int mdx_batch_launch_nb(const BatchKind& k, int lanes, bool s14, bool energy, hipStream_t s);
 // Rebuild the nonbonded kernel's Ewald-correction table when beta or the cutoff changed.
 #define MDX_CTAB_N 256
 int mdx_coulomb_table_update(mdx_ctx* c);

@@ -408,15 +408,52 @@ __device__ __forceinline__ int cyc_dist(int a, int b, int n) {
 #define BL_U 4       // candidates per lane per pass
 #define BL_WIN 32    // index window of the exclusion masks: partners i-32 .. i+31
 
+// The arguments of one context's list-build launch (passed by value to the single-context
+// kernels, or one replica's record of a batched launch).
+struct ListCall {
+  ListArgs a;
+  const float4* xq;
+  const int* sorted_orig;
+  const int* cell_start;
+  const int* cell_count;
+  const int* stencil;
+  const int* excl_ptr;
+  const int* excl_idx;
+  const int* s14_ptr;
+  const int* s14_idx;
+  const unsigned long long* ex_win;
+  const unsigned long long* s4_win;
+  const unsigned char* win_bad;
+  const int* ref_cell_sorted;
+  const double* pos;
+  int* nbr_count;
+  unsigned* nbr_list;
+  DevState* st;
+  double thr2;
+};
+
+// the build of home cell `cell`
 template <bool EXACT64, bool FAST>
-__global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
-    ListArgs a, const float4* __restrict__ xq, const int* __restrict__ sorted_orig,
-    const int* __restrict__ cell_start, const int* __restrict__ cell_count, const int* __restrict__ stencil,
-    const int* __restrict__ excl_ptr, const int* __restrict__ excl_idx, const int* __restrict__ s14_ptr,
-    const int* __restrict__ s14_idx, const unsigned long long* __restrict__ ex_win,
-    const unsigned long long* __restrict__ s4_win, const unsigned char* __restrict__ win_bad,
-    const int* __restrict__ ref_cell_sorted, const double* __restrict__ pos, int* __restrict__ nbr_count,
-    unsigned* __restrict__ nbr_list, DevState* st, double thr2) {
+__device__ __forceinline__ void build_list_cell_body(const ListCall& L, const int cell) {
+  const ListArgs& a = L.a;
+  const float4* __restrict__ xq = L.xq;
+  const int* __restrict__ sorted_orig = L.sorted_orig;
+  const int* __restrict__ cell_start = L.cell_start;
+  const int* __restrict__ cell_count = L.cell_count;
+  const int* __restrict__ stencil = L.stencil;
+  const int* __restrict__ excl_ptr = L.excl_ptr;
+  const int* __restrict__ excl_idx = L.excl_idx;
+  const int* __restrict__ s14_ptr = L.s14_ptr;
+  const int* __restrict__ s14_idx = L.s14_idx;
+  const unsigned long long* __restrict__ ex_win = L.ex_win;
+  const unsigned long long* __restrict__ s4_win = L.s4_win;
+  const unsigned char* __restrict__ win_bad = L.win_bad;
+  const int* __restrict__ ref_cell_sorted = L.ref_cell_sorted;
+  const double* __restrict__ pos = L.pos;
+  int* __restrict__ nbr_count = L.nbr_count;
+  unsigned* __restrict__ nbr_list = L.nbr_list;
+  DevState* st = L.st;
+  const double thr2 = L.thr2;
   if (!need_rebuild(st, thr2)) return;
   __shared__ float4 cxq[BL_CH];
   __shared__ int2 cto[BL_CH];      // (sorted slot, original atom id)
@@ -433,7 +470,6 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
   __shared__ int pool_ex[BL_POOL], pool_s4[BL_POOL];
   __shared__ int fast;
   const GridArgs& gr = a.g;
-  const int cell = blockIdx.x;
   const int hb = cell_start[cell], hn = cell_count[cell];
   if (hn > SORT_MAX) {  // oversize cells are flagged where the cells are counted: empty lists
     for (int t = threadIdx.x; t < hn; t += blockDim.x) nbr_count[hb + t] = 0;
@@ -677,6 +713,11 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
   }
 }
 
+template <bool EXACT64, bool FAST>
+__global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(ListCall L) {
+  build_list_cell_body<EXACT64, FAST>(L, blockIdx.x);
+}
+
 // The common case of the build above as its own kernel: shifted candidates, exact pair mode,
 // float32 list radius, window masks present. One wave per home atom, so every ballot, count and
 // branch is wave-uniform; exclusions, the atom itself (bit 32) and 1-4 partners are mask bits, the
@@ -686,13 +727,24 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_cell(
 // scan their CSR lists. A group of 64 candidates with none inside the list radius skips the mask
 // and store work. Produces the same lists, in the same order, as k_build_list_cell.
 #define BLF_PASS (64 * BL_U)
-__global__ __launch_bounds__(BL_THREADS) void k_build_list_fast(
-    ListArgs a, const float4* __restrict__ xq, const int* __restrict__ sorted_orig,
-    const int* __restrict__ cell_start, const int* __restrict__ cell_count, const int* __restrict__ stencil,
-    const int* __restrict__ excl_ptr, const int* __restrict__ excl_idx, const int* __restrict__ s14_ptr,
-    const int* __restrict__ s14_idx, const unsigned long long* __restrict__ ex_win,
-    const unsigned long long* __restrict__ s4_win, const unsigned char* __restrict__ win_bad,
-    int* __restrict__ nbr_count, unsigned* __restrict__ nbr_list, DevState* st, double thr2) {
+__device__ __forceinline__ void build_list_fast_body(const ListCall& L, const int cell) {
+  const ListArgs& a = L.a;
+  const float4* __restrict__ xq = L.xq;
+  const int* __restrict__ sorted_orig = L.sorted_orig;
+  const int* __restrict__ cell_start = L.cell_start;
+  const int* __restrict__ cell_count = L.cell_count;
+  const int* __restrict__ stencil = L.stencil;
+  const int* __restrict__ excl_ptr = L.excl_ptr;
+  const int* __restrict__ excl_idx = L.excl_idx;
+  const int* __restrict__ s14_ptr = L.s14_ptr;
+  const int* __restrict__ s14_idx = L.s14_idx;
+  const unsigned long long* __restrict__ ex_win = L.ex_win;
+  const unsigned long long* __restrict__ s4_win = L.s4_win;
+  const unsigned char* __restrict__ win_bad = L.win_bad;
+  int* __restrict__ nbr_count = L.nbr_count;
+  unsigned* __restrict__ nbr_list = L.nbr_list;
+  DevState* st = L.st;
+  const double thr2 = L.thr2;
   static_assert(BL_CH % BLF_PASS == 0, "chunks must hold whole passes");
   if (!need_rebuild(st, thr2)) return;
   __shared__ float4 cxq[BL_CH];
@@ -706,7 +758,6 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_fast(
   __shared__ unsigned long long hexm[BL_HMAX], hs4m[BL_HMAX];
   __shared__ unsigned char hbad[BL_HMAX];
   const GridArgs& gr = a.g;
-  const int cell = blockIdx.x;
   const int hb = cell_start[cell], hn = cell_count[cell];
   if (hn > SORT_MAX) {  // oversize cells are flagged where the cells are counted: empty lists
     for (int t = threadIdx.x; t < hn; t += blockDim.x) nbr_count[hb + t] = 0;
@@ -887,6 +938,21 @@ __global__ __launch_bounds__(BL_THREADS) void k_build_list_fast(
   }
 }
 
+__global__ __launch_bounds__(BL_THREADS) void k_build_list_fast(ListCall L) { build_list_fast_body(L, blockIdx.x); }
+
+// every replica's list build in one launch (the fast build, or the cell build of a grid whose
+// stencil wraps onto itself)
+__global__ __launch_bounds__(BL_THREADS) void k_build_list_fast_batch(BatchMap map, const ListCall* __restrict__ calls) {
+  int cell;
+  const int r = batch_replica(map, &cell);
+  build_list_fast_body(calls[r], cell);
+}
+__global__ __launch_bounds__(BL_THREADS) void k_build_list_cell_batch(BatchMap map, const ListCall* __restrict__ calls) {
+  int cell;
+  const int r = batch_replica(map, &cell);
+  build_list_cell_body<false, false>(calls[r], cell);
+}
+
 __global__ void k_request_rebuild(DevState* st) { st->rebuild = 1; }
 
 int mdx_enqueue_request_rebuild(mdx_ctx* c) {
@@ -1051,12 +1117,11 @@ int mdx_nlist_update_order(mdx_ctx* c, bool force_rebuild) {
   return pro ? 0 : mdx_nlist_enqueue_pack(c);
 }
 
-// Second part: the neighbour-list build (its workgroups return at once unless rebuilding).
-int mdx_nlist_update_list(mdx_ctx* c) {
-  const double thr2 = rebuild_thr2(c);
-  DevState* st = c->dstate.p;
+// The list-build launch of a context: its arguments and kernel (0: the fast build; 1-4: the cell
+// build <EXACT64, FAST> = <false,false>, <false,true>, <true,false>, <true,true>).
+static int list_call(mdx_ctx* c, ListCall* L, int* kind) {
   const bool ref = c->pair_mode == MDX_PAIRS_REFERENCE;
-  ListArgs a;
+  ListArgs& a = L->a;
   make_grid(c, &a.g);
   a.n = c->n;
   a.cap = c->nbr_cap;
@@ -1077,22 +1142,95 @@ int mdx_nlist_update_list(mdx_ctx* c) {
     if (2 * c->reach[d] + 1 >= c->gn[d]) a.shifted = 0;
   const bool win = c->win_ready && c->h_win_bad.size() == (size_t)c->n;
   const bool fastk = a.shifted && !a.ref_mode && win;
-  if (fastk && !a.exact64) {
-    k_build_list_fast<<<c->ncell, BL_THREADS, 0, c->stream>>>(
-        a, c->xq.p, c->sorted_orig.p, c->cell_start.p, c->cell_count.p, c->d_stencil.p,
-        c->has_excl ? c->excl_ptr.p : nullptr, c->excl_idx.p, c->has_s14 ? c->s14_ptr.p : nullptr, c->s14_idx.p,
-        c->ex_win.p, c->s4_win.p, c->win_bad.p, c->nbr_count.p, c->nbr_list.p, st, thr2);
-    MDX_LAUNCH_CHECK();
-    return 0;
+  L->xq = c->xq.p;
+  L->sorted_orig = c->sorted_orig.p;
+  L->cell_start = c->cell_start.p;
+  L->cell_count = c->cell_count.p;
+  L->stencil = c->d_stencil.p;
+  L->excl_ptr = c->has_excl ? c->excl_ptr.p : nullptr;
+  L->excl_idx = c->excl_idx.p;
+  L->s14_ptr = c->has_s14 ? c->s14_ptr.p : nullptr;
+  L->s14_idx = c->s14_idx.p;
+  L->ex_win = win ? c->ex_win.p : nullptr;
+  L->s4_win = win ? c->s4_win.p : nullptr;
+  L->win_bad = win ? c->win_bad.p : nullptr;
+  L->ref_cell_sorted = c->ref_cell_sorted.p;
+  L->pos = c->pos;
+  L->nbr_count = c->nbr_count.p;
+  L->nbr_list = c->nbr_list.p;
+  L->st = c->dstate.p;
+  L->thr2 = rebuild_thr2(c);
+  *kind = fastk && !a.exact64 ? 0 : 1 + (a.exact64 ? 2 : 0) + (fastk ? 1 : 0);
+  return 0;
+}
+
+// Second part: the neighbour-list build (its workgroups return at once unless rebuilding).
+int mdx_nlist_update_list(mdx_ctx* c) {
+  ListCall L;
+  int kind, rc = list_call(c, &L, &kind);
+  if (rc) return rc;
+  dim3 grid(c->ncell), block(BL_THREADS);
+  switch (kind) {
+    case 0: k_build_list_fast<<<grid, block, 0, c->stream>>>(L); break;
+    case 1: k_build_list_cell<false, false><<<grid, block, 0, c->stream>>>(L); break;
+    case 2: k_build_list_cell<false, true><<<grid, block, 0, c->stream>>>(L); break;
+    case 3: k_build_list_cell<true, false><<<grid, block, 0, c->stream>>>(L); break;
+    default: k_build_list_cell<true, true><<<grid, block, 0, c->stream>>>(L); break;
   }
-  auto build = a.exact64 ? (fastk ? k_build_list_cell<true, true> : k_build_list_cell<true, false>)
-                         : (fastk ? k_build_list_cell<false, true> : k_build_list_cell<false, false>);
-  build<<<c->ncell, BL_THREADS, 0, c->stream>>>(
-      a, c->xq.p, c->sorted_orig.p, c->cell_start.p, c->cell_count.p, c->d_stencil.p,
-      c->has_excl ? c->excl_ptr.p : nullptr, c->excl_idx.p, c->has_s14 ? c->s14_ptr.p : nullptr, c->s14_idx.p,
-      win ? c->ex_win.p : nullptr, win ? c->s4_win.p : nullptr, win ? c->win_bad.p : nullptr, c->ref_cell_sorted.p,
-      c->pos, c->nbr_count.p, c->nbr_list.p, st, thr2);
   MDX_LAUNCH_CHECK();
+  return 0;
+}
+
+// ---- batched replicas: the sort/fill and list-build launches of every replica
+__global__ __launch_bounds__(256) void k_grid_sort_fill_batch(BatchMap map, const FillArgs* __restrict__ calls) {
+  int cell;
+  const int r = batch_replica(map, &cell);
+  const FillArgs& f = calls[r];
+  if (!need_rebuild(f.st, f.thr2)) return;
+  grid_sort_fill_block(f, cell);
+}
+
+int mdx_batch_prepare_list(mdx_ctx* const* cs, int n, BatchKind* fill, BatchKind* fast, BatchKind* cell) {
+  std::vector<FillArgs> fa;
+  std::vector<ListCall> lf, lc;
+  std::vector<int> bfa, bf, bc;
+  for (int r = 0; r < n; ++r) {
+    mdx_ctx* c = cs[r];
+    fa.push_back(fill_args(c, rebuild_thr2(c)));
+    bfa.push_back(c->ncell);
+    ListCall L;
+    int kind, rc = list_call(c, &L, &kind);
+    if (rc) return rc;
+    MDX_CHECK(kind <= 1, MDX_ERR_INVALID, "replica %d: batched steps need a list radius larger than the pair cutoff", r);
+    // every replica holds a place in both builds (zero workgroups in the one it does not use)
+    lf.push_back(L);
+    lc.push_back(L);
+    bf.push_back(kind == 0 ? c->ncell : 0);
+    bc.push_back(kind == 1 ? c->ncell : 0);
+  }
+  int rc;
+  if ((rc = batch_kind_upload(fill, fa, bfa, 0)) || (rc = batch_kind_upload(fast, lf, bf, 0)) ||
+      (rc = batch_kind_upload(cell, lc, bc, 0)))
+    return rc;
+  return 0;
+}
+
+int mdx_batch_launch_fill(const BatchKind& k, hipStream_t s) {
+  if (!k.blocks) return 0;
+  k_grid_sort_fill_batch<<<k.blocks, 256, 0, s>>>(k.map, (const FillArgs*)k.calls);
+  MDX_LAUNCH_CHECK();
+  return 0;
+}
+
+int mdx_batch_launch_list(const BatchKind& fast, const BatchKind& cell, hipStream_t s) {
+  if (fast.blocks) {
+    k_build_list_fast_batch<<<fast.blocks, BL_THREADS, 0, s>>>(fast.map, (const ListCall*)fast.calls);
+    MDX_LAUNCH_CHECK();
+  }
+  if (cell.blocks) {
+    k_build_list_cell_batch<<<cell.blocks, BL_THREADS, 0, s>>>(cell.map, (const ListCall*)cell.calls);
+    MDX_LAUNCH_CHECK();
+  }
   return 0;
 }
 

@@ -244,3 +244,67 @@ int mdx_enqueue_coulomb_plain(mdx_ctx* c) {
   MDX_LAUNCH_CHECK();
   return 0;
 }
+
+// ------------------------------------------------------------------------------------------
+// Batched replicas: every replica's nonbonded workgroups (one per unit of work, no persistent
+// loop) followed by its bonded ones, in one launch.
+struct NbCall {
+  NbArgs a;
+  NbPtrs P;
+  BondedArgs ba;
+  int nb_blocks;  // the replica's nonbonded workgroups; the rest of its workgroups are bonded
+};
+
+template <bool LJ, bool COUL, bool ENERGY, int G, bool S14>
+__global__ __launch_bounds__(NB_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_nonbonded_batch(
+    BatchMap map, const NbCall* __restrict__ calls) {
+  int b;
+  const NbCall& k = calls[batch_replica(map, &b)];
+  if (b >= k.nb_blocks) {
+    bonded_block(k.ba, b - k.nb_blocks);
+    return;
+  }
+  nb_block<LJ, COUL, ENERGY, G, NB_BLOCK, S14>(k.a, k.P, b);
+}
+
+int mdx_batch_prepare_nb(mdx_ctx* const* cs, int n, unsigned bonded_terms, BatchKind* out) {
+  std::vector<NbCall> calls;
+  std::vector<int> blocks;
+  for (int r = 0; r < n; ++r) {
+    mdx_ctx* c = cs[r];
+    NbCall k;
+    k.a = nb_args(c);
+    k.P = nb_ptrs(c);
+    MDX_CHECK(k.a.ctab && c->ctab_beta == c->beta && c->ctab_rc == c->rc, MDX_ERR_INVALID,
+              "replica %d: Ewald-correction table not built for the current beta and cutoff", r);
+    MDX_CHECK(c->nb_lanes == cs[0]->nb_lanes && c->has_s14 == cs[0]->has_s14, MDX_ERR_INVALID,
+              "replica %d: batched replicas need the same lanes per atom and 1-4 pair mode", r);
+    k.ba = BondedArgs{};
+    k.nb_blocks = (int)div_up((long long)c->n * c->nb_lanes, NB_BLOCK);
+    const int nbo = bonded_terms ? mdx_bonded_args(c, bonded_terms, &k.ba) : 0;
+    calls.push_back(k);
+    blocks.push_back(k.nb_blocks + nbo);
+  }
+  return batch_kind_upload(out, calls, blocks, 0);
+}
+
+// LJ + direct-space Coulomb, the batched step's pair terms
+int mdx_batch_launch_nb(const BatchKind& k, int lanes, bool s14, bool energy, hipStream_t s) {
+  const NbCall* calls = (const NbCall*)k.calls;
+#define NBB(E_, G_, S_) k_nonbonded_batch<true, true, E_, G_, S_><<<k.blocks, NB_BLOCK, 0, s>>>(k.map, calls)
+#define NBB_G(E_, S_)                 \
+  do {                                \
+    if (lanes == 16) NBB(E_, 16, S_); \
+    else if (lanes == 64) NBB(E_, 64, S_); \
+    else NBB(E_, 32, S_);             \
+  } while (0)
+  if (energy) {
+    if (s14) NBB_G(true, true); else NBB_G(true, false);
+  } else {
+    if (s14) NBB_G(false, true); else NBB_G(false, false);
+  }
+#undef NBB_G
+#undef NBB
+  MDX_LAUNCH_CHECK();
+  return 0;
+}

@@ -1024,10 +1024,11 @@ extern "C" int mdx_pme_set_fused(mdx_ctx* c, int enable) {
   return 0;
 }
 
-static int fft_pass(mdx_ctx* c, int axis, int dir, int mode) {
+// one axis pass of a context's grid: its transform plan, line layout, twiddles and dynamic LDS
+static void fft_axis_of(const mdx_ctx* c, int axis, FftPlan* plo, FftLines* lno, const float2** two, size_t* ldso) {
   const int Kx = c->K[0], Ky = c->K[1], Kz = c->K[2];
-  FftPlan pl;
-  FftLines ln;
+  FftPlan& pl = *plo;
+  FftLines& ln = *lno;
   int Kmax = std::max(Kx, std::max(Ky, Kz));
   if (axis == 2) {
     factorize(Kz, &pl);
@@ -1054,9 +1055,17 @@ static int fft_pass(mdx_ctx* c, int axis, int dir, int mode) {
     ln.estride = Ky * Kz;
     ln.B = std::min(16, Kz);
   }
-  const float2* tw = c->twiddle.p + (size_t)axis * Kmax;
+  *two = c->twiddle.p + (size_t)axis * Kmax;
   const size_t ls = ln.estride == 1 ? pl.K : pl.K + 1;
-  size_t lds = (2 * (size_t)ln.B * ls + pl.K) * sizeof(float2);
+  *ldso = (2 * (size_t)ln.B * ls + pl.K) * sizeof(float2);
+}
+
+static int fft_pass(mdx_ctx* c, int axis, int dir, int mode) {
+  FftPlan pl;
+  FftLines ln;
+  const float2* tw;
+  size_t lds;
+  fft_axis_of(c, axis, &pl, &ln, &tw, &lds);
   dim3 grid(div_up(ln.nlines, ln.B));
 #define FFT_L(D, M)                                                                                    \
   k_fft_axis<D, M><<<grid, FFT_THREADS, lds, c->stream>>>(pl, ln, tw, c->cgrid.p, c->qfix.p, c->conv.p, c->bc.p, \
@@ -1076,9 +1085,10 @@ static int fft_pass(mdx_ctx* c, int axis, int dir, int mode) {
 // grid (current for this evaluation) and two complex planes in LDS.
 // parts: bit 0 the plane forward kernel, bit 1 the x-axis transform pass, bit 2 the inverse +
 // gather kernel (all three for an evaluation; one at a time for timing)
-static int pme_enqueue_planes(mdx_ctx* c, unsigned parts = 7u) {
+// the plane kernels' arguments of a context (a.W: the forward kernel's record window)
+static void plane_args_of(const mdx_ctx* c, PlaneArgs* out) {
   const int Kx = c->K[0], Ky = c->K[1], Kz = c->K[2];
-  PlaneArgs a;
+  PlaneArgs& a = *out;
   a.n = c->n;
   a.Kx = Kx;
   a.Ky = Ky;
@@ -1095,6 +1105,13 @@ static int pme_enqueue_planes(mdx_ctx* c, unsigned parts = 7u) {
   a.wx = c->box[0] / c->gn[0];
   a.margin = 0.5 * c->skin + 1e-6 * c->box[0] + 1e-9;
   a.rec_cap = c->pme_rec_cap;
+  a.W = c->pme_wf;
+}
+
+static int pme_enqueue_planes(mdx_ctx* c, unsigned parts = 7u) {
+  const int Kx = c->K[0], Ky = c->K[1], Kz = c->K[2];
+  PlaneArgs a;
+  plane_args_of(c, &a);
   MDX_CHECK(c->pme_grec.n >= (size_t)Kx * a.rec_cap * 6, MDX_ERR_INVALID, "PME configured for another system size");
   GatherRec* grec = (GatherRec*)c->pme_grec.p;
   FftPlan plz, ply;
@@ -1270,3 +1287,118 @@ int mdx_pme_enqueue(mdx_ctx* c, bool energy) {
   return 0;
 }
 
+
+// ---------------------------------------------------------------- batched replicas
+// The three plane-path launches of every replica of a batch: each replica's arguments are the
+// record a single-context launch passes by value.
+struct PlaneCall {
+  PlaneArgs a;
+  FftPlan plz, ply;
+  const float2* twz;
+  const float2* twy;
+  const int* sorted_orig;
+  const int* cell_start;
+  const double* pos_sorted;
+  const float4* xq;
+  float2* cg;
+  GatherRec* grec;
+  int* grec_n;
+  double* fpart;
+  DevState* st;
+};
+struct FftCall {
+  FftPlan pl;
+  FftLines ln;
+  const float2* tw;
+  float2* cg;
+  long long* qfix;
+  float* rout;
+  const float* bc;
+  long long* eslot;
+};
+
+__global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_fwd_batch(BatchMap map, const PlaneCall* __restrict__ calls) {
+  extern __shared__ float2 lds[];
+  int x0;
+  const PlaneCall& p = calls[batch_replica(map, &x0)];
+  plane_fwd_body(p.a, p.plz, p.ply, p.twz, p.twy, p.sorted_orig, p.cell_start, p.pos_sorted, p.xq, p.cg, p.grec,
+                 p.grec_n, p.st, x0, lds);
+}
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_batch(BatchMap map, const FftCall* __restrict__ calls) {
+  extern __shared__ float2 lds[];
+  int ch;
+  const FftCall& f = calls[batch_replica(map, &ch)];
+  fft_axis_body<-1, 3>(f.pl, f.ln, f.tw, f.cg, f.qfix, f.rout, f.bc, f.eslot, ch * f.ln.B, lds);
+}
+__global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_inv_gather_batch(BatchMap map,
+                                                                             const PlaneCall* __restrict__ calls) {
+  extern __shared__ float2 lds[];
+  int x0;
+  const PlaneCall& p = calls[batch_replica(map, &x0)];
+  plane_gather_body(p.a, p.plz, p.ply, p.twz, p.twy, p.sorted_orig, p.cell_start, p.pos_sorted, p.xq, p.cg, p.grec,
+                    p.grec_n, p.fpart, p.st, x0, lds);
+}
+
+int mdx_batch_prepare_pme(mdx_ctx* const* cs, int n, BatchKind* fwd, BatchKind* fft, BatchKind* gat) {
+  std::vector<PlaneCall> pc;
+  std::vector<FftCall> fc;
+  std::vector<int> bp, bf;
+  size_t lf = 0, lx = 0, lg = 0;
+  for (int r = 0; r < n; ++r) {
+    mdx_ctx* c = cs[r];
+    MDX_CHECK(c->pme_configured && c->pme_planes && c->nb_configured && c->has_charges, MDX_ERR_INVALID,
+              "replica %d: batched steps need the PME plane path", r);
+    PlaneCall p;
+    plane_args_of(c, &p.a);
+    MDX_CHECK(c->pme_grec.n >= (size_t)c->K[0] * p.a.rec_cap * 6 && c->fpart.p && c->fpart.n >= 12 * (size_t)c->n,
+              MDX_ERR_INVALID, "replica %d: PME configured for another system size", r);
+    factorize(c->K[2], &p.plz);
+    factorize(c->K[1], &p.ply);
+    const int Kmax = std::max(c->K[0], std::max(c->K[1], c->K[2]));
+    p.twy = c->twiddle.p + (size_t)Kmax;
+    p.twz = c->twiddle.p + 2 * (size_t)Kmax;
+    p.sorted_orig = c->sorted_orig.p;
+    p.cell_start = c->cell_start.p;
+    p.pos_sorted = c->pos_sorted.p;
+    p.xq = c->xq.p;
+    p.cg = c->cgrid.p;
+    p.grec = (GatherRec*)c->pme_grec.p;
+    p.grec_n = c->pme_grec_n.p;
+    p.fpart = c->fpart.p;
+    p.st = c->dstate.p;
+    pc.push_back(p);
+    bp.push_back(c->K[0]);
+    lf = std::max(lf, plane_fwd_lds(c->K[1], c->K[2], p.a.W));
+    lg = std::max(lg, plane_gather_lds(c->K[1], c->K[2]));
+    FftCall f;
+    size_t l;
+    fft_axis_of(c, 0, &f.pl, &f.ln, &f.tw, &l);
+    f.cg = c->cgrid.p;
+    f.qfix = c->qfix.p;
+    f.rout = c->conv.p;
+    f.bc = c->bc.p;
+    f.eslot = c->efix.p + 2;
+    fc.push_back(f);
+    bf.push_back(div_up(f.ln.nlines, f.ln.B));
+    lx = std::max(lx, l);
+  }
+  MDX_HIP(hipFuncSetAttribute((const void*)k_pme_plane_fwd_batch, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lf));
+  MDX_HIP(hipFuncSetAttribute((const void*)k_fft_axis_batch, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lx));
+  MDX_HIP(hipFuncSetAttribute((const void*)k_pme_plane_inv_gather_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lg));
+  int rc;
+  if ((rc = batch_kind_upload(fwd, pc, bp, lf)) || (rc = batch_kind_upload(fft, fc, bf, lx)) ||
+      (rc = batch_kind_upload(gat, pc, bp, lg)))
+    return rc;
+  return 0;
+}
+
+int mdx_batch_launch_pme(const BatchKind& fwd, const BatchKind& fft, const BatchKind& gat, hipStream_t s) {
+  k_pme_plane_fwd_batch<<<fwd.blocks, PLANE_THREADS, fwd.lds, s>>>(fwd.map, (const PlaneCall*)fwd.calls);
+  MDX_LAUNCH_CHECK();
+  k_fft_axis_batch<<<fft.blocks, FFT_THREADS, fft.lds, s>>>(fft.map, (const FftCall*)fft.calls);
+  MDX_LAUNCH_CHECK();
+  k_pme_plane_inv_gather_batch<<<gat.blocks, PLANE_THREADS, gat.lds, s>>>(gat.map, (const PlaneCall*)gat.calls);
+  MDX_LAUNCH_CHECK();
+  return 0;
+}

@@ -2,15 +2,20 @@
 
 Each replica is a full system (its own box, neighbour-list grid, PME grid, integrator and
 Philox stream) held by its own libmdx context; a batch (``mdx_batch_*``) advances all of them
-in one of three ways:
+in one of these ways:
 
+``fused``       every kernel of a step is one launch over all replicas' workgroups (per-replica
+                arguments in device tables), the steps replayed from hipGraphs: the batch mode
+                for many replicas (no replica-count limit);
 ``streams``     every replica replays its own graph on its own stream, launches interleaved;
 ``sequential``  replica after replica (the per-replica loop the batch is measured against);
 ``one_graph``   one hipGraph with a branch per replica — disabled in the library for now
                 (its first GPU run ended in a host-side crash whose cause is not known).
 
-The autotuner picks, per replica, the neighbour-list skin that minimises its step time, and
-then the batch mode with the highest aggregate atom-steps/s.
+``streams`` and ``sequential`` capture graphs per replica and are limited to ``MAX_REPLICAS``
+(a 64-replica run of them crashed on the host, cause not known). ``autotune_fused`` picks the
+list skin (hence the cell grids) and the nonbonded lanes per atom of the fused batch;
+``autotune_skins`` picks a skin per replica for the per-replica modes.
 """
 from __future__ import annotations
 
@@ -36,16 +41,14 @@ def maxwell_velocities(masses, temperature, seed):
 class ReplicaBatch:
     """Independent replicas on one device, advanced by one batch call."""
 
-    # A 64-replica run (bench replicas leg) ended in a host-side segmentation fault whose cause
-    # has not been identified; batches stay at the size the GPU tests cover until it is.
+    # A 64-replica run of the per-replica modes (bench replicas leg) ended in a host-side
+    # segmentation fault whose cause has not been identified; those modes stay at the size the
+    # GPU tests cover. The fused mode captures no per-replica graphs and has no such limit.
     MAX_REPLICAS = 8
 
     def __init__(self, systems, device=0, skins=None, temperature=300.0, friction=1e-3, dt=1.0, seed=0,
                  terms=TERMS_WATER, energy_every=50):
         import torch
-        if len(systems) > self.MAX_REPLICAS:
-            raise ValueError("ReplicaBatch is limited to %d replicas for now (a 64-replica run crashed on the "
-                             "host; cause under investigation)" % self.MAX_REPLICAS)
         self.torch = torch
         self.systems = list(systems)
         self.terms = int(terms)
@@ -74,10 +77,13 @@ class ReplicaBatch:
         stream = self.torch.cuda.current_stream().cuda_stream
         L.call("mdx_batch_create", arr, len(self.engines), ctypes.c_void_p(stream), ctypes.byref(self._batch))
 
-    def integrate(self, nsteps, mode="streams"):
+    def integrate(self, nsteps, mode="fused"):
+        if mode != "fused" and len(self.engines) > self.MAX_REPLICAS:
+            raise ValueError("batch mode %r is limited to %d replicas (a 64-replica run crashed on the host; "
+                             "cause not known): use mode='fused'" % (mode, self.MAX_REPLICAS))
         L.call("mdx_batch_integrate", self._batch, self.terms, int(nsteps), L.BATCH_MODES[mode])
 
-    def time(self, nsteps, mode="streams"):
+    def time(self, nsteps, mode="fused"):
         """Wall seconds for nsteps of every replica (synchronised on both sides)."""
         self.torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -85,7 +91,7 @@ class ReplicaBatch:
         self.torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    def atom_steps_per_s(self, nsteps, mode="streams"):
+    def atom_steps_per_s(self, nsteps, mode="fused"):
         return self.n_atoms * nsteps / self.time(nsteps, mode)
 
     def status(self):
@@ -110,6 +116,32 @@ class ReplicaBatch:
             best.append(candidates[k])
         self.skins = best
         return best
+
+    def set_lanes(self, lanes):
+        """Nonbonded lanes per atom of every replica (16, 32 or 64; the fused mode needs one value)."""
+        for e in self.engines:
+            e.tune_nonbonded(L.TERM_LJ | L.TERM_COUL_DIRECT, int(lanes))
+
+    def set_skin(self, skin):
+        """The neighbour-list skin of every replica (it sets the cell grids)."""
+        for e in self.engines:
+            e.configure_nonbonded(e.cutoff, skin=float(skin))
+        self.skins = [float(skin)] * len(self.engines)
+
+    def autotune_fused(self, skins=(1.0, 1.5, 2.0), lanes=(16, 32), steps=60, warmup=20):
+        """The (skin, lanes) pair with the highest aggregate atom-steps/s of the fused batch, each
+        candidate timed on the whole batch; the batch is left at the best one."""
+        rates = {}
+        for skin in skins:
+            self.set_skin(skin)
+            for g in lanes:
+                self.set_lanes(g)
+                self.integrate(warmup, "fused")
+                rates[(float(skin), int(g))] = self.atom_steps_per_s(steps, "fused")
+        best = max(rates, key=rates.get)
+        self.set_skin(best[0])
+        self.set_lanes(best[1])
+        return best, rates
 
     def autotune_mode(self, steps=100, warmup=20):
         """The batch mode with the highest aggregate atom-steps/s."""

@@ -98,3 +98,79 @@ def test_batch_repeated_calls_and_autotune():
     torch.cuda.synchronize()
     assert all(st == 0 for st in b.status())
     b.close()
+
+
+def _run_batch(sys_list, calls, mode, every=10, kind=None):
+    """positions, velocities and energy histories after each call: the batch in `mode`, or each
+    replica alone ('single': its own integrate calls, direct launches)."""
+    import torch
+    from mdpy_amd import _lib as L
+    from mdpy_amd.replicas import ReplicaBatch
+    b = ReplicaBatch(sys_list, energy_every=every)
+    if kind == "vv":
+        for e in b.engines:
+            e.configure_integrator(L.INT_VELOCITY_VERLET, dt=1.0)
+    out = []
+    for n in calls:
+        if mode == "single":
+            for e in b.engines:
+                e.integrate(b.terms, n, False, every)
+        else:
+            b.integrate(n, mode)
+        torch.cuda.synchronize()
+        out.append([(e.positions().copy(), e.velocities().copy(), e.energy_history().copy()) for e in b.engines])
+    status = b.status()
+    b.close()
+    return out, status
+
+
+@pytest.mark.parametrize("kind", ["gjf", "vv"])
+def test_fused_batch_bitwise_equal_to_single_runs(kind):
+    """MDX_BATCH_FUSED (one launch per kernel per step over all replicas) against each replica
+    advanced alone: two calls (30 and 17 steps, energies every 10), positions, velocities and
+    energy histories equal bit for bit, for G-JF Langevin and for velocity Verlet."""
+    sys_list = _systems(5)
+    say("case: fused batch vs single runs (%s), sizes %s, calls 30 + 17" % (kind, [s.num_atoms for s in sys_list]))
+    fused, sf = _run_batch(sys_list, (30, 17), "fused", kind=kind)
+    single, ss = _run_batch(sys_list, (30, 17), "single", kind=kind)
+    assert all(v == 0 for v in sf) and all(v == 0 for v in ss)
+    for k in range(2):
+        for r in range(len(sys_list)):
+            for a, c in zip(fused[k][r], single[k][r]):
+                assert np.array_equal(a, c), (k, r)
+    assert all(h.shape[0] == 2 and np.all(np.isfinite(h)) for _, _, h in fused[1])
+    moved = [np.abs(p - s.positions).max() for (p, _, _), s in zip(fused[1], sys_list)]
+    assert all(m > 1e-3 for m in moved)
+
+
+def test_fused_batch_64_replicas_equal_single_runs():
+    """All 64 replicas of the batched-replica workload (300-3,000 waters each, 321,897 atoms) in
+    one fused batch: 20 steps equal each replica's own run bit for bit."""
+    from mdpy_amd import systems
+    reps = systems.water_replicas(64, seed=0)
+    say("case: fused batch, 64 replicas, %d atoms, 20 steps" % sum(s.num_atoms for s in reps))
+    fused, sf = _run_batch(reps, (20,), "fused")
+    single, ss = _run_batch(reps, (20,), "single")
+    assert all(v == 0 for v in sf) and all(v == 0 for v in ss)
+    for r in range(len(reps)):
+        for a, c in zip(fused[0][r], single[0][r]):
+            assert np.array_equal(a, c), r
+
+
+def test_fused_batch_autotune_and_repeated_calls():
+    """The fused batch's (skin, lanes) autotuner over a few candidates, then repeated calls that
+    replay the captured graphs (and recapture after a setting changes)."""
+    import torch
+    from mdpy_amd.replicas import ReplicaBatch
+    sys_list = _systems(3)
+    say("case: fused batch autotune, sizes", [s.num_atoms for s in sys_list])
+    b = ReplicaBatch(sys_list, energy_every=10)
+    best, rates = b.autotune_fused(skins=(1.0, 1.5), lanes=(16, 32), steps=20, warmup=10)
+    say("  rates", {k: "%.3g" % v for k, v in rates.items()}, "best", best)
+    assert best in rates and len(rates) == 4 and all(v > 0 for v in rates.values())
+    for n in (20, 33, 1, 0, 12):
+        b.integrate(n, "fused")
+    torch.cuda.synchronize()
+    assert all(st == 0 for st in b.status())
+    assert all(np.all(np.isfinite(e.positions())) for e in b.engines)
+    b.close()
