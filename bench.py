@@ -228,6 +228,7 @@ def replicas_leg(steps=200):
     progress("replicas: fused batch at skin %.1f A, %d lanes per atom" % best)
     b.integrate(40, "fused")
     fused = b.atom_steps_per_s(steps, "fused")
+    rooflines = batch_rooflines(b, reps)
     seq_steps = 50
     for e in b.engines:  # warm: list built at the chosen skin, lanes set
         e.integrate(b.terms, 10, False, 50)
@@ -253,7 +254,45 @@ def replicas_leg(steps=200):
             "sequential_per_replica_loop": sequential, "speedup_vs_sequential": fused / sequential,
             "autotuned": {"skin_A": best[0], "lanes": best[1],
                           "candidates": {"%.1f/%d" % k: v for k, v in tune.items()}},
-            "first_8_replicas": r8, "device_flags": flags}
+            "first_8_replicas": r8, "rooflines": rooflines, "device_flags": flags}
+
+
+def batch_rooflines(b, reps):
+    """Rooflines of the fused batch's kernels, each launch covering all 64 replicas (their
+    working set, ~3.2e5 atoms and 64 grids, exceeds the single box's cache-resident one), timed
+    alone with HIP events on the batch's stream; bytes are the per-replica rules of the single-box
+    kernels summed over the replicas, pairs the unique in-cutoff pairs."""
+    n = b.n_atoms
+    slots = pairs = 0
+    for e in b.engines:
+        sl, pr = e.pair_counts()
+        slots += sl
+        pairs += pr
+    k3 = fftb = 0
+    for s in reps:
+        kx, ky, kz = (int(k) for k in s.pme_grid)
+        k3 += kx * ky * kz
+        fftb += 2 * (4 * kx * ky * kz + 8 * ky * kz * (kx // 2 + 1))
+    items = [
+        ("k_nonbonded_batch", 0, 20, None),
+        ("k_step_pro_batch", 4, 20, 424 * n),
+        ("k_build_list_batch", 5, 5, 4 * slots + 24 * n),
+        ("k_pme_plane_fwd_batch", 6, 20, 16 * n + 4 * k3),
+        ("k_fft_axis_batch", 7, 20, fftb),
+        ("k_pme_plane_inv_gather_batch", 8, 20, 40 * n + 4 * k3),
+    ]
+    out = {"atoms": n, "list_slots": slots, "unique_pairs": pairs, "grid_points": k3}
+    for name, which, nrep, nbytes in items:
+        ms = b.time_kernel(which, reps=nrep)
+        if nbytes is None:
+            tf = pairs * FLOPS_PER_PAIR / (ms * 1e-3) / 1e12
+            out[name] = {"bound": "valu", "kernel_ms": ms, "achieved": tf, "peak": 157.3, "unit": "TFLOP/s",
+                         "frac": tf / 157.3, "flops_per_pair": FLOPS_PER_PAIR}
+        else:
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            out[name] = {"bound": "hbm", "kernel_ms": ms, "bytes": nbytes, "achieved": gbs, "peak": PEAK_HBM_GBPS,
+                         "unit": "GB/s", "frac": gbs / PEAK_HBM_GBPS}
+    return out
 
 
 PEAK_HBM_GBPS = 8000.0

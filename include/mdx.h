@@ -240,6 +240,12 @@ int mdx_batch_create(mdx_ctx** ctxs, int n_ctx, void* hip_stream, mdx_batch** ou
  * replicas need the same energy interval). Energy histories are per context, as after
  * mdx_integrate. */
 int mdx_batch_integrate(mdx_batch* batch, unsigned terms, int nsteps, int mode);
+/* Average ms per launch of one batched kernel of MDX_BATCH_FUSED, alone over every replica (HIP
+ * events on the batch's stream), after a fused integrate call: which 0 nonbonded (forces only),
+ * 4 step prologue (the replicas' state saved and restored), 5 list build of a rebuild step,
+ * 6 / 7 / 8 PME plane forward / x-axis pass / inverse + gather. The next call re-evaluates the
+ * replicas' forces. */
+int mdx_batch_time_kernel(mdx_batch* batch, int which, int reps, double* h_avg_ms);
 int mdx_batch_destroy(mdx_batch* batch);
 
 /* ---- learned surrogate (new; no reference counterpart) ---------------------------------- */

@@ -110,6 +110,7 @@ SIGNATURES = {
     "mdx_nonbonded_persist": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mdx_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _vp, ctypes.POINTER(_vp)]),
     "mdx_batch_integrate": (ctypes.c_int, [_vp, ctypes.c_uint, ctypes.c_int, ctypes.c_int]),
+    "mdx_batch_time_kernel": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _c_dbl_p]),
     "mdx_batch_destroy": (ctypes.c_int, [_vp]),
     "mdx_mlp_create": (ctypes.c_int, [_vp, _c_int_p, ctypes.c_double, ctypes.c_double, ctypes.POINTER(_vp)]),
     "mdx_mlp_destroy": (ctypes.c_int, [_vp]),

@@ -2322,31 +2322,52 @@ extern "C" int mdx_nonbonded_persist(mdx_ctx* c, int per_cu) {
 // The step prologue (G-JF or velocity Verlet, folding the previous velocity half) timed alone:
 // the state it writes is saved first and put back after, no rebuild is decided and nothing is
 // recorded, so the trajectory continues as if it had not run.
-static int time_prologue(mdx_ctx* c, int reps, double* h_avg_ms) {
+// The device state a timed prologue launch overwrites (saved before the timing, restored after)
+struct StatePart {
+  void* p;
+  size_t bytes;
+};
+static const int kProStateParts = 9;
+static int prologue_state_parts(mdx_ctx* c, StatePart* parts) {
   MDX_CHECK(c->int_configured && c->acc_prev.p && c->gsave.p && c->xq.p && c->pos_sorted.p &&
                 (c->int_kind == MDX_INT_LANGEVIN || c->int_kind == MDX_INT_VELOCITY_VERLET),
             MDX_ERR_INVALID, "prologue timing needs a G-JF or velocity-Verlet context with a list grid");
   const size_t n3 = 3 * (size_t)c->n;
-  struct Part {
-    void* p;
-    size_t bytes;
-  } parts[] = {{c->pos, n3 * 8}, {c->vel, n3 * 8}, {c->acc_prev.p, n3 * 8}, {c->gsave.p, n3 * 8},
-               {c->ffix.p, n3 * 8}, {c->efix.p, (MDX_NTERMS + 1) * 8}, {c->dstate.p, sizeof(DevState)},
-               {c->xq.p, (size_t)c->n * sizeof(float4)}, {c->pos_sorted.p, n3 * 8}};
+  const StatePart q[kProStateParts] = {
+      {c->pos, n3 * 8},        {c->vel, n3 * 8},  {c->acc_prev.p, n3 * 8},
+      {c->gsave.p, n3 * 8},    {c->ffix.p, n3 * 8}, {c->efix.p, (MDX_NTERMS + 1) * 8},
+      {c->dstate.p, sizeof(DevState)}, {c->xq.p, (size_t)c->n * sizeof(float4)}, {c->pos_sorted.p, n3 * 8}};
+  for (int k = 0; k < kProStateParts; ++k) parts[k] = q[k];
+  return 0;
+}
+static size_t state_bytes(const StatePart* parts) {
   size_t total = 0;
-  for (const Part& q : parts) total += (q.bytes + 255) & ~(size_t)255;
+  for (int k = 0; k < kProStateParts; ++k) total += (parts[k].bytes + 255) & ~(size_t)255;
+  return total;
+}
+// copy the parts into `save` (or back from it) on stream s
+static int state_copy(const StatePart* parts, char* save, bool restore, hipStream_t s) {
+  size_t off = 0;
+  for (int k = 0; k < kProStateParts; ++k) {
+    if (restore) MDX_HIP(hipMemcpyAsync(parts[k].p, save + off, parts[k].bytes, hipMemcpyDeviceToDevice, s));
+    else MDX_HIP(hipMemcpyAsync(save + off, parts[k].p, parts[k].bytes, hipMemcpyDeviceToDevice, s));
+    off += (parts[k].bytes + 255) & ~(size_t)255;
+  }
+  return 0;
+}
+
+static int time_prologue(mdx_ctx* c, int reps, double* h_avg_ms) {
+  StatePart parts[kProStateParts];
+  int rc0 = prologue_state_parts(c, parts);
+  if (rc0) return rc0;
+  const size_t total = state_bytes(parts);
   struct Save {  // freed on every return (hipFree waits for the device)
     DevBuf<char> b;
     ~Save() { b.release(); }
   } keep;
   DevBuf<char>& save = keep.b;
   int rc = save.ensure(total);
-  if (rc) return rc;
-  size_t off = 0;
-  for (const Part& q : parts) {
-    MDX_HIP(hipMemcpyAsync(save.p + off, q.p, q.bytes, hipMemcpyDeviceToDevice, c->stream));
-    off += (q.bytes + 255) & ~(size_t)255;
-  }
+  if (rc || (rc = state_copy(parts, save.p, false, c->stream))) return rc;
   ProCall pc = pro_call(c);
   pc.P.bin = 0;
   pc.P.record = 0;
@@ -2365,11 +2386,7 @@ static int time_prologue(mdx_ctx* c, int reps, double* h_avg_ms) {
   }
   MDX_LAUNCH_CHECK();
   MDX_HIP(hipEventRecord(e1, c->stream));
-  off = 0;
-  for (const Part& q : parts) {
-    MDX_HIP(hipMemcpyAsync(q.p, save.p + off, q.bytes, hipMemcpyDeviceToDevice, c->stream));
-    off += (q.bytes + 255) & ~(size_t)255;
-  }
+  if ((rc = state_copy(parts, save.p, true, c->stream))) return rc;
   MDX_HIP(hipStreamSynchronize(c->stream));
   float ms = 0;
   MDX_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -2439,4 +2456,110 @@ extern "C" int mdx_time_kernel(mdx_ctx* c, int which, unsigned terms, int reps, 
   (void)hipEventDestroy(e1);
   *h_avg_ms = ms / reps;
   MDX_LEAVE_RET(rc);
+}
+
+// One batched kernel kind of MDX_BATCH_FUSED timed alone over every replica (HIP events on the
+// lead stream): which 0 nonbonded (forces only), 4 step prologue (state saved and restored),
+// 5 list build of a rebuild step, 6 / 7 / 8 PME plane forward / x-axis pass / inverse + gather.
+// Uses the tables of the last fused call; the replicas' forces are re-evaluated by the next call.
+extern "C" int mdx_batch_time_kernel(mdx_batch* b, int which, int reps, double* h_avg_ms) {
+  MDX_CHECK(b && h_avg_ms && reps > 0, MDX_ERR_INVALID, "mdx_batch_time_kernel: bad arguments");
+  MDX_CHECK(b->fz.ready, MDX_ERR_INVALID, "mdx_batch_time_kernel: run a MDX_BATCH_FUSED integrate call first");
+  MDX_CHECK(which == 0 || (which >= 4 && which <= 8), MDX_ERR_INVALID, "mdx_batch_time_kernel: unknown kernel %d",
+            which);
+  MDX_HIP(hipSetDevice(b->ctx[0]->device));
+  BatchFused& f = b->fz;
+  hipStream_t s = b->lead;
+  MDX_HIP(hipEventRecord(b->ev_in, b->ext));
+  MDX_HIP(hipStreamWaitEvent(s, b->ev_in, 0));
+  for (mdx_ctx* c : b->ctx) c->forces_current = false;  // the timed launches accumulate into the forces
+  struct Keep {  // freed on every return (hipFree waits for the device)
+    DevBuf<char> save;
+    BatchKind pro;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Keep() {
+      save.release();
+      pro.buf.release();
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } k;
+  MDX_HIP(hipEventCreate(&k.e0));
+  MDX_HIP(hipEventCreate(&k.e1));
+  int rc = 0;
+  double total_ms = 0;
+  std::vector<StatePart> parts;
+  if (which == 4) {  // the prologue alone: no binning, no energy record, no rebuild decision
+    const int n = (int)b->ctx.size();
+    parts.resize((size_t)n * kProStateParts);
+    size_t bytes = 0;
+    std::vector<ProCall> calls;
+    std::vector<int> blocks;
+    for (int r = 0; r < n && !rc; ++r) {
+      mdx_ctx* c = b->ctx[r];
+      if ((rc = prologue_state_parts(c, &parts[(size_t)r * kProStateParts]))) break;
+      bytes += state_bytes(&parts[(size_t)r * kProStateParts]);
+      ProCall pc = pro_call(c);
+      pc.P.bin = 0;
+      pc.P.record = 0;
+      pc.P.thr2 = HUGE_VAL;
+      calls.push_back(pc);
+      blocks.push_back(pc.blocks);
+    }
+    if (rc || (rc = k.save.ensure(bytes)) || (rc = batch_kind_upload(&k.pro, calls, blocks, 0))) return rc;
+    size_t off = 0;
+    for (int r = 0; r < n && !rc; ++r) {
+      rc = state_copy(&parts[(size_t)r * kProStateParts], k.save.p + off, false, s);
+      off += state_bytes(&parts[(size_t)r * kProStateParts]);
+    }
+    if (rc) return rc;
+  }
+  if (which != 5) MDX_HIP(hipEventRecord(k.e0, s));
+  for (int rep = 0; rep < reps && !rc; ++rep) {
+    if (which == 0) {
+      rc = mdx_batch_launch_nb(f.nb, f.lanes, f.s14, false, s);
+    } else if (which == 4) {
+      const ProCall* pc = (const ProCall*)k.pro.calls;
+      if (f.gjf) k_step_pro_batch<true, true><<<k.pro.blocks, PRO_THREADS, 0, s>>>(k.pro.map, pc);
+      else k_step_pro_batch<false, true><<<k.pro.blocks, PRO_THREADS, 0, s>>>(k.pro.map, pc);
+      MDX_LAUNCH_CHECK();
+    } else if (which == 5) {  // each repetition rebuilds every replica's list from its cell order
+      for (mdx_ctx* c : b->ctx)
+        if ((rc = on_lead(b, c, [&]() { return mdx_enqueue_request_rebuild(c); }))) break;
+      if (rc) break;
+      MDX_HIP(hipEventRecord(k.e0, s));
+      if ((rc = mdx_batch_launch_list(f.lfast, f.lcell, s))) break;
+      MDX_HIP(hipEventRecord(k.e1, s));
+      MDX_HIP(hipEventSynchronize(k.e1));
+      float ms = 0;
+      MDX_HIP(hipEventElapsedTime(&ms, k.e0, k.e1));
+      total_ms += ms;
+    } else {
+      rc = mdx_batch_launch_pme(f.pfwd, f.pfft, f.pgat, s, 1u << (which - 6));
+    }
+  }
+  if (rc) return rc;
+  if (which != 5) {
+    MDX_HIP(hipEventRecord(k.e1, s));
+    MDX_HIP(hipEventSynchronize(k.e1));
+    float ms = 0;
+    MDX_HIP(hipEventElapsedTime(&ms, k.e0, k.e1));
+    total_ms = ms;
+  }
+  if (which == 4) {
+    size_t off = 0;
+    for (size_t r = 0; r < b->ctx.size() && !rc; ++r) {
+      rc = state_copy(&parts[r * kProStateParts], k.save.p + off, true, s);
+      off += state_bytes(&parts[r * kProStateParts]);
+    }
+    if (rc) return rc;
+  }
+  if (which == 5)  // the next evaluation rebuilds in full (binning included)
+    for (mdx_ctx* c : b->ctx)
+      if ((rc = on_lead(b, c, [&]() { return mdx_enqueue_request_rebuild(c); }))) return rc;
+  MDX_HIP(hipStreamSynchronize(s));
+  MDX_HIP(hipEventRecord(b->ev_out[0], s));
+  MDX_HIP(hipStreamWaitEvent(b->ext, b->ev_out[0], 0));
+  *h_avg_ms = total_ms / reps;
+  return 0;
 }

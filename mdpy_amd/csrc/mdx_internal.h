@@ -489,7 +489,8 @@ int mdx_batch_prepare_list(mdx_ctx* const* cs, int n, BatchKind* fill, BatchKind
 int mdx_batch_launch_fill(const BatchKind& k, hipStream_t s);
 int mdx_batch_launch_list(const BatchKind& fast, const BatchKind& cell, hipStream_t s);
 int mdx_batch_prepare_pme(mdx_ctx* const* cs, int n, BatchKind* fwd, BatchKind* fft, BatchKind* gat);
-int mdx_batch_launch_pme(const BatchKind& fwd, const BatchKind& fft, const BatchKind& gat, hipStream_t s);
+int mdx_batch_launch_pme(const BatchKind& fwd, const BatchKind& fft, const BatchKind& gat, hipStream_t s,
+                         unsigned parts = 7u);  // bits: forward, x-axis pass, inverse + gather
 int mdx_batch_prepare_nb(mdx_ctx* const* cs, int n, unsigned bonded_terms, BatchKind* out);
 int mdx_batch_launch_nb(const BatchKind& k, int lanes, bool s14, bool energy, hipStream_t s);
 // Rebuild the nonbonded kernel's Ewald-correction table when beta or the cutoff changed.

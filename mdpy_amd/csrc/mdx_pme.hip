@@ -1393,12 +1393,19 @@ int mdx_batch_prepare_pme(mdx_ctx* const* cs, int n, BatchKind* fwd, BatchKind* 
   return 0;
 }
 
-int mdx_batch_launch_pme(const BatchKind& fwd, const BatchKind& fft, const BatchKind& gat, hipStream_t s) {
-  k_pme_plane_fwd_batch<<<fwd.blocks, PLANE_THREADS, fwd.lds, s>>>(fwd.map, (const PlaneCall*)fwd.calls);
-  MDX_LAUNCH_CHECK();
-  k_fft_axis_batch<<<fft.blocks, FFT_THREADS, fft.lds, s>>>(fft.map, (const FftCall*)fft.calls);
-  MDX_LAUNCH_CHECK();
-  k_pme_plane_inv_gather_batch<<<gat.blocks, PLANE_THREADS, gat.lds, s>>>(gat.map, (const PlaneCall*)gat.calls);
-  MDX_LAUNCH_CHECK();
+int mdx_batch_launch_pme(const BatchKind& fwd, const BatchKind& fft, const BatchKind& gat, hipStream_t s,
+                         unsigned parts) {
+  if (parts & 1u) {
+    k_pme_plane_fwd_batch<<<fwd.blocks, PLANE_THREADS, fwd.lds, s>>>(fwd.map, (const PlaneCall*)fwd.calls);
+    MDX_LAUNCH_CHECK();
+  }
+  if (parts & 2u) {
+    k_fft_axis_batch<<<fft.blocks, FFT_THREADS, fft.lds, s>>>(fft.map, (const FftCall*)fft.calls);
+    MDX_LAUNCH_CHECK();
+  }
+  if (parts & 4u) {
+    k_pme_plane_inv_gather_batch<<<gat.blocks, PLANE_THREADS, gat.lds, s>>>(gat.map, (const PlaneCall*)gat.calls);
+    MDX_LAUNCH_CHECK();
+  }
   return 0;
 }

@@ -94,6 +94,15 @@ class ReplicaBatch:
     def atom_steps_per_s(self, nsteps, mode="fused"):
         return self.n_atoms * nsteps / self.time(nsteps, mode)
 
+    def time_kernel(self, which, reps=20):
+        """Average ms per launch of one batched kernel of the fused mode over all replicas, alone:
+        0 nonbonded (forces only), 4 step prologue (state restored), 5 list build of a rebuild
+        step, 6 / 7 / 8 PME plane forward / x-axis pass / inverse + gather. Needs a fused call
+        first; the next call re-evaluates the forces."""
+        ms = ctypes.c_double()
+        L.call("mdx_batch_time_kernel", self._batch, int(which), int(reps), ctypes.byref(ms))
+        return ms.value
+
     def status(self):
         return [e.status() for e in self.engines]
 

@@ -174,3 +174,34 @@ def test_fused_batch_autotune_and_repeated_calls():
     assert all(st == 0 for st in b.status())
     assert all(np.all(np.isfinite(e.positions())) for e in b.engines)
     b.close()
+
+
+def test_fused_batch_kernel_timing_leaves_the_trajectory_unchanged():
+    """Timing the fused batch's nonbonded, prologue (state saved and restored) and PME launches
+    between two calls changes nothing: the next call re-evaluates the forces and follows the
+    untimed trajectory bit for bit. Timing the list build (a full rebuild on the next call)
+    leaves a valid, finite run."""
+    import torch
+    from mdpy_amd.replicas import ReplicaBatch
+    sys_list = _systems(3)
+    say("case: fused batch kernel timing between calls, sizes", [s.num_atoms for s in sys_list])
+    out = []
+    for timed in (False, True):
+        b = ReplicaBatch(sys_list, energy_every=10)
+        b.integrate(20, "fused")
+        if timed:
+            for which in (0, 4, 6, 7, 8):
+                assert b.time_kernel(which, reps=3) > 0
+        b.integrate(15, "fused")
+        torch.cuda.synchronize()
+        out.append([(e.positions().copy(), e.velocities().copy()) for e in b.engines])
+        if timed:
+            assert b.time_kernel(5, reps=2) > 0
+            b.integrate(10, "fused")
+            torch.cuda.synchronize()
+            assert all(np.all(np.isfinite(e.positions())) for e in b.engines)
+        assert all(st == 0 for st in b.status())
+        b.close()
+    for r in range(len(sys_list)):
+        for a, c in zip(out[0][r], out[1][r]):
+            assert np.array_equal(a, c), r
