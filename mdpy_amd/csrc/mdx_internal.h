@@ -136,6 +136,7 @@ struct BatchKind {
   const void* calls = nullptr;
   int blocks = 0;  // workgroups of the launch, all replicas
   size_t lds = 0;
+  int threads = 0;  // lanes per workgroup, where the kind chooses it
 };
 template <class Call>
 int batch_kind_upload(BatchKind* k, const std::vector<Call>& calls, const std::vector<int>& blocks, size_t lds) {

@@ -498,11 +498,12 @@ __device__ void plane_fwd_body(const PlaneArgs& a, const FftPlan& plz, const Fft
                                const int* __restrict__ sorted_orig, const int* __restrict__ cell_start,
                                const double* __restrict__ pos_sorted, const float4* __restrict__ xq,
                                float2* __restrict__ cg, GatherRec* __restrict__ grec, int* __restrict__ grec_n,
-                               DevState* st, int x0, float2* lds) {
+                               DevState* st, int x0, float2* lds, bool wave64 = true) {
   __shared__ int nrec;
   __shared__ int ngrec;
   const int P = a.Ky * a.Kz;
-  const bool p64 = a.Ky == 64 && a.Kz == 64;  // wave-level transforms on a padded plane
+  // wave-level transforms on a padded plane (they need PLANE_THREADS lanes)
+  const bool p64 = wave64 && a.Ky == 64 && a.Kz == 64;
   float2* twz = lds;
   float2* twy = lds + a.Kz;
   float2* A = twy + a.Ky;
@@ -639,10 +640,11 @@ __device__ void plane_gather_body(const PlaneArgs& a, const FftPlan& plz, const 
                                   const double* __restrict__ pos_sorted, const float4* __restrict__ xq,
                                   const float2* __restrict__ cg, const GatherRec* __restrict__ grec,
                                   const int* __restrict__ grec_n, double* __restrict__ fpart, DevState* st, int x0,
-                                  float2* lds) {
+                                  float2* lds, bool wave64 = true) {
   __shared__ int bcount;
   const int P = a.Ky * a.Kz;
-  const bool p64 = a.Ky == 64 && a.Kz == 64;  // wave-level transforms on a padded plane
+  // wave-level transforms on a padded plane (they need PLANE_THREADS lanes)
+  const bool p64 = wave64 && a.Ky == 64 && a.Kz == 64;
   float2* twz = lds;
   float2* twy = lds + a.Kz;
   float* convr = (float*)(twy + a.Ky);
@@ -1317,12 +1319,15 @@ struct FftCall {
   long long* eslot;
 };
 
-__global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_fwd_batch(BatchMap map, const PlaneCall* __restrict__ calls) {
+// Batched plane kernels of BS lanes: PLANE_THREADS, or fewer when no replica has 64 x 64 planes
+// (which need all PLANE_THREADS lanes for their wave-level transforms).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_pme_plane_fwd_batch(BatchMap map, const PlaneCall* __restrict__ calls) {
   extern __shared__ float2 lds[];
   int x0;
   const PlaneCall& p = calls[batch_replica(map, &x0)];
   plane_fwd_body(p.a, p.plz, p.ply, p.twz, p.twy, p.sorted_orig, p.cell_start, p.pos_sorted, p.xq, p.cg, p.grec,
-                 p.grec_n, p.st, x0, lds);
+                 p.grec_n, p.st, x0, lds, BS == PLANE_THREADS);
 }
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_batch(BatchMap map, const FftCall* __restrict__ calls) {
   extern __shared__ float2 lds[];
@@ -1330,26 +1335,35 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_batch(BatchMap map, co
   const FftCall& f = calls[batch_replica(map, &ch)];
   fft_axis_body<-1, 3>(f.pl, f.ln, f.tw, f.cg, f.qfix, f.rout, f.bc, f.eslot, ch * f.ln.B, lds);
 }
-__global__ __launch_bounds__(PLANE_THREADS) void k_pme_plane_inv_gather_batch(BatchMap map,
-                                                                             const PlaneCall* __restrict__ calls) {
+template <int BS>
+__global__ __launch_bounds__(BS) void k_pme_plane_inv_gather_batch(BatchMap map, const PlaneCall* __restrict__ calls) {
   extern __shared__ float2 lds[];
   int x0;
   const PlaneCall& p = calls[batch_replica(map, &x0)];
   plane_gather_body(p.a, p.plz, p.ply, p.twz, p.twy, p.sorted_orig, p.cell_start, p.pos_sorted, p.xq, p.cg, p.grec,
-                    p.grec_n, p.fpart, p.st, x0, lds);
+                    p.grec_n, p.fpart, p.st, x0, lds, BS == PLANE_THREADS);
 }
+// Without 64 x 64 planes the batched forward kernel runs 256-lane workgroups with a window of
+// 1,024 candidates (more planes in flight: 109 vs 133 us over the 64 water replicas); the gather
+// keeps PLANE_THREADS lanes (159 us, against 172 with 256 lanes and 180 with 512).
+#define PLANE_BATCH_SMALL 256
+#define PLANE_BATCH_W 1024
 
 int mdx_batch_prepare_pme(mdx_ctx* const* cs, int n, BatchKind* fwd, BatchKind* fft, BatchKind* gat) {
   std::vector<PlaneCall> pc;
   std::vector<FftCall> fc;
   std::vector<int> bp, bf;
   size_t lf = 0, lx = 0, lg = 0;
+  bool any64 = false;
+  for (int r = 0; r < n; ++r) any64 |= cs[r]->K[1] == 64 && cs[r]->K[2] == 64;
+  const int fthreads = any64 ? PLANE_THREADS : PLANE_BATCH_SMALL;
   for (int r = 0; r < n; ++r) {
     mdx_ctx* c = cs[r];
     MDX_CHECK(c->pme_configured && c->pme_planes && c->nb_configured && c->has_charges, MDX_ERR_INVALID,
               "replica %d: batched steps need the PME plane path", r);
     PlaneCall p;
     plane_args_of(c, &p.a);
+    if (!any64) p.a.W = std::min(p.a.W, PLANE_BATCH_W);
     MDX_CHECK(c->pme_grec.n >= (size_t)c->K[0] * p.a.rec_cap * 6 && c->fpart.p && c->fpart.n >= 12 * (size_t)c->n,
               MDX_ERR_INVALID, "replica %d: PME configured for another system size", r);
     factorize(c->K[2], &p.plz);
@@ -1382,21 +1396,31 @@ int mdx_batch_prepare_pme(mdx_ctx* const* cs, int n, BatchKind* fwd, BatchKind* 
     bf.push_back(div_up(f.ln.nlines, f.ln.B));
     lx = std::max(lx, l);
   }
-  MDX_HIP(hipFuncSetAttribute((const void*)k_pme_plane_fwd_batch, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lf));
+  const void* kf = fthreads == PLANE_THREADS ? (const void*)k_pme_plane_fwd_batch<PLANE_THREADS>
+                                             : (const void*)k_pme_plane_fwd_batch<PLANE_BATCH_SMALL>;
+  MDX_HIP(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lf));
   MDX_HIP(hipFuncSetAttribute((const void*)k_fft_axis_batch, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lx));
-  MDX_HIP(hipFuncSetAttribute((const void*)k_pme_plane_inv_gather_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lg));
+  MDX_HIP(hipFuncSetAttribute((const void*)k_pme_plane_inv_gather_batch<PLANE_THREADS>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lg));
   int rc;
   if ((rc = batch_kind_upload(fwd, pc, bp, lf)) || (rc = batch_kind_upload(fft, fc, bf, lx)) ||
       (rc = batch_kind_upload(gat, pc, bp, lg)))
     return rc;
+  fwd->threads = fthreads;
+  fft->threads = FFT_THREADS;
+  gat->threads = PLANE_THREADS;
   return 0;
 }
 
 int mdx_batch_launch_pme(const BatchKind& fwd, const BatchKind& fft, const BatchKind& gat, hipStream_t s,
                          unsigned parts) {
+  const bool big = fwd.threads == PLANE_THREADS;
   if (parts & 1u) {
-    k_pme_plane_fwd_batch<<<fwd.blocks, PLANE_THREADS, fwd.lds, s>>>(fwd.map, (const PlaneCall*)fwd.calls);
+    if (big)
+      k_pme_plane_fwd_batch<PLANE_THREADS><<<fwd.blocks, PLANE_THREADS, fwd.lds, s>>>(fwd.map, (const PlaneCall*)fwd.calls);
+    else
+      k_pme_plane_fwd_batch<PLANE_BATCH_SMALL><<<fwd.blocks, PLANE_BATCH_SMALL, fwd.lds, s>>>(fwd.map,
+                                                                                          (const PlaneCall*)fwd.calls);
     MDX_LAUNCH_CHECK();
   }
   if (parts & 2u) {
@@ -1404,7 +1428,8 @@ int mdx_batch_launch_pme(const BatchKind& fwd, const BatchKind& fft, const Batch
     MDX_LAUNCH_CHECK();
   }
   if (parts & 4u) {
-    k_pme_plane_inv_gather_batch<<<gat.blocks, PLANE_THREADS, gat.lds, s>>>(gat.map, (const PlaneCall*)gat.calls);
+    k_pme_plane_inv_gather_batch<PLANE_THREADS><<<gat.blocks, PLANE_THREADS, gat.lds, s>>>(
+        gat.map, (const PlaneCall*)gat.calls);
     MDX_LAUNCH_CHECK();
   }
   return 0;
