@@ -224,8 +224,10 @@ def replicas_leg(steps=200):
     reps = systems.water_replicas(64, seed=0)
     progress("replicas: 64 contexts, %d atoms" % sum(s.num_atoms for s in reps))
     b = ReplicaBatch(reps, energy_every=50)
-    best, tune = b.autotune_fused(skins=(1.0, 1.5, 2.0), lanes=(16, 32), steps=40, warmup=20)
-    progress("replicas: fused batch at skin %.1f A, %d lanes per atom" % best)
+    best, tune = b.autotune_fused(skins=(1.0, 1.5, 2.0), lanes=(16, 32), rebuild_every=(0, 8, 10, 12), steps=40,
+                                  warmup=20)
+    progress("replicas: fused batch at skin %.1f A, %d lanes per atom, lists rebuilt together every %d steps "
+             "(0: displacement test only)" % best)
     b.integrate(40, "fused")
     fused = b.atom_steps_per_s(steps, "fused")
     rooflines = batch_rooflines(b, reps)
@@ -243,6 +245,7 @@ def replicas_leg(steps=200):
     progress("replicas: 8-replica comparison with the per-replica graph modes")
     b8 = ReplicaBatch(reps[:8], energy_every=50, skins=[best[0]] * 8)
     b8.set_lanes(best[1])
+    b8.set_rebuild_every(best[2])
     r8 = {}
     for mode in ("fused", "streams", "sequential"):
         b8.integrate(20, mode)
@@ -252,8 +255,8 @@ def replicas_leg(steps=200):
     return {"metric": "aggregate atom-steps/s, 64 water replicas (fused batch)", "value": fused,
             "atoms": int(sum(s.num_atoms for s in reps)), "steps": steps,
             "sequential_per_replica_loop": sequential, "speedup_vs_sequential": fused / sequential,
-            "autotuned": {"skin_A": best[0], "lanes": best[1],
-                          "candidates": {"%.1f/%d" % k: v for k, v in tune.items()}},
+            "autotuned": {"skin_A": best[0], "lanes": best[1], "rebuild_every": best[2],
+                          "candidates": {"%.1f/%d/%d" % k: v for k, v in tune.items()}},
             "first_8_replicas": r8, "rooflines": rooflines, "device_flags": flags}
 
 

@@ -295,10 +295,17 @@ int mdx_time_kernel(mdx_ctx* ctx, int which, unsigned terms, int reps, double* h
  * keep the fastest. The choice goes to *h_lanes (may be NULL). Synchronises. */
 int mdx_nonbonded_tune(mdx_ctx* ctx, unsigned terms, int lanes, int reps, int* h_lanes);
 /* Resident nonbonded workgroups (no reference counterpart): per_cu > 0 launches per_cu
- * workgroups per compute unit that loop over the nonbonded work, so the PME kernels running
- * beside it find free registers on every compute unit (default 3); 0 launches one workgroup per
- * unit of work. Same forces either way. */
+ * workgroups per compute unit that loop over the nonbonded work (default 3, the measured best
+ * beside the concurrent PME kernels); 0 launches one workgroup per unit of work. Same forces
+ * either way. */
 int mdx_nonbonded_persist(mdx_ctx* ctx, int per_cu);
+/* Scheduled list rebuilds (no reference counterpart): every > 0 also rebuilds the neighbour
+ * list in the integrator step whose step counter is a multiple of `every`, besides the rebuilds
+ * the displacement test asks for (which still happen whenever an atom moves more than skin/2);
+ * 0 (default) rebuilds on the displacement test only. Replicas of a batch that share the
+ * interval rebuild together, so a batched step holds either all their list builds or none.
+ * Applies to the fused step prologue (G-JF and velocity Verlet on a list grid). */
+int mdx_nonbonded_rebuild_every(mdx_ctx* ctx, int every);
 /* Candidate (list) pair slots and in-cutoff unique pairs of the last nonbonded evaluation
  * (counts the list on the device; for roofline accounting; synchronises). */
 int mdx_pair_counts(mdx_ctx* ctx, int64_t* h_list_slots, int64_t* h_unique_in_cutoff);

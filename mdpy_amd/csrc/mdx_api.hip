@@ -286,6 +286,7 @@ struct StepPro {
   int* cell_count;
   int ncell;
   double thr2;
+  int rebuild_every;  // > 0: also rebuild on steps whose counter is a multiple of it
   // binning by the last workgroup (bin != 0): every lane stores its atom's list-grid cell; on a
   // rebuild step the last workgroup counts, scans and scatters the atoms into cell order
   int bin;
@@ -415,11 +416,16 @@ __device__ void prologue_block_end(const StepPro& P, DevState* st, double d2, bo
   if (threadIdx.x == 0) __threadfence();  // acquire for the block (device-scope fences act on whole caches)
   __syncthreads();
   const bool rebuild = atomicAdd(&st->rebuild, 0) != 0 ||
-                       !(dec_double_order(atomicAdd(&st->max_disp2_bits, 0ull)) <= P.thr2);
+                       !(dec_double_order(atomicAdd(&st->max_disp2_bits, 0ull)) <= P.thr2) ||
+                       (P.rebuild_every > 0 && st->step % P.rebuild_every == 0);
   if (rebuild) {
     if (P.bin) last_block_bin(P, st);
     else
       for (int k = threadIdx.x; k < P.ncell; k += blockDim.x) P.cell_count[k] = 0;
+    // the sort/fill and list-build kernels that follow test the flag and the displacement
+    // themselves; the flag carries this decision to them whatever made it (a scheduled rebuild
+    // included). The nonbonded kernel clears it.
+    if (threadIdx.x == 0) atomicExch(&st->rebuild, 1);
   }
   if (threadIdx.x == 0) {
     if (P.record) {
@@ -1302,6 +1308,7 @@ static StepPro step_pro(mdx_ctx* c) {
   P.ncell = c->ncell;
   const double half = 0.5 * c->skin;
   P.thr2 = c->skin > 0 ? half * half : -1.0;
+  P.rebuild_every = c->rebuild_every;
   // the last workgroup bins the atoms itself when the grid's counts fit its LDS
   P.bin = 2 * (size_t)c->ncell * sizeof(int) <= PRO_BIN_LDS_MAX ? 1 : 0;
   P.atom_cell = c->atom_cell.p;
@@ -2312,6 +2319,15 @@ extern "C" int mdx_nonbonded_tune(mdx_ctx* c, unsigned terms, int lanes, int rep
   return 0;
 }
 
+extern "C" int mdx_nonbonded_rebuild_every(mdx_ctx* c, int every) {
+  MDX_CHECK(c && every >= 0, MDX_ERR_INVALID, "mdx_nonbonded_rebuild_every: bad arguments");
+  if (every != c->rebuild_every) {
+    c->rebuild_every = every;
+    mdx_invalidate_graph(c);  // captured prologues hold the interval
+  }
+  return 0;
+}
+
 extern "C" int mdx_nonbonded_persist(mdx_ctx* c, int per_cu) {
   MDX_CHECK(c && per_cu >= 0, MDX_ERR_INVALID, "mdx_nonbonded_persist: bad arguments");
   c->nb_persist = c->ncu > 0 ? per_cu : 0;
@@ -2372,6 +2388,7 @@ static int time_prologue(mdx_ctx* c, int reps, double* h_avg_ms) {
   pc.P.bin = 0;
   pc.P.record = 0;
   pc.P.thr2 = HUGE_VAL;
+  pc.P.rebuild_every = 0;
   const bool gjf = c->int_kind == MDX_INT_LANGEVIN;
   dim3 gp(pc.blocks), bp(PRO_THREADS);
   hipEvent_t e0, e1;
@@ -2503,6 +2520,7 @@ extern "C" int mdx_batch_time_kernel(mdx_batch* b, int which, int reps, double* 
       pc.P.bin = 0;
       pc.P.record = 0;
       pc.P.thr2 = HUGE_VAL;
+      pc.P.rebuild_every = 0;
       calls.push_back(pc);
       blocks.push_back(pc.blocks);
     }

@@ -251,6 +251,7 @@ struct mdx_ctx {
   // nonbonded workgroups per compute unit looping over the work (0: one per unit of work); 3 (12
   // waves of 123 VGPRs) leaves a quarter of every SIMD's registers to the PME kernels beside it
   int nb_persist = 3;
+  int rebuild_every = 0;  // scheduled list rebuilds (step counter multiple), 0 = displacement test only
   int ncu = 0;                   // compute units of the device
   int list_stride = 0;
   bool list_valid = false;

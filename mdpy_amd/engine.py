@@ -273,6 +273,11 @@ class Engine:
         """Nonbonded workgroups per compute unit looping over the work (0: one per unit of work)."""
         L.call("mdx_nonbonded_persist", self.ctx, int(per_cu))
 
+    def set_rebuild_every(self, every):
+        """Also rebuild the neighbour list every `every` integrator steps (0: only when the
+        displacement test asks for it, which it still does either way)."""
+        L.call("mdx_nonbonded_rebuild_every", self.ctx, int(every))
+
     def time_kernel(self, which, terms, reps=20):
         """Average ms per launch (HIP events on the library stream): which 0 nonbonded,
         1 force evaluation, 2 PME reciprocal chain, 3 integrator step, 4 step prologue alone

@@ -137,19 +137,40 @@ class ReplicaBatch:
             e.configure_nonbonded(e.cutoff, skin=float(skin))
         self.skins = [float(skin)] * len(self.engines)
 
-    def autotune_fused(self, skins=(1.0, 1.5, 2.0), lanes=(16, 32), steps=60, warmup=20):
-        """The (skin, lanes) pair with the highest aggregate atom-steps/s of the fused batch, each
-        candidate timed on the whole batch; the batch is left at the best one."""
+    def set_rebuild_every(self, every):
+        """Rebuild every replica's list together every `every` steps, besides the rebuilds the
+        displacement test asks for (0: displacement test only)."""
+        for e in self.engines:
+            e.set_rebuild_every(every)
+        self.rebuild_every = int(every)
+
+    def autotune_fused(self, skins=(1.0, 1.5, 2.0), lanes=(16, 32), rebuild_every=(0, 8, 10, 12), steps=60,
+                       warmup=20):
+        """The (skin, lanes, rebuild interval) with the highest aggregate atom-steps/s of the fused
+        batch, each candidate timed on the whole batch ((skin, lanes) first, at the first interval
+        candidate, then the interval); the batch is left at the best one."""
         rates = {}
+
+        def timed(key):
+            self.integrate(warmup, "fused")
+            rates[key] = self.atom_steps_per_s(steps, "fused")
+
+        self.set_rebuild_every(rebuild_every[0])
         for skin in skins:
             self.set_skin(skin)
             for g in lanes:
                 self.set_lanes(g)
-                self.integrate(warmup, "fused")
-                rates[(float(skin), int(g))] = self.atom_steps_per_s(steps, "fused")
+                timed((float(skin), int(g), int(rebuild_every[0])))
+        skin, g, _ = max(rates, key=rates.get)
+        self.set_skin(skin)
+        self.set_lanes(g)
+        for every in rebuild_every[1:]:
+            self.set_rebuild_every(every)
+            timed((skin, g, int(every)))
         best = max(rates, key=rates.get)
         self.set_skin(best[0])
         self.set_lanes(best[1])
+        self.set_rebuild_every(best[2])
         return best, rates
 
     def autotune_mode(self, steps=100, warmup=20):

@@ -173,6 +173,31 @@ def test_graph_replay_on_fresh_context_reads_reciprocal_forces():
         assert np.array_equal(a, b)
 
 
+def test_scheduled_list_rebuilds_match_displacement_rebuilds():
+    """Lists also rebuilt every 4 steps (mdx_nonbonded_rebuild_every) against rebuilds on the
+    displacement test only: no device flag, more rebuilds, and the same trajectory up to the
+    float32 summation order of the PME spreading (the cell order changes at other steps)."""
+    from mdpy_amd import _lib as L, systems
+    from mdpy_amd.engine import Engine
+    say("case: scheduled list rebuilds vs displacement rebuilds, water648, G-JF 40 steps")
+    s = systems.water_box_small()
+    out = []
+    for every in (0, 4):
+        e = Engine(s, skin=1.0)
+        e.set_positions(s.positions)
+        e.set_velocities(thermal_v(s, 300.0, 2))
+        e.configure_integrator(L.INT_LANGEVIN, dt=1.0, temperature=300.0, friction=1e-3, seed=5)
+        e.set_rebuild_every(every)
+        e.integrate(L.TERM_LJ | L.TERM_PME | L.TERM_BOND | L.TERM_ANGLE, 40, use_graph=True, energy_every=10)
+        out.append((e.positions(), e.energy_history(), e.stats()["rebuilds"], e.status()))
+        e.close()
+    (p0, h0, r0, f0), (p1, h1, r1, f1) = out
+    assert f0 == 0 and f1 == 0
+    assert r1 >= 10 and r1 > r0  # every 4th of 40 steps, the displacement rebuilds pre-empted
+    assert np.abs(p0 - p1).max() < 1e-6
+    assert np.allclose(h0, h1, rtol=1e-6, atol=1e-6)
+
+
 def test_kernel_timing_leaves_the_trajectory_unchanged():
     """Timing the step prologue (state saved and restored) and the three PME plane launches
     between two integrate calls changes nothing: the next call starts from a fresh evaluation at

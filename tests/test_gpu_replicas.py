@@ -165,13 +165,13 @@ def test_fused_batch_autotune_and_repeated_calls():
     sys_list = _systems(3)
     say("case: fused batch autotune, sizes", [s.num_atoms for s in sys_list])
     b = ReplicaBatch(sys_list, energy_every=10)
-    best, rates = b.autotune_fused(skins=(1.0, 1.5), lanes=(16, 32), steps=20, warmup=10)
+    best, rates = b.autotune_fused(skins=(1.0, 1.5), lanes=(16, 32), rebuild_every=(0, 10), steps=20, warmup=10)
     say("  rates", {k: "%.3g" % v for k, v in rates.items()}, "best", best)
-    assert best in rates and len(rates) == 4 and all(v > 0 for v in rates.values())
+    assert best in rates and len(rates) == 5 and all(v > 0 for v in rates.values())
     for n in (20, 33, 1, 0, 12):
         b.integrate(n, "fused")
     torch.cuda.synchronize()
-    assert all(st == 0 for st in b.status())
+    assert all(st == 0 for st in b.status()), b.status()
     assert all(np.all(np.isfinite(e.positions())) for e in b.engines)
     b.close()
 
@@ -205,3 +205,35 @@ def test_fused_batch_kernel_timing_leaves_the_trajectory_unchanged():
     for r in range(len(sys_list)):
         for a, c in zip(out[0][r], out[1][r]):
             assert np.array_equal(a, c), r
+
+
+def test_fused_batch_with_scheduled_rebuilds_equal_single_runs():
+    """Lists rebuilt together every 7 steps (besides the displacement test): the fused batch and
+    each replica alone with the same interval stay bitwise equal, over calls that do not start on
+    a multiple of the interval."""
+    import torch
+    from mdpy_amd.replicas import ReplicaBatch
+    sys_list = _systems(4)
+    say("case: fused batch with lists rebuilt every 7 steps, sizes", [s.num_atoms for s in sys_list])
+    out = []
+    for mode in ("fused", "single"):
+        b = ReplicaBatch(sys_list, energy_every=10)
+        b.set_rebuild_every(7)
+        run = []
+        for n in (23, 12):
+            if mode == "single":
+                for e in b.engines:
+                    e.integrate(b.terms, n, False, 10)
+            else:
+                b.integrate(n, "fused")
+            torch.cuda.synchronize()
+            run.append([(e.positions().copy(), e.velocities().copy(), e.energy_history().copy()) for e in b.engines])
+        rebuilds = [e.stats()["rebuilds"] for e in b.engines]
+        assert all(st == 0 for st in b.status()), b.status()
+        b.close()
+        out.append((run, rebuilds))
+    assert out[0][1] == out[1][1] and all(r >= 5 for r in out[0][1])  # 35 steps: at least 5 scheduled
+    for k in range(2):
+        for r in range(len(sys_list)):
+            for a, c in zip(out[0][0][k][r], out[1][0][k][r]):
+                assert np.array_equal(a, c), (k, r)
