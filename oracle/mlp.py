@@ -54,14 +54,41 @@ def gauss(r, mu, eta, rd=R_D):
     return e * fc, e * (dfc - 2.0 * eta * (r - mu) * fc)
 
 
+DENSE_MAX = 4096  # atoms up to which the neighbour search is the dense N x N one
+
+
 def neighbor_pairs(pos, box, rd=R_D):
-    """All ordered pairs (i, j), i != j, with minimum-image |r_ij| < rd; r_ij = x_j - x_i."""
+    """All ordered pairs (i, j), i != j, with minimum-image |r_ij| < rd; r_ij = x_j - x_i,
+    sorted by (i, j). Dense for small systems, a periodic k-d tree beyond DENSE_MAX atoms (the
+    same pairs, order and arithmetic: tests/test_mlp_oracle.py)."""
+    if len(pos) > DENSE_MAX:
+        return neighbor_pairs_tree(pos, box, rd)
     n = len(pos)
     d = pos[None, :, :] - pos[:, None, :]
     d -= box * np.round(d / box)
     r = np.sqrt((d * d).sum(-1))
     i, j = np.nonzero((r < rd) & ~np.eye(n, dtype=bool))
     return i, j, d[i, j], r[i, j]
+
+
+def neighbor_pairs_tree(pos, box, rd=R_D):
+    """neighbor_pairs through scipy's periodic cKDTree: candidates within rd (1 + 1e-9) of the
+    wrapped positions, then the minimum-image vector and distance as the dense search forms them
+    and its strict r < rd."""
+    from scipy.spatial import cKDTree
+    box = np.asarray(box, dtype=np.float64)
+    w = pos - box * np.floor(pos / box)
+    w = np.where(w >= box, w - box, w)
+    pr = cKDTree(w, boxsize=box).query_pairs(rd * (1.0 + 1e-9), output_type="ndarray")
+    i = np.concatenate([pr[:, 0], pr[:, 1]])
+    j = np.concatenate([pr[:, 1], pr[:, 0]])
+    order = np.lexsort((j, i))
+    i, j = i[order], j[order]
+    d = pos[j] - pos[i]
+    d -= box * np.round(d / box)
+    r = np.sqrt((d * d).sum(-1))
+    keep = r < rd
+    return i[keep], j[keep], d[keep], r[keep]
 
 
 # ------------------------------------------------------------------ moment tensors (sym storage)
@@ -279,9 +306,10 @@ class Surrogate:
     def energy(self, pos, box, elem):
         return self._per_atom(pos, box, elem)[0].sum()
 
-    def forces(self, pos, box, elem):
+    def forces(self, pos, box, elem, atoms=None):
+        """Forces (complete rows for `atoms` only, when given) and the total energy."""
         E, gG, T, _, _ = self._per_atom(pos, box, elem)
-        return assemble_forces(pos, box, elem, gG, T, self.rd), E.sum()
+        return assemble_forces(pos, box, elem, gG, T, self.rd, atoms=atoms), E.sum()
 
     def loss_and_grad(self, pos, box, elem, f_ref):
         """L = sum |F_hat - F|^2 / (3N) and dL/dW for both element networks."""
@@ -299,14 +327,23 @@ class Surrogate:
         return L, grads, F
 
 
-def assemble_forces(pos, box, elem, gG, T, rd=R_D):
-    """F_j = sum_i dE_j/dr_ji - dE_i/dr_ij over neighbour pairs (gG = dE_i/dG_i)."""
+def assemble_forces(pos, box, elem, gG, T, rd=R_D, atoms=None, chunk=200000):
+    """F_j = sum_i dE_j/dr_ji - dE_i/dr_ij over neighbour pairs (gG = dE_i/dG_i). With `atoms`,
+    only the pairs that touch those atoms are summed: their rows are complete, the others not.
+    Pairs are taken in chunks (the Jacobian of a chunk is chunk x 96 x 3)."""
     i, j, d, r = neighbor_pairs(pos, box, rd)
-    J = pair_grads(T, elem, i, j, d, r)                  # dG_i/dr_ij
-    v = np.einsum("pf,pfa->pa", gG[i], J)                # dE_i/dr_ij
+    if atoms is not None:
+        sel = np.zeros(len(pos), dtype=bool)
+        sel[np.asarray(atoms)] = True
+        m = sel[i] | sel[j]
+        i, j, d, r = i[m], j[m], d[m], r[m]
     F = np.zeros_like(pos)
-    np.add.at(F, j, -v)
-    np.add.at(F, i, v)
+    for a in range(0, len(i), chunk):
+        s = slice(a, a + chunk)
+        J = pair_grads(T, elem, i[s], j[s], d[s], r[s])        # dG_i/dr_ij
+        v = np.einsum("pf,pfa->pa", gG[i[s]], J)                # dE_i/dr_ij
+        np.add.at(F, j[s], -v)
+        np.add.at(F, i[s], v)
     return F
 
 

@@ -223,3 +223,42 @@ def test_surrogate_inside_integrator_and_fallback(box192):
     lax.run(30, dt_fs=0.5, energy_every=10)
     assert lax.on_surrogate and len(lax.checks) == 3
     sur.attach(False)
+
+
+def test_descriptors_and_forces_match_oracle_at_23328_atoms():
+    """The surrogate at the 23,328-atom water box (its benchmark size): every atom's 96
+    descriptors against the oracle, and forces with random bf16-weight networks on 400 sampled
+    atoms (the oracle's rows complete for those atoms) plus the total energy."""
+    from mdpy_amd import systems
+    from mdpy_amd.engine import Engine
+    from mdpy_amd.mlp import Surrogate
+    say("case: MLP descriptors and forces vs oracle, water23328")
+    s = systems.water_box_large(seed=0)
+    e = Engine(s, skin=1.0)
+    e.set_positions(s.positions)
+    elem = M.elements_of(s.types)
+    sur = Surrogate(e, elem, rd=6.0, skin=1.0, seed=1)
+    try:
+        x = e.positions()
+        box = np.asarray(s.box, dtype=np.float64)
+        G = sur.descriptors().cpu().numpy()
+        Gr, _ = M.descriptors(x, box, elem)
+        scale = np.abs(Gr).max(0) + 1e-12
+        err = (np.abs(G - Gr) / scale).max()
+        say("  descriptors: max error / feature scale %.2e" % err)
+        assert err < 1e-4
+        mean, std = Gr.mean(0), Gr.std(0) + 1e-3
+        nets = _nets(5)
+        sur.set_params(nets)
+        sur.set_norm(mean, std)
+        F, E = sur.eval()
+        F = F.cpu().numpy()
+        ref = M.Surrogate(nets, mean.astype(np.float32).astype(np.float64), std.astype(np.float32).astype(np.float64))
+        atoms = np.sort(np.random.default_rng(2).choice(len(x), 400, replace=False))
+        Fr, Er = ref.forces(x, box, elem, atoms=atoms)
+        say("  forces on 400 atoms: rel RMS %.3e; energy %.6e vs %.6e" % (rel_rms(F[atoms], Fr[atoms]), E, Er))
+        assert rel_rms(F[atoms], Fr[atoms]) < 3e-2
+        assert abs(E - Er) < 1e-2 * abs(Er) + 1e-3 * np.abs(Fr[atoms]).max()
+    finally:
+        sur.close()
+        e.close()

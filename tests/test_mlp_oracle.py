@@ -92,3 +92,36 @@ def test_force_matching_weight_gradient_matches_finite_difference(small):
         nets[e][k][idx] = orig
         fd = (lp - lm) / (2 * h)
         assert grads[e][k][idx] == pytest.approx(fd, rel=2e-5, abs=1e-9), (e, k, idx)
+
+
+def test_tree_neighbour_search_equals_dense():
+    """The periodic k-d tree search (used above 4,096 atoms) returns the dense search's pairs in
+    the same order with bitwise the same vectors and distances: 192- and 648-atom boxes, and the
+    648-atom box with atoms displaced out of the primary cell."""
+    from mdpy_amd import systems
+    cases = []
+    for s in (systems.water_box(4, 3.1, seed=3, cutoff=6.0, pme_grid=(12, 12, 12), name="w64"),
+              systems.water_box_small()):
+        cases.append((np.asarray(s.positions, dtype=np.float64), np.asarray(s.box, dtype=np.float64)))
+    x, box = cases[1]
+    shift = np.random.default_rng(1).integers(-2, 3, size=x.shape) * box
+    cases.append((x + shift, box))
+    for pos, box in cases:
+        a = M.neighbor_pairs(pos, box)
+        b = M.neighbor_pairs_tree(pos, box)
+        assert len(a[0]) > 1000
+        for u, v in zip(a, b):
+            assert np.array_equal(u, v)
+
+
+def test_force_assembly_over_an_atom_subset(small):
+    """Forces assembled from the pairs touching a subset of atoms equal the full assembly on
+    those atoms' rows."""
+    pos, box, elem = small
+    mean, std = _normaliser(pos, box, elem)
+    sur = M.Surrogate(_surrogate(32, seed=4), mean, std)
+    F, E = sur.forces(pos, box, elem)
+    atoms = np.array([0, 5, 17, 100, 191])
+    Fs, Es = sur.forces(pos, box, elem, atoms=atoms)
+    assert Es == E
+    np.testing.assert_allclose(Fs[atoms], F[atoms], rtol=1e-12, atol=1e-12 * np.abs(F).max())
