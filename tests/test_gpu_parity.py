@@ -19,7 +19,8 @@ ENERGY_RTOL = 1e-5     # relative energy error per term (large terms)
 # 4.6e-3 (water648) and 1.2e-4 (water23328, chain) of the sum of |pair energies|. Measured errors:
 # 0.9e-6, 2.0e-6 and 1.55e-5 |E_direct| (on the chain, with its +-0.6 e charges, 1.9e-9 of the
 # sum of |pair energies|); the chain's bound is therefore 2e-5, the others 1e-5.
-DIRECT_ENERGY_RTOL = {"water648": 1e-5, "water23328": 1e-5, "chain": 2e-5}
+DIRECT_ENERGY_RTOL = {"water648": 1e-5, "water23328": 1e-5, "chain": 2e-5, "replica_small": 1e-5,
+                      "replica_large": 1e-5}
 
 
 def say(*a):
@@ -47,6 +48,24 @@ def water23328():
 def chain():
     from mdpy_amd import systems
     return systems.solvated_chain()
+
+
+def _replica(which):
+    """The smallest or largest of the 64 batched water replicas (r_c 9 A, PME ~1 point/A)."""
+    from mdpy_amd import systems
+    reps = systems.water_replicas(64, seed=0)
+    sizes = [s.num_atoms for s in reps]
+    return reps[int(np.argmin(sizes) if which == "small" else np.argmax(sizes))]
+
+
+@pytest.fixture(scope="module")
+def replica_small():
+    return _replica("small")
+
+
+@pytest.fixture(scope="module")
+def replica_large():
+    return _replica("large")
 
 
 def wrapped(s):
@@ -177,7 +196,7 @@ TERM_BITS = {"lj": 1, "coulomb_direct": 2, "coulomb_reciprocal": 4, "bond": 8, "
              "improper": 64}
 
 
-@pytest.mark.parametrize("name", ["water648", "water23328", "chain"])
+@pytest.mark.parametrize("name", ["water648", "water23328", "chain", "replica_small", "replica_large"])
 def test_forces_match_oracle(name, request):
     from mdpy_amd.engine import Engine
     from oracle import mdref as R
