@@ -1934,30 +1934,37 @@ static int fused_integrate(mdx_batch* b, unsigned terms, int nsteps) {
   if (nsteps > 1 && (rc = fused_graphs(b))) return rc;
   MDX_HIP(hipEventRecord(b->ev_in, b->ext));
   MDX_HIP(hipStreamWaitEvent(b->lead, b->ev_in, 0));
-  // per replica: the prime (forces at the current positions, unless the last call left them)
-  // and the call's opening kernel
-  for (mdx_ctx* c : b->ctx)
-    if ((rc = on_lead(b, c, [&]() { return integ_begin(c, terms, nsteps); }))) return rc;
-  mdx_ctx* c0 = b->ctx[0];
-  if (nsteps > 0 && (rc = fused_step(b, energy_step(c0, 0, nsteps), false))) return rc;
-  std::vector<int> seq;
-  graph_schedule(c0, nsteps, seq);
-  for (int k : seq) {
-    hipError_t e = hipGraphLaunch(k == kGraphEnergy ? b->fz.ge : b->fz.gf[k], b->lead);
-    if (e != hipSuccess) { mdx_set_error("hipGraphLaunch: %s", hipGetErrorString(e)); return MDX_ERR_HIP; }
-  }
-  for (mdx_ctx* c : b->ctx) {
-    if ((rc = on_lead(b, c, [&]() { return integ_close(c, terms, nsteps); })) || (rc = integ_end(c, nsteps, false)))
-      return rc;
-    if (nsteps > 0) {
-      c->fpart_live = true;
-      c->forces_current = true;
-      c->forces_terms = terms;
+  auto steps = [&]() -> int {
+    // per replica: the prime (forces at the current positions, unless the last call left them)
+    // and the call's opening kernel
+    for (mdx_ctx* c : b->ctx)
+      if ((rc = on_lead(b, c, [&]() { return integ_begin(c, terms, nsteps); }))) return rc;
+    mdx_ctx* c0 = b->ctx[0];
+    if (nsteps > 0 && (rc = fused_step(b, energy_step(c0, 0, nsteps), false))) return rc;
+    std::vector<int> seq;
+    graph_schedule(c0, nsteps, seq);
+    for (int k : seq) {
+      hipError_t e = hipGraphLaunch(k == kGraphEnergy ? b->fz.ge : b->fz.gf[k], b->lead);
+      if (e != hipSuccess) { mdx_set_error("hipGraphLaunch: %s", hipGetErrorString(e)); return MDX_ERR_HIP; }
     }
-  }
+    for (mdx_ctx* c : b->ctx) {
+      if ((rc = on_lead(b, c, [&]() { return integ_close(c, terms, nsteps); })) || (rc = integ_end(c, nsteps, false)))
+        return rc;
+      if (nsteps > 0) {
+        c->fpart_live = true;
+        c->forces_current = true;
+        c->forces_terms = terms;
+      }
+    }
+    return 0;
+  };
+  rc = steps();
+  // the caller's stream waits for whatever was enqueued, also when a step above failed
+  if (rc)
+    for (mdx_ctx* c : b->ctx) c->forces_current = false;
   MDX_HIP(hipEventRecord(b->ev_out[0], b->lead));
   MDX_HIP(hipStreamWaitEvent(b->ext, b->ev_out[0], 0));
-  return 0;
+  return rc;
 }
 
 extern "C" int mdx_batch_integrate(mdx_batch* b, unsigned terms, int nsteps, int mode) {
