@@ -171,3 +171,75 @@ def test_charmm_forcefield_parameters_6po6():
     ff._parameters["nonbonded"] = {k: v for k, v in ff._parameters["nonbonded"].items() if k != "NH3"}
     with pytest.raises(ParameterPoorDefinedError):
         ff.check_parameters()
+
+
+# ------------------------------------------------------------------ DCD (mdpy/test/test_dcd_parser.py)
+def test_dcd_parser_attributes():
+    # test_dcd_parser.py:25-31, on the reference's own file (an OpenMM trajectory)
+    from mdpy_amd.io import DCDParser
+    p = DCDParser(os.path.join(DATA, "test_dcd_parser.dcd"))
+    assert p.positions.shape[0] == 20
+    assert p.positions.shape[1] == 14984
+    assert p.pbc_matrix[0, 0] == 100
+    # the coordinates lie in the file's 100 x 40 x 40 A box (x, y, z records in that order)
+    x = p.positions
+    for d, edge in enumerate((100.0, 40.0, 40.0)):
+        assert -2.0 < x[..., d].min() < 2.0 and edge - 2.0 < x[..., d].max() < edge + 2.0
+    np.testing.assert_array_equal(np.diag(p.pbc_matrix), [100.0, 40.0, 40.0])
+    assert np.count_nonzero(p.pbc_matrix - np.diag(np.diag(p.pbc_matrix))) == 0
+
+
+def test_dcd_parser_exceptions():
+    # test_dcd_parser.py:33-46
+    from mdpy_amd.io import DCDParser
+    path = os.path.join(DATA, "test_dcd_parser.dcd")
+    with pytest.raises(FileFormatError):
+        DCDParser("te.dc")
+    with pytest.raises(ParserPoorDefinedError):
+        DCDParser(path, is_parse_all=False).positions
+    with pytest.raises(ArrayDimError):
+        DCDParser(path, is_parse_all=False).get_positions(100)
+    with pytest.raises(ArrayDimError):
+        DCDParser(path, is_parse_all=False).get_positions(100, 101)
+    lazy = DCDParser(path, is_parse_all=False)
+    full = DCDParser(path)
+    np.testing.assert_array_equal(lazy.get_positions(7), full.positions[7])
+    np.testing.assert_array_equal(lazy.get_positions(0, 19), full.positions[[0, 19]])
+
+
+def _dcd_bytes(x, cell, order):
+    """A DCD file written record by record (frames x atoms x 3 float32, a 6-double unit cell)."""
+    import struct
+    i4, f4, f8 = order + "i", order + "f", order + "d"
+
+    def rec(payload):
+        return struct.pack(i4, len(payload)) + payload + struct.pack(i4, len(payload))
+    ic = [len(x), 0, 1, len(x), 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 24]
+    head = b"CORD" + b"".join(struct.pack(i4, v) for v in ic[:9]) + struct.pack(f4, 0.5) + \
+        b"".join(struct.pack(i4, v) for v in ic[10:])
+    out = rec(head) + rec(struct.pack(i4, 1) + b"synthetic".ljust(80, b" ")) + rec(struct.pack(i4, x.shape[1]))
+    for frame in x:
+        out += rec(b"".join(struct.pack(f8, v) for v in cell))
+        for d in range(3):
+            out += rec(frame[:, d].astype(order + "f4").tobytes())
+    return out
+
+
+@pytest.mark.parametrize("order", ["<", ">"])
+def test_dcd_parser_byte_order_and_triclinic_cell(tmp_path, order):
+    """Both byte orders; a triclinic cell given in degrees (a, gamma, b, beta, alpha, c)."""
+    from mdpy_amd.io import DCDParser
+    rng = np.random.default_rng(0)
+    x = rng.uniform(0, 20, size=(3, 7, 3)).astype(np.float32)
+    cell = (20.0, 80.0, 22.0, 90.0, 90.0, 25.0)  # gamma = 80 degrees
+    path = str(tmp_path / "t.dcd")
+    with open(path, "wb") as f:
+        f.write(_dcd_bytes(x, cell, order))
+    p = DCDParser(path)
+    np.testing.assert_array_equal(p.positions, x)
+    m = p.pbc_matrix
+    g = math.radians(80.0)
+    np.testing.assert_allclose(m[0], [20.0, 0, 0], atol=1e-5)
+    np.testing.assert_allclose(m[1], [22.0 * math.cos(g), 22.0 * math.sin(g), 0], atol=1e-5)
+    np.testing.assert_allclose(m[2], [0, 0, 25.0], atol=1e-5)
+    assert abs(p.time_step - 0.5) < 1e-7
