@@ -24,4 +24,7 @@ def __getattr__(name):
     if name in ("Simulation", "LogDumper", "Dumper", "PDBDumper"):
         from . import simulation
         return getattr(simulation, name)
+    if name == "HDF5Dumper":
+        from .dumper import HDF5Dumper
+        return HDF5Dumper
     raise AttributeError(name)

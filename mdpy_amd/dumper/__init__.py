@@ -1,4 +1,4 @@
-"""Dumpers (mdpy/dumper/*.py): LogDumper and PDBDumper.
+"""Dumpers (mdpy/dumper/*.py): LogDumper, PDBDumper and HDF5Dumper.
 
 A dumper formats a Frame: the step, the positions, the per-term potential energies and the
 kinetic energy at that step, the box and the wall-clock time the frame was reached. During
@@ -174,4 +174,23 @@ class PDBDumper(Dumper):
         self._num_dumpped_frames += 1
 
 
-__all__ = ["Dumper", "LogDumper", "PDBDumper", "Frame"]
+class HDF5Dumper(Dumper):
+    """mdpy/dumper/hdf5_dumper.py:14-23: the topology and box at the first dumped frame, then the
+    positions of every dumped frame (mdpy's HDF5 layout, mdpy_amd.io.HDF5Writer)."""
+    SUFFIX = "hdf5"
+    NEEDS_POSITIONS = True
+
+    def __init__(self, file_path: str, dump_frequency: int) -> None:
+        from ..io import HDF5Writer
+        super().__init__(file_path, dump_frequency)
+        self._writer = HDF5Writer(file_path, "w")
+
+    def dump_frame(self, frame, simulation):
+        if self._num_dumpped_frames == 0:
+            self._writer.topology = simulation.ensemble.topology
+            self._writer.pbc_matrix = frame.pbc_matrix
+        self._writer.write(frame.positions)
+        self._num_dumpped_frames += 1
+
+
+__all__ = ["Dumper", "LogDumper", "PDBDumper", "HDF5Dumper", "Frame"]

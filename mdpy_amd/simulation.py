@@ -15,7 +15,7 @@ import time
 
 import numpy as np
 
-from .dumper import Dumper, Frame, LogDumper, PDBDumper  # noqa: F401  (re-exported)
+from .dumper import Dumper, Frame, HDF5Dumper, LogDumper, PDBDumper  # noqa: F401  (re-exported)
 from .error import DumperPoorDefinedError
 
 FRAMES_IN_FLIGHT = 3

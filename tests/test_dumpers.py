@@ -74,3 +74,28 @@ def test_log_dumper_columns(tmp_path):
     # 100 steps of 2 fs in 2 s of wall time: 0.1 ps/s = 8.64 ns/day; 200 steps left take 4 s
     assert rows[2][8] == "8.6400" and rows[2][9] == "0:00:04"
     assert log.records[-1]["total_energy"] == pytest.approx(-5.3 * KJ_PER_MOL)
+
+
+def test_hdf5_dumper_frames(tmp_path):
+    """mdpy/dumper/hdf5_dumper.py: the first dumped frame writes the ensemble's topology and box,
+    every frame its positions; read back with the HDF5 parser."""
+    from mdpy_amd.dumper import HDF5Dumper
+    from mdpy_amd.io import HDF5Parser
+
+    class Ens:
+        topology = PSFParser(os.path.join(DATA, "6PO6.psf")).topology
+
+    class Sim:
+        ensemble = Ens()
+    with pytest.raises(FileFormatError):
+        HDF5Dumper(str(tmp_path / "a.pdb"), 10)
+    x = PDBParser(os.path.join(DATA, "6PO6.pdb")).positions.astype(np.float64)
+    path = str(tmp_path / "run.hdf5")
+    d = HDF5Dumper(path, 10)
+    pbc = np.diag([30.0, 31.0, 32.0])
+    for k in range(3):
+        d.dump_frame(Frame(10 * k, x + k, 0.0, 0.0, pbc, None, 1.0, 0.0), Sim())
+    p = HDF5Parser(path)
+    assert p.num_frames == 3 and p.topology.num_particles == 49 and p.topology.num_bonds == 49
+    np.testing.assert_allclose(p.positions[2], x + 2, rtol=1e-6)
+    np.testing.assert_allclose(p.pbc_matrix, pbc)

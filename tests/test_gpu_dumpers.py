@@ -1,4 +1,4 @@
-"""GPU test: Simulation.integrate with a LogDumper and a PDBDumper writes its frames from
+"""GPU test: Simulation.integrate with a LogDumper, a PDBDumper and an HDF5Dumper writes its frames from
 asynchronous device copies; every frame is the state at its step (the last one equals the
 ensemble after the run) and the log's energies are the device energies of those steps."""
 import os
@@ -11,12 +11,12 @@ pytestmark = pytest.mark.gpu
 
 
 def test_simulation_async_frames(tmp_path):
-    from mdpy_amd import Ensemble, LogDumper, PDBDumper, Simulation, systems
+    from mdpy_amd import Ensemble, HDF5Dumper, LogDumper, PDBDumper, Simulation, systems
     from mdpy_amd.constraint import (CharmmAngleConstraint, CharmmBondConstraint, CharmmNonbondedConstraint,
                                      ElectrostaticPMEConstraint)
     from mdpy_amd.core import Topology
     from mdpy_amd.integrator import LangevinIntegrator
-    from mdpy_amd.io import PDBParser
+    from mdpy_amd.io import HDF5Parser, PDBParser
     from mdpy_amd.unit import KJ_PER_MOL
     print("case: Simulation with asynchronous log and PDB frames, water648, 200 steps", flush=True)
     s = systems.water_box_small()
@@ -34,7 +34,8 @@ def test_simulation_async_frames(tmp_path):
     log = LogDumper(log_path, 20, step=True, potential_energy=True, kinetic_energy=True, total_energy=True,
                     temperature=True, sim_speed=True)
     pdb = PDBDumper(pdb_path, 50)
-    sim.add_dumpers(log, pdb)
+    h5_path = str(tmp_path / "run.hdf5")
+    sim.add_dumpers(log, pdb, HDF5Dumper(h5_path, 100))
     sim.integrate(200)
     assert sim.cur_step == 200
     rows = [r.rstrip("\t").split("\t") for r in open(log_path).read().strip().split("\n")]
@@ -49,3 +50,7 @@ def test_simulation_async_frames(tmp_path):
     last = p.get_positions(3).astype(np.float64)
     assert np.abs(last - np.asarray(ens.state.positions, np.float64)).max() < 1e-3
     assert np.abs(p.get_positions(0) - last).max() > 0.01  # frames differ: the run moved
+    h = HDF5Parser(h5_path)
+    assert h.num_frames == 2 and h.topology.num_particles == t.num_particles
+    assert np.abs(h.get_positions(1) - np.asarray(ens.state.positions, np.float64)).max() < 1e-4
+    np.testing.assert_allclose(h.get_positions(0), p.get_positions(1), atol=1e-3)  # step 100 in both files
