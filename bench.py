@@ -226,8 +226,8 @@ def replicas_leg(steps=200):
     b = ReplicaBatch(reps, energy_every=50)
     best, tune = b.autotune_fused(skins=(1.0, 1.5, 2.0), lanes=(16, 32), rebuild_every=(0, 8, 10, 12), steps=40,
                                   warmup=20)
-    progress("replicas: fused batch at skin %.1f A, %d lanes per atom, lists rebuilt together every %d steps "
-             "(0: displacement test only)" % best)
+    progress("replicas: fused batch at skin %.1f A, %d lanes per atom, %d list cells per radius, lists rebuilt "
+             "together every %d steps (0: displacement test only)" % best)
     b.integrate(40, "fused")
     fused = b.atom_steps_per_s(steps, "fused")
     rooflines = batch_rooflines(b, reps)
@@ -245,7 +245,8 @@ def replicas_leg(steps=200):
     progress("replicas: 8-replica comparison with the per-replica graph modes")
     b8 = ReplicaBatch(reps[:8], energy_every=50, skins=[best[0]] * 8)
     b8.set_lanes(best[1])
-    b8.set_rebuild_every(best[2])
+    b8.set_list_cells(best[2])
+    b8.set_rebuild_every(best[3])
     r8 = {}
     for mode in ("fused", "streams", "sequential"):
         b8.integrate(20, mode)
@@ -255,8 +256,8 @@ def replicas_leg(steps=200):
     return {"metric": "aggregate atom-steps/s, 64 water replicas (fused batch)", "value": fused,
             "atoms": int(sum(s.num_atoms for s in reps)), "steps": steps,
             "sequential_per_replica_loop": sequential, "speedup_vs_sequential": fused / sequential,
-            "autotuned": {"skin_A": best[0], "lanes": best[1], "rebuild_every": best[2],
-                          "candidates": {"%.1f/%d/%d" % k: v for k, v in tune.items()}},
+            "autotuned": {"skin_A": best[0], "lanes": best[1], "list_cells_per_radius": best[2],
+                          "rebuild_every": best[3], "candidates": {"%.1f/%d/%d/%d" % k: v for k, v in tune.items()}},
             "first_8_replicas": r8, "rooflines": rooflines, "device_flags": flags}
 
 
@@ -423,6 +424,8 @@ def main():
     ap.add_argument("--energy-every", type=int, default=50,
                     help="potential energies every this many steps (forces every step)")
     ap.add_argument("--lanes", type=int, default=0, help="nonbonded lanes per atom (0 = autotune)")
+    ap.add_argument("--cells", type=int, default=0,
+                    help="neighbour-list grid cells per list radius, 1-3 (0 = autotune between 2 and 3)")
     ap.add_argument("--persist", type=int, default=3,
                     help="nonbonded workgroups per compute unit looping over the work (0: one per unit of work)")
     ap.add_argument("--phases", default="off",
@@ -470,6 +473,22 @@ def main():
         phases = [float(v) for v in args.phases.split(",")]
         e.set_force_phases(True, phases[0], phases[1])
     e.integrate(terms, args.warmup, use_graph, args.energy_every)
+    # autotune the neighbour-list grid's cells per list radius (2 or 3; the same pairs either
+    # way): each candidate integrates 1,000 steps twice, alternating, and the faster minimum wins
+    cells, cell_s = args.cells, {}
+    if cells == 0:
+        for _ in range(2):
+            for k in (2, 3):
+                e.set_list_cells(k)
+                e.integrate(terms, 200, use_graph, args.energy_every)
+                torch.cuda.synchronize()
+                tc = time.perf_counter()
+                e.integrate(terms, 1000, use_graph, args.energy_every)
+                torch.cuda.synchronize()
+                cell_s[k] = min(cell_s.get(k, 1e9), time.perf_counter() - tc)
+        cells = min(cell_s, key=cell_s.get)
+    e.set_list_cells(cells)
+    e.integrate(terms, 200, use_graph, args.energy_every)
     # autotune the nonbonded kernel's lanes per atom on the current list (each candidate timed
     # over 1,500 launches, ~75 ms: stable choices, and the GPU reaches its steady clock before the
     # timed steps however short the warm-up), then re-warm the graph
@@ -520,7 +539,7 @@ def main():
                 % args.equilibrate,
         "config": {"workload": "water23328: 7,776 TIP3P-like waters, 61.9 A cube, r_c 12 A, PME 64^3 "
                                "order 4, G-JF Langevin 300 K, dt 1 fs, plain FP32 path",
-                   "atoms": s.num_atoms, "list_skin_A": args.skin, "hipgraph": use_graph, "nb_lanes": lanes,
+                   "atoms": s.num_atoms, "list_skin_A": args.skin, "list_cells_per_radius": cells, "hipgraph": use_graph, "nb_lanes": lanes,
                    "energy_every": args.energy_every, "force_phases": phases, "nb_resident_per_cu": args.persist,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "kernel": "k_nonbonded (LJ + erfc Coulomb)",
@@ -532,6 +551,7 @@ def main():
         "secondary_rooflines": secondary_rooflines(e, s, terms, slots),
         "list_rebuilds": stats["rebuilds"],
         "list_rebuilds_timed": rebuilds_timed,
+        "list_cells_tuning_ms_per_1000_steps": {str(k): round(v * 1e3, 3) for k, v in cell_s.items()} or None,
         "energies_finite": finite,
         "device_flags": status,
     }

@@ -306,6 +306,14 @@ int mdx_nonbonded_persist(mdx_ctx* ctx, int per_cu);
  * interval rebuild together, so a batched step holds either all their list builds or none.
  * Applies to the fused step prologue (G-JF and velocity Verlet on a list grid). */
 int mdx_nonbonded_rebuild_every(mdx_ctx* ctx, int every);
+/* Cell size of the neighbour-list grid (no reference counterpart; the reference's own cell list,
+ * mdpy/core/cell_list.py, keeps its cutoff-wide cells): per_radius = 1, 2 (default) or 3 cells
+ * per list radius r_c + skin along each box edge, so cells at least (r_c + skin)/per_radius
+ * wide. Smaller cells scan fewer candidates outside the list radius but visit more stencil
+ * cells. The lists hold the same pairs for every choice, rows in another order: forces and
+ * energies agree to float32 summation rounding (each row's float32 partial sums follow its
+ * order). The next evaluation bins the atoms again and rebuilds. */
+int mdx_nonbonded_cells(mdx_ctx* ctx, int per_radius);
 /* Candidate (list) pair slots and in-cutoff unique pairs of the last nonbonded evaluation
  * (counts the list on the device; for roofline accounting; synchronises). */
 int mdx_pair_counts(mdx_ctx* ctx, int64_t* h_list_slots, int64_t* h_unique_in_cutoff);

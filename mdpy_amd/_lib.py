@@ -109,6 +109,7 @@ SIGNATURES = {
     "mdx_nonbonded_tune": (ctypes.c_int, [_vp, ctypes.c_uint, ctypes.c_int, ctypes.c_int, _c_int_p]),
     "mdx_nonbonded_persist": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mdx_nonbonded_rebuild_every": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "mdx_nonbonded_cells": (ctypes.c_int, [_vp, ctypes.c_int]),
     "mdx_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _vp, ctypes.POINTER(_vp)]),
     "mdx_batch_integrate": (ctypes.c_int, [_vp, ctypes.c_uint, ctypes.c_int, ctypes.c_int]),
     "mdx_batch_time_kernel": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _c_dbl_p]),

@@ -2335,6 +2335,16 @@ extern "C" int mdx_nonbonded_rebuild_every(mdx_ctx* c, int every) {
   return 0;
 }
 
+extern "C" int mdx_nonbonded_cells(mdx_ctx* c, int per_radius) {
+  MDX_CHECK(c && per_radius >= 1 && per_radius <= 3, MDX_ERR_INVALID,
+            "mdx_nonbonded_cells: cells per list radius must be 1, 2 or 3");
+  if (per_radius == c->cells_per_rl) return 0;
+  c->cells_per_rl = per_radius;
+  mdx_invalidate_graph(c);  // captured launches hold the grid's size
+  // a new grid: atoms are binned again and the list rebuilt by the next evaluation
+  return c->nb_configured ? mdx_nlist_prepare(c) : 0;
+}
+
 extern "C" int mdx_nonbonded_persist(mdx_ctx* c, int per_cu) {
   MDX_CHECK(c && per_cu >= 0, MDX_ERR_INVALID, "mdx_nonbonded_persist: bad arguments");
   c->nb_persist = c->ncu > 0 ? per_cu : 0;

@@ -252,6 +252,7 @@ struct mdx_ctx {
   // waves of 123 VGPRs) leaves a quarter of every SIMD's registers to the PME kernels beside it
   int nb_persist = 3;
   int rebuild_every = 0;  // scheduled list rebuilds (step counter multiple), 0 = displacement test only
+  int cells_per_rl = 2;   // list-grid cells per list radius r_c + skin along each axis (1..3)
   int ncu = 0;                   // compute units of the device
   int list_stride = 0;
   bool list_valid = false;

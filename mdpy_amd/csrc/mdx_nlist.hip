@@ -992,7 +992,7 @@ static void make_grid(mdx_ctx* c, GridArgs* g) {
 int mdx_nlist_prepare(mdx_ctx* c) {
   double rl = c->rc + c->skin;
   for (int d = 0; d < 3; ++d) {
-    int k = (int)std::floor(c->box[d] / (0.5 * rl));
+    int k = (int)std::floor(c->box[d] * c->cells_per_rl / rl);
     if (k < 1) k = 1;
     c->gn[d] = k;
   }

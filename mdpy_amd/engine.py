@@ -278,6 +278,12 @@ class Engine:
         displacement test asks for it, which it still does either way)."""
         L.call("mdx_nonbonded_rebuild_every", self.ctx, int(every))
 
+    def set_list_cells(self, per_radius):
+        """Neighbour-list grid cells per list radius along each edge (1, 2 = default, 3): the
+        same pairs either way, in another row order (forces equal to float32 summation
+        rounding); the next evaluation rebuilds."""
+        L.call("mdx_nonbonded_cells", self.ctx, int(per_radius))
+
     def time_kernel(self, which, terms, reps=20):
         """Average ms per launch (HIP events on the library stream): which 0 nonbonded,
         1 force evaluation, 2 PME reciprocal chain, 3 integrator step, 4 step prologue alone

@@ -65,6 +65,7 @@ class ReplicaBatch:
             e.set_energy_interval(self.energy_every)
             self.engines.append(e)
         self.skins = skins
+        self.list_cells = 2
         self.n_atoms = int(sum(s.num_atoms for s in self.systems))
         self._batch = None
         self._make_batch()
@@ -144,11 +145,18 @@ class ReplicaBatch:
             e.set_rebuild_every(every)
         self.rebuild_every = int(every)
 
-    def autotune_fused(self, skins=(1.0, 1.5, 2.0), lanes=(16, 32), rebuild_every=(0, 8, 10, 12), steps=60,
-                       warmup=20):
-        """The (skin, lanes, rebuild interval) with the highest aggregate atom-steps/s of the fused
-        batch, each candidate timed on the whole batch ((skin, lanes) first, at the first interval
-        candidate, then the interval); the batch is left at the best one."""
+    def set_list_cells(self, per_radius):
+        """Neighbour-list grid cells per list radius of every replica (1, 2 or 3)."""
+        for e in self.engines:
+            e.set_list_cells(per_radius)
+        self.list_cells = int(per_radius)
+
+    def autotune_fused(self, skins=(1.0, 1.5, 2.0), lanes=(16, 32), cells=(2, 3), rebuild_every=(0, 8, 10, 12),
+                       steps=60, warmup=20):
+        """The (skin, lanes, list cells per radius, rebuild interval) with the highest aggregate
+        atom-steps/s of the fused batch, each candidate timed on the whole batch: (skin, lanes)
+        at the first cell and interval candidates, then the cell size, then the interval. The
+        batch is left at the best one."""
         rates = {}
 
         def timed(key):
@@ -156,21 +164,28 @@ class ReplicaBatch:
             rates[key] = self.atom_steps_per_s(steps, "fused")
 
         self.set_rebuild_every(rebuild_every[0])
+        self.set_list_cells(cells[0])
         for skin in skins:
             self.set_skin(skin)
             for g in lanes:
                 self.set_lanes(g)
-                timed((float(skin), int(g), int(rebuild_every[0])))
-        skin, g, _ = max(rates, key=rates.get)
+                timed((float(skin), int(g), int(cells[0]), int(rebuild_every[0])))
+        skin, g, _, _ = max(rates, key=rates.get)
         self.set_skin(skin)
         self.set_lanes(g)
+        for k in cells[1:]:
+            self.set_list_cells(k)
+            timed((skin, g, int(k), int(rebuild_every[0])))
+        k = max(rates, key=rates.get)[2]
+        self.set_list_cells(k)
         for every in rebuild_every[1:]:
             self.set_rebuild_every(every)
-            timed((skin, g, int(every)))
+            timed((skin, g, k, int(every)))
         best = max(rates, key=rates.get)
         self.set_skin(best[0])
         self.set_lanes(best[1])
-        self.set_rebuild_every(best[2])
+        self.set_list_cells(best[2])
+        self.set_rebuild_every(best[3])
         return best, rates
 
     def autotune_mode(self, steps=100, warmup=20):

@@ -198,6 +198,47 @@ def test_scheduled_list_rebuilds_match_displacement_rebuilds():
     assert np.allclose(h0, h1, rtol=1e-6, atol=1e-6)
 
 
+def test_list_grid_cell_sizes_give_the_same_forces_and_trajectory():
+    """One, two or three list-grid cells per list radius (mdx_nonbonded_cells): the lists hold the
+    same pairs (the fast shifted-candidate kernel on some grids, the general cell kernel on the
+    others) in another row order. Each row's float32 partial sums follow the row order and the
+    PME spreading the atoms' cell order, so forces, energies and a 30-step trajectory agree to
+    float32 rounding (measured 6.5e-9 kJ/mol/A on forces of ~1e-2 between 2 and 3 cells)."""
+    from mdpy_amd import _lib as L, systems
+    from mdpy_amd.engine import Engine
+    reps = systems.water_replicas(64, seed=0)
+    mid = sorted(reps, key=lambda s: s.num_atoms)[40]
+    for s in (mid, systems.water_box_large(seed=0)):
+        say("case: list cells per radius 2 / 3 / 1, %d atoms, box %.1f A" % (s.num_atoms, s.pbc_matrix[0, 0]))
+        direct = L.TERM_LJ | L.TERM_COUL_DIRECT | L.TERM_BOND | L.TERM_ANGLE
+        full = direct | L.TERM_COUL_RECIP
+        out = []
+        for k in (2, 3, 1):
+            e = Engine(s, skin=1.5)
+            e.set_positions(s.positions)
+            e.set_velocities(thermal_v(s, 300.0, 4))
+            e.configure_integrator(L.INT_LANGEVIN, dt=1.0, temperature=300.0, friction=1e-3, seed=9)
+            e.set_list_cells(k)
+            f0, en0 = e.compute(direct)
+            f0 = f0.cpu().numpy().copy()
+            f1, en1 = e.compute(full)
+            f1 = f1.cpu().numpy().copy()
+            e.integrate(full, 30, use_graph=True, energy_every=10)
+            out.append((f0, en0, f1, e.positions(), e.energy_history(), e.status()))
+            e.close()
+        base = out[0]
+        for k, o in zip((3, 1), out[1:]):
+            assert o[5] == 0 and base[5] == 0
+            d0 = np.sqrt(np.mean((o[0] - base[0]) ** 2)) / np.sqrt(np.mean(base[0] ** 2))
+            assert d0 < 1e-6, (k, d0)
+            for t in ("lj", "coulomb_direct", "bond", "angle"):
+                assert o[1][t] == pytest.approx(base[1][t], rel=1e-6, abs=1e-6), (k, t)
+            scale = np.sqrt(np.mean(base[2] ** 2))
+            assert np.sqrt(np.mean((o[2] - base[2]) ** 2)) < 1e-6 * scale, k
+            assert np.abs(o[3] - base[3]).max() < 1e-5, k
+            assert np.allclose(o[4], base[4], rtol=1e-6, atol=1e-4), k
+
+
 def test_kernel_timing_leaves_the_trajectory_unchanged():
     """Timing the step prologue (state saved and restored) and the three PME plane launches
     between two integrate calls changes nothing: the next call starts from a fresh evaluation at

@@ -167,7 +167,7 @@ def test_fused_batch_autotune_and_repeated_calls():
     b = ReplicaBatch(sys_list, energy_every=10)
     best, rates = b.autotune_fused(skins=(1.0, 1.5), lanes=(16, 32), rebuild_every=(0, 10), steps=20, warmup=10)
     say("  rates", {k: "%.3g" % v for k, v in rates.items()}, "best", best)
-    assert best in rates and len(rates) == 5 and all(v > 0 for v in rates.values())
+    assert best in rates and len(rates) == 6 and all(v > 0 for v in rates.values())
     for n in (20, 33, 1, 0, 12):
         b.integrate(n, "fused")
     torch.cuda.synchronize()
