@@ -152,19 +152,22 @@ __device__ __forceinline__ void stage_neighbours(const MlAtoms& a, const MlConst
 
 // ---------------------------------------------------------------- descriptors
 // One wave per atom. Each chunk of 64 list neighbours is first evaluated with one lane per
-// neighbour: the in-range ones (r < r_d) are compacted in list order into LDS tables of their 32
-// radial Gaussians, 4 angular channel weights (all times the cutoff function) and 20 unit-vector
-// monomials. Then lane f accumulates radial feature f (element f/32, Gaussian f%32) and moment
-// components lane, lane+64, lane+128 (< 160) over the chunk's table rows in order.
+// neighbour: the in-range ones (r < r_d) are compacted per element, in list order, into LDS rows
+// (element 0 from the front, element 1 from the back) holding r, the cutoff function, the 4
+// angular channel weights (times the cutoff function) and the 20 unit-vector monomials. Then, per
+// element, the lanes walk that element's rows: the radial Gaussians two rows at a time (lanes
+// 0-31 the even row, lanes 32-63 the odd one, Gaussian lane & 31, the halves added at the end),
+// the moment components c = lane and 64 + lane (< 80) of the element's tensor one row at a time.
+// Per row this is 1 exponential per lane pair and 2 products, against 32 exponentials and 3
+// element-selected products when every lane walks every row.
 __global__ __launch_bounds__(64) void k_ml_desc(MlAtoms a, MlConst k, const float4* __restrict__ xq,
                                                const int* __restrict__ sorted_orig, const int* __restrict__ nbr_count,
                                                const unsigned* __restrict__ nbr_list, const int* __restrict__ row_of,
                                                const float* __restrict__ mean, const float* __restrict__ rstd,
                                                float* __restrict__ G, float* __restrict__ T, bf16_t* __restrict__ X) {
-  __shared__ float gr[64][ML_RAD + 1];
+  __shared__ float rr[64], rfc[64];
   __shared__ float ga[64][ML_CH];
   __shared__ float mo[64][ML_MOM + 1];
-  __shared__ int ge[64];
   __shared__ float sm[ML_T];
   const int s = blockIdx.x;
   const int lane = threadIdx.x;
@@ -173,20 +176,17 @@ __global__ __launch_bounds__(64) void k_ml_desc(MlAtoms a, MlConst k, const floa
   const int row = row_of[o];
   const int cnt = nbr_count[s];
   const unsigned* lrow = nbr_list + (long long)s * a.cap;
-  const int fe = lane >> 5, fk = lane & 31;
-  float rad = 0.f;
-  float mom[3] = {0.f, 0.f, 0.f};
-  int me[3], mch[3], mcomp[3];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const int c = lane + 64 * q;
-    const int cc = c < ML_T ? c : 0;
-    me[q] = c < ML_T ? cc / 80 : -1;
-    mch[q] = (cc % 80) / 20;
-    mcomp[q] = cc % 20;
-  }
+  const int half = lane >> 5;
+  const float mu = k.rad_mu0 + (float)(lane & 31) * k.rad_dmu;  // this lane's Gaussian centre
+  // moment components of this lane within an element's 80: c0 = lane, c1 = 64 + lane (lanes < 16)
+  const int ch0 = lane / ML_MOM, cp0 = lane % ML_MOM;
+  const int c1 = 64 + (lane & 15);
+  const int ch1 = c1 / ML_MOM, cp1 = c1 % ML_MOM;
+  const bool has1 = lane < 16;
+  float rad[2] = {0.f, 0.f};  // per element: Gaussian lane & 31 over this lane's half of the rows
+  float m0[2] = {0.f, 0.f}, m1[2] = {0.f, 0.f};
   for (int base = 0; base < cnt; base += 64) {
-    // lane = neighbour: geometry, in-range test, compaction
+    // lane = neighbour: geometry, in-range test, per-element compaction
     const int kk = base + lane;
     float ux = 0.f, uy = 0.f, uz = 0.f, r = 1e30f;
     int e = 0;
@@ -197,10 +197,10 @@ __global__ __launch_bounds__(64) void k_ml_desc(MlAtoms a, MlConst k, const floa
       dx -= a.Lx * rintf(dx * a.ix);
       dy -= a.Ly * rintf(dy * a.iy);
       dz -= a.Lz * rintf(dz * a.iz);
-      const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
-      if (rr < k.rd && rr > 0.f) {
-        r = rr;
-        const float ir = __builtin_amdgcn_rcpf(rr);
+      const float rr2 = sqrtf(dx * dx + dy * dy + dz * dz);
+      if (rr2 < k.rd && rr2 > 0.f) {
+        r = rr2;
+        const float ir = __builtin_amdgcn_rcpf(rr2);
         ux = dx * ir;
         uy = dy * ir;
         uz = dz * ir;
@@ -208,17 +208,15 @@ __global__ __launch_bounds__(64) void k_ml_desc(MlAtoms a, MlConst k, const floa
       e = (int)xj.w;
     }
     const bool keep = r < k.rd;
-    const unsigned long long ball = __ballot(keep);
-    const int slot = __popcll(ball & ((1ull << lane) - 1ull));
-    const int m = __popcll(ball);
+    const unsigned long long b0 = __ballot(keep && e == 0), b1 = __ballot(keep && e != 0);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int n0 = __popcll(b0), n1 = __popcll(b1);
     if (keep) {
+      const int slot = e == 0 ? __popcll(b0 & below) : 63 - __popcll(b1 & below);
       float fc, dfc;
       cutoff_fc(k, r, &fc, &dfc);
-#pragma unroll 8
-      for (int q = 0; q < ML_RAD; ++q) {
-        const float dr = r - (k.rad_mu0 + (float)q * k.rad_dmu);
-        gr[slot][q] = __expf(-k.rad_eta * dr * dr) * fc;
-      }
+      rr[slot] = r;
+      rfc[slot] = fc;
 #pragma unroll
       for (int ch = 0; ch < ML_CH; ++ch) {
         const float dn = r - k.ang_nu[ch];
@@ -246,37 +244,48 @@ __global__ __launch_bounds__(64) void k_ml_desc(MlAtoms a, MlConst k, const floa
       mm[17] = yy * uz;
       mm[18] = uy * zz;
       mm[19] = zz * uz;
-      ge[slot] = e;
     }
     __syncthreads();
-    // lane = feature: accumulate over the chunk's in-range neighbours, in list order
-#pragma unroll 4
-    for (int jj = 0; jj < m; ++jj) {
-      const int ej = ge[jj];
-      const float g = gr[jj][fk];
-      rad += ej == fe ? g : 0.f;
+    // lane = feature: accumulate over each element's rows of the chunk, in list order
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const float w = ga[jj][mch[q]] * mo[jj][mcomp[q]];
-        mom[q] += me[q] == ej ? w : 0.f;
+    for (int el = 0; el < 2; ++el) {
+      const int n = el == 0 ? n0 : n1;
+      float acc = 0.f;
+      for (int jj = half; jj < n; jj += 2) {  // radial: two rows per step
+        const int q = el == 0 ? jj : 63 - jj;
+        const float d = rr[q] - mu;
+        acc += __expf(-k.rad_eta * d * d) * rfc[q];
       }
+      rad[el] += acc;
+      float a0 = 0.f, a1 = 0.f;
+#pragma unroll 4
+      for (int jj = 0; jj < n; ++jj) {  // moments: one row per step
+        const int q = el == 0 ? jj : 63 - jj;
+        a0 += ga[q][ch0] * mo[q][cp0];
+        a1 += ga[q][ch1] * mo[q][cp1];
+      }
+      m0[el] += a0;
+      m1[el] += a1;
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const int c = lane + 64 * q;
-    if (c < ML_T) {
-      sm[c] = mom[q];
-      T[(long long)row * ML_T + c] = mom[q];
+  for (int el = 0; el < 2; ++el) {
+    sm[el * 80 + lane] = m0[el];
+    T[(long long)row * ML_T + el * 80 + lane] = m0[el];
+    if (has1) {
+      sm[el * 80 + c1] = m1[el];
+      T[(long long)row * ML_T + el * 80 + c1] = m1[el];
     }
   }
   __syncthreads();
   // features: radial f = e*48 + k (lane < 64), angular f = e*48 + 32 + ch*4 + l (lanes 0..31)
   {
-    const int f = fe * ML_PER + fk;
-    G[(long long)row * ML_F + f] = rad;
-    X[(long long)row * ML_F + f] = f2bf((rad - mean[f]) * rstd[f]);
+    const float g0 = rad[0] + __shfl_xor(rad[0], 32, 64), g1 = rad[1] + __shfl_xor(rad[1], 32, 64);
+    const float g = half == 0 ? g0 : g1;
+    const int f = half * ML_PER + (lane & 31);
+    G[(long long)row * ML_F + f] = g;
+    X[(long long)row * ML_F + f] = f2bf((g - mean[f]) * rstd[f]);
   }
   if (lane < 32) {
     const int e = lane >> 4, ch = (lane >> 2) & 3, l = lane & 3;
@@ -292,34 +301,51 @@ __global__ __launch_bounds__(64) void k_ml_desc(MlAtoms a, MlConst k, const floa
 
 // dG_i/dr_ij . w for all 96 features of atom i is assembled by the JVP kernel; the force kernel
 // needs the transposed product gG . dG_i/dr_ij for one pair, computed here (gG = dE_i/dG_i,
-// tt = the atom's moment tensors, both in LDS)
+// tt = the atom's moment tensors, both in LDS).
+// Radial terms: only the 2 ML_RWIN + 1 Gaussians centred nearest r. The centres are dmu apart and
+// eta dmu^2 = 1/2, so a Gaussian k centres away weighs exp(-k^2/2): the terms left out
+// (k >= ML_RWIN + 1 = 9, |r - mu| >= 8.5 dmu) weigh below exp(-36) = 2e-16 of the nearest one,
+// far under float32 rounding.
+#define ML_RWIN 8
 __device__ __forceinline__ void pair_force(const MlConst& k, const float* gG, const float* tt, int e, float ux,
                                            float uy, float uz, float r, float* vx, float* vy, float* vz) {
   float fc, dfc;
   cutoff_fc(k, r, &fc, &dfc);
   const float* gr = gG + e * ML_PER;
   float srad = 0.f;
-  for (int q = 0; q < ML_RAD; ++q) {
+  const int qc = (int)rintf((r - k.rad_mu0) / k.rad_dmu);
+  const int q0 = min(max(qc - ML_RWIN, 0), ML_RAD - (2 * ML_RWIN + 1));
+#pragma unroll
+  for (int t = 0; t < 2 * ML_RWIN + 1; ++t) {
+    const int q = q0 + t;
     const float dr = r - (k.rad_mu0 + q * k.rad_dmu);
     const float ex = __expf(-k.rad_eta * dr * dr);
     srad += gr[q] * ex * (dfc - 2.f * k.rad_eta * dr * fc);
   }
   float ax = srad * ux, ay = srad * uy, az = srad * uz;
   const float ir = 1.f / r;
+  // d/dr_ij [g(r) s_l(u)] = g' s_l u + (g/r) (v_l - (v_l . u) u), and v_l . u = l s_l (s_l is a
+  // homogeneous polynomial of degree l in u, v_l its gradient): per channel the scalars
+  // sum_l cf_l s_l (g' - l g/r) along u and the vector (g/r) sum_l cf_l v_l.
   for (int ch = 0; ch < ML_CH; ++ch) {
     const float dn = r - k.ang_nu[ch];
     const float ex = __expf(-k.ang_eta * dn * dn);
     const float g = ex * fc, dg = ex * (dfc - 2.f * k.ang_eta * dn * fc);
+    const float gi = g * ir;
     float s[4], v[4][3];
     contract(tt + e * 80 + ch * 20, ux, uy, uz, s, v);
+    float su = 0.f, wx = 0.f, wy = 0.f, wz = 0.f;
+#pragma unroll
     for (int l = 0; l < 4; ++l) {
       const float cf = 2.f * gr[ML_RAD + ch * ML_L + l];
-      const float vu = v[l][0] * ux + v[l][1] * uy + v[l][2] * uz;
-      const float a1 = cf * dg * s[l], a2 = cf * g * ir;
-      ax += a1 * ux + a2 * (v[l][0] - vu * ux);
-      ay += a1 * uy + a2 * (v[l][1] - vu * uy);
-      az += a1 * uz + a2 * (v[l][2] - vu * uz);
+      su += cf * s[l] * (dg - (float)l * gi);
+      wx += cf * v[l][0];
+      wy += cf * v[l][1];
+      wz += cf * v[l][2];
     }
+    ax += su * ux + gi * wx;
+    ay += su * uy + gi * wy;
+    az += su * uz + gi * wz;
   }
   *vx = ax;
   *vy = ay;
