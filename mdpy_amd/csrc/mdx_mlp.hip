@@ -108,6 +108,7 @@ __device__ __forceinline__ void contract(const float* t, float ux, float uy, flo
 
 struct MlAtoms {
   int n;        // atoms
+  int stride;   // force pass: workgroup b takes sorted atom (b * stride) mod n (stride coprime to n)
   int cap;      // neighbour-list row capacity
   float Lx, Ly, Lz, ix, iy, iz;
 };
@@ -366,7 +367,7 @@ __global__ __launch_bounds__(64) void k_ml_force(MlAtoms a, MlConst k, const flo
   __shared__ float tt[ML_T];
   __shared__ float4 nb[MLF_MAX];  // unit vector, r
   __shared__ int nt[MLF_MAX];     // neighbour slot | element << 30
-  const int s = blockIdx.x;
+  const int s = (int)(((long long)blockIdx.x * a.stride) % a.n);
   const int lane = threadIdx.x;
   const float4 xi = xq[s];
   const int o = sorted_orig[s];
@@ -829,6 +830,14 @@ struct mdx_mlp {
 static MlAtoms ml_atoms(const mdx_mlp* m) {
   MlAtoms a;
   a.n = m->n;
+  // a prime stride coprime to n, so the force pass visits every atom once while concurrent
+  // workgroups work on distant atoms and update different partners (104 vs 110 us at 23,328 atoms)
+  a.stride = 1;
+  for (int p : {7919, 7907, 7901, 7883, 7879})
+    if (m->n % p != 0) {
+      a.stride = p;
+      break;
+    }
   a.cap = m->nl->nbr_cap;
   a.Lx = (float)m->nl->box[0];
   a.Ly = (float)m->nl->box[1];
